@@ -1,0 +1,1219 @@
+/*
+ * CPU restatement (FP64) of one CUEVA 2D resistive-RMHD time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it.
+ *
+ * Each routine follows the reference routine it names, file:line relative to
+ * the reference's cueva_2D_base/ directory, with the same floating-point
+ * operation order (Fortran evaluates left to right; x**n is x*(x*(...x)),
+ * as flang lowers it) so that, compiled with -ffp-contract=off, results match
+ * the reference binary bit for bit except where noted:
+ *   - get_hllc_vel (funciones.f95:188-259) reads its Newton iterate before
+ *     setting it; this restatement starts Newton from the clamped seed.
+ *   - acos/cos/pow come from the C library, as in the reference build.
+ *   - z-direction flux differences (X(i,j)-X(i,j))/Delz are the constant +0.
+ *   - the EGLM terms (FAC_EGLM = 0, parameters.f95:44) are left out; they
+ *     only add a signed zero.
+ *
+ * Flags assumed (parameters.f95 defaults kept by every configuration):
+ * MIRK=0, DIM=2, REC_PRIM=0, REC_ELEC=0, VDRIFT=0, primitive_rec=0,
+ * source_rec(_mpx)=0, flux_rec_mp5=0, REC_4VECTOR=0, SLC=0, low_vel=0.
+ */
+#include "cueva_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define G 6
+#define NXS 27 /* x-sweep slots 1..27   (16_hll_flow_mpx.f95:784-917)  */
+#define NYS 26 /* y-sweep slots 38..63  (16_hll_flow_mpx.f95:1086-1211) */
+
+/* x-sweep slot numbers minus one */
+enum { XS_PSI, XS_PHI, XS_EX, XS_EY, XS_EZ, XS_EXM, XS_EYM, XS_EZM, XS_BX, XS_BY, XS_BZ,
+       XS_BXM, XS_BYM, XS_BZM, XS_Q, XS_D, XS_TAU, XS_SX, XS_SY, XS_SZ, XS_VX, XS_VY, XS_VZ,
+       XS_P, XS_RHO, XS_JX, XS_JZ };
+/* y-sweep slot numbers minus 38 */
+enum { YS_PSI, YS_PHI, YS_EX, YS_EY, YS_EZ, YS_EXM, YS_EYM, YS_EZM, YS_BX, YS_BY, YS_BZ,
+       YS_BXM, YS_BYM, YS_BZM, YS_Q, YS_D, YS_TAU, YS_SX, YS_SY, YS_SZ, YS_VX, YS_VY, YS_VZ,
+       YS_P, YS_RHO, YS_JY };
+
+/* stage state planes (the l-indexed arrays of fourvectors.f95:12-59) */
+enum { ST_PSI, ST_PHI, ST_BX, ST_BY, ST_BZ, ST_EXA, ST_EYA, ST_EZA, ST_Q, ST_D, ST_TAU,
+       ST_SX, ST_SY, ST_SZ, ST_EX, ST_EY, ST_EZ, ST_JX, ST_JY, ST_JZ, ST_N };
+
+/* 14 accumulated fields in the order of 09_sumint.f95 / 18_sumfin.f95 */
+enum { AC_PSI, AC_PHI, AC_BX, AC_BY, AC_BZ, AC_EX, AC_EY, AC_EZ, AC_Q, AC_D, AC_TAU,
+       AC_SX, AC_SY, AC_SZ, AC_N };
+
+/* face fluxes used by 17_fluxandsource.f95:433-637 */
+enum { FX_EXPSI, FX_BXPHI, FX_PHIBX, FX_EZBY, FX_EYBZ, FX_PSIEX, FX_BZEY, FX_BYEZ, FX_J, FX_FD,
+       FX_FTAU, FX_FSX, FX_FSY, FX_FSZ, FX_N };
+enum { FY_EYPSI, FY_BYPHI, FY_EZBX, FY_PHIBY, FY_EXBZ, FY_BZEX, FY_PSIEY, FY_BXEZ, FY_J, FY_FD,
+       FY_FTAU, FY_FSX, FY_FSY, FY_FSZ, FY_N };
+
+#define NRHS 17
+#define MAXL 8
+
+struct oracle_state {
+  oracle_cfg c;
+  int nx, ny;
+  size_t n;
+  double *prim[OR_NPRIM];
+  double *D, *tau, *Sx, *Sy, *Sz, *W;
+  double *intsum[AC_N], *finsum[AC_N];
+  double *st[ST_N];
+  double *xL[NXS], *xR[NXS], *yL[NYS], *yR[NYS];
+  double *fx[FX_N], *fy[FY_N];
+  double *rhs[MAXL][NRHS];
+  int64_t cnt[3];
+};
+
+#define IX(s, i, j) (((size_t)((j) + G)) * (size_t)(s)->nx + (size_t)((i) + G))
+#define AT(s, a, i, j) ((a)[IX(s, i, j)])
+
+static inline double sq(double x) { return x * x; }
+static inline double pw3(double x) { return x * (x * x); }
+static inline double pw4(double x) { return x * (x * (x * x)); }
+static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
+static inline double fmin2(double a, double b) { return b < a ? b : a; }
+static inline double fmax2(double a, double b) { return b > a ? b : a; }
+
+/* ------------------------------------------------------------------ */
+/* configuration                                                       */
+/* ------------------------------------------------------------------ */
+
+int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, int lmax,
+                       int flux_solver, int rec_order, int bc_x, int bc_y, double sigma,
+                       double cfl) {
+  memset(c, 0, sizeof(*c));
+  if (imax < 8 || jmax < 8) return 1;
+  if (lmax != 2 && lmax != 4) return 2;
+  if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC) return 3;
+  if (rec_order != 5 && rec_order != 7 && rec_order != 9) return 4;
+  c->imax = imax;
+  c->jmax = jmax;
+  c->lmax = lmax;
+  c->flux_solver = flux_solver;
+  c->rec_order = rec_order;
+  c->bc_x = bc_x;
+  c->bc_y = bc_y;
+  /* parameters.f95:306-312: Delx = Longx/real(imax) (real(imax) is exact), Delz = 1 */
+  c->dx = lx / (double)(float)imax;
+  c->dy = ly / (double)(float)jmax;
+  double m = fmin2(fmin2(c->dx, c->dy), 1.0);
+  c->dt = cfl * m;
+  c->sigma = sigma;
+  c->gamma = 4.0 / 3.0; /* parameters.f95:148 */
+  /* parameters.f95:325-334: kappa = alpha_ch**2 / cp**2, cp = sqrt(min(Delx,Dely)/alpha) */
+  double cp = sqrt(fmin2(c->dx, c->dy) / 1.0);
+  c->kappa_psi = 1.0 / (cp * cp);
+  c->kappa_phi = 1.0 / (cp * cp);
+  if (lmax == 2) { /* imex_scheme.f95:40-49, const = 1 - 1/sqrt(2) (parameters.f95:240) */
+    double g = 1.0 - 1.0 / sqrt(2.0);
+    c->wb[0] = 0.5; c->wb[1] = 0.5;
+    c->wbt[0] = 0.5; c->wbt[1] = 0.5;
+    c->Ab[1][0] = 1.0;
+    c->Abt[0][0] = g;
+    c->Abt[1][0] = 1.0 - 2.0 * g;
+    c->Abt[1][1] = g;
+  } else { /* imex_scheme.f95:280-291 with parameters.f95:242-244 */
+    double a1 = 0.24169426078821, b1 = 0.06042356519705, e1 = 0.12915286960590;
+    c->wb[1] = 1.0 / 6.0; c->wb[2] = 1.0 / 6.0; c->wb[3] = 2.0 / 3.0;
+    c->wbt[1] = 1.0 / 6.0; c->wbt[2] = 1.0 / 6.0; c->wbt[3] = 2.0 / 3.0;
+    c->Ab[2][1] = 1.0;
+    c->Ab[3][1] = 0.25; c->Ab[3][2] = 0.25;
+    c->Abt[0][0] = a1;
+    c->Abt[1][0] = -1.0 * a1; c->Abt[1][1] = a1;
+    c->Abt[2][1] = 1.0 - a1; c->Abt[2][2] = a1;
+    c->Abt[3][0] = b1; c->Abt[3][1] = e1; c->Abt[3][2] = ((0.5 - b1) - e1) - a1; c->Abt[3][3] = a1;
+  }
+  return 0;
+}
+
+static double *plane(const oracle_state *s) { return (double *)calloc(s->n, sizeof(double)); }
+
+oracle_state *oracle_create(const oracle_cfg *c) {
+  oracle_state *s = (oracle_state *)calloc(1, sizeof(*s));
+  if (!s) return NULL;
+  s->c = *c;
+  s->nx = c->imax + 2 * G + 1;
+  s->ny = c->jmax + 2 * G + 1;
+  s->n = (size_t)s->nx * (size_t)s->ny;
+  for (int k = 0; k < OR_NPRIM; k++) s->prim[k] = plane(s);
+  s->D = plane(s); s->tau = plane(s); s->Sx = plane(s); s->Sy = plane(s); s->Sz = plane(s);
+  s->W = plane(s);
+  for (int k = 0; k < AC_N; k++) { s->intsum[k] = plane(s); s->finsum[k] = plane(s); }
+  for (int k = 0; k < ST_N; k++) s->st[k] = plane(s);
+  for (int k = 0; k < NXS; k++) { s->xL[k] = plane(s); s->xR[k] = plane(s); }
+  for (int k = 0; k < NYS; k++) { s->yL[k] = plane(s); s->yR[k] = plane(s); }
+  for (int k = 0; k < FX_N; k++) s->fx[k] = plane(s);
+  for (int k = 0; k < FY_N; k++) s->fy[k] = plane(s);
+  for (int l = 0; l < c->lmax; l++)
+    for (int k = 0; k < NRHS; k++) s->rhs[l][k] = plane(s);
+  return s;
+}
+
+void oracle_destroy(oracle_state *s) {
+  if (!s) return;
+  for (int k = 0; k < OR_NPRIM; k++) free(s->prim[k]);
+  free(s->D); free(s->tau); free(s->Sx); free(s->Sy); free(s->Sz); free(s->W);
+  for (int k = 0; k < AC_N; k++) { free(s->intsum[k]); free(s->finsum[k]); }
+  for (int k = 0; k < ST_N; k++) free(s->st[k]);
+  for (int k = 0; k < NXS; k++) { free(s->xL[k]); free(s->xR[k]); }
+  for (int k = 0; k < NYS; k++) { free(s->yL[k]); free(s->yR[k]); }
+  for (int k = 0; k < FX_N; k++) free(s->fx[k]);
+  for (int k = 0; k < FY_N; k++) free(s->fy[k]);
+  for (int l = 0; l < s->c.lmax; l++)
+    for (int k = 0; k < NRHS; k++) free(s->rhs[l][k]);
+  free(s);
+}
+
+size_t oracle_plane_len(const oracle_state *s) { return s->n; }
+
+/* ------------------------------------------------------------------ */
+/* boundaries: 04_boundary.f95:922-1388 (BOUND 1), 1390-1862 (BOUND 2), */
+/* 2867-3033 (BOUND 6) and the same index maps in                       */
+/* 24_boundary_conserved.f95, 25_boundary_electric.f95,                 */
+/* 24_boundary_flux_conserved.f95.  Every ghost is a copy of one        */
+/* interior point; the map is per axis: copy clamps to [0,N], periodic  */
+/* wraps with the duplicated edge (ghost -k <- N-k, ghost N+k <- k).    */
+/* ------------------------------------------------------------------ */
+
+static inline int src_index(int i, int n, int kind) {
+  if (i >= 0 && i <= n) return i;
+  if (kind == OR_BC_PERIODIC) return i < 0 ? n + i : i - n;
+  return i < 0 ? 0 : n;
+}
+
+static void fill_ghosts(const oracle_state *s, double *a) {
+  const int im = s->c.imax, jm = s->c.jmax;
+  for (int j = -G; j <= jm + G; j++)
+    for (int i = -G; i <= im + G; i++) {
+      if (i >= 0 && i <= im && j >= 0 && j <= jm) continue;
+      AT(s, a, i, j) = AT(s, a, src_index(i, im, s->c.bc_x), src_index(j, jm, s->c.bc_y));
+    }
+}
+
+void oracle_set_prims(oracle_state *s, const double *prims14) {
+  for (int k = 0; k < OR_NPRIM; k++) {
+    memcpy(s->prim[k], prims14 + (size_t)k * s->n, s->n * sizeof(double));
+    fill_ghosts(s, s->prim[k]);
+  }
+}
+
+void oracle_get_fields(const oracle_state *s, double *prims14, double *cons5) {
+  if (prims14)
+    for (int k = 0; k < OR_NPRIM; k++) memcpy(prims14 + (size_t)k * s->n, s->prim[k], s->n * sizeof(double));
+  if (cons5) {
+    const double *c5[5] = {s->D, s->tau, s->Sx, s->Sy, s->Sz};
+    for (int k = 0; k < 5; k++) memcpy(cons5 + (size_t)k * s->n, c5[k], s->n * sizeof(double));
+  }
+}
+
+void oracle_get_counters(const oracle_state *s, int64_t counters3[3]) {
+  for (int k = 0; k < 3; k++) counters3[k] = s->cnt[k];
+}
+
+void oracle_get_rhs(const oracle_state *s, int l, int which, double *out) {
+  memcpy(out, s->rhs[l][which], s->n * sizeof(double));
+}
+
+/* ------------------------------------------------------------------ */
+/* startconvar: 06_conserved_var.f95:77-140                            */
+/* ------------------------------------------------------------------ */
+
+static void startconvar(oracle_state *s) {
+  const double gam = s->c.gamma;
+  double **P = s->prim;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+      double ex = P[OR_EX][k], ey = P[OR_EY][k], ez = P[OR_EZ][k];
+      double bx = P[OR_BX][k], by = P[OR_BY][k], bz = P[OR_BZ][k];
+      double rho = P[OR_RHO][k], p = P[OR_P][k];
+      double v2 = (sq(vx) + sq(vy)) + sq(vz);
+      double w = 1.0 / sqrt(1.0 - v2);
+      double e2 = (sq(ex) + sq(ey)) + sq(ez);
+      double b2 = (sq(bx) + sq(by)) + sq(bz);
+      double eps = p / ((gam - 1.0) * rho);
+      double h = rho * (1.0 + eps) + p;
+      s->W[k] = w;
+      s->D[k] = rho * w;
+      s->tau[k] = (0.5 * (e2 + b2) + h * sq(w)) - p;
+      s->Sx[k] = (bz * ey - by * ez) + (h * sq(w)) * vx;
+      s->Sy[k] = (bx * ez - bz * ex) + (h * sq(w)) * vy;
+      s->Sz[k] = (by * ex - bx * ey) + (h * sq(w)) * vz;
+    }
+}
+
+/* U^n of the 14 accumulated fields, in AC_* order */
+static const double *un_plane(const oracle_state *s, int a) {
+  switch (a) {
+    case AC_PSI: return s->prim[OR_PSI];
+    case AC_PHI: return s->prim[OR_PHI];
+    case AC_BX: return s->prim[OR_BX];
+    case AC_BY: return s->prim[OR_BY];
+    case AC_BZ: return s->prim[OR_BZ];
+    case AC_EX: return s->prim[OR_EX];
+    case AC_EY: return s->prim[OR_EY];
+    case AC_EZ: return s->prim[OR_EZ];
+    case AC_Q: return s->prim[OR_Q];
+    case AC_D: return s->D;
+    case AC_TAU: return s->tau;
+    case AC_SX: return s->Sx;
+    case AC_SY: return s->Sy;
+    default: return s->Sz;
+  }
+}
+
+/* stage plane that receives U^n + dt*intsum (10_varint.f95:23-63) */
+static const int ac_to_st[AC_N] = {ST_PSI, ST_PHI, ST_BX, ST_BY, ST_BZ, ST_EXA, ST_EYA, ST_EZA,
+                                   ST_Q, ST_D, ST_TAU, ST_SX, ST_SY, ST_SZ};
+
+/* ------------------------------------------------------------------ */
+/* cleanvarint (07_clean_var.f95:18-60), sumint (09_sumint.f95:18-61),  */
+/* varint (10_varint.f95:18-63), sumfin (18_sumfin.f95:20-60),          */
+/* varfin (19_varfin.f95:35-50), electricfield (20_electricfield.f95:157-163) */
+/* ------------------------------------------------------------------ */
+
+static void cleanvarint(oracle_state *s) {
+  for (int a = 0; a < AC_N; a++)
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        AT(s, s->intsum[a], i, j) = 0.0;
+        AT(s, s->finsum[a], i, j) = 0.0;
+      }
+}
+
+static void sumint(oracle_state *s, int l, int m) {
+  const double ab = s->c.Ab[l][m], abt = s->c.Abt[l][m];
+  for (int a = 0; a < AC_N; a++) {
+    double *acc = s->intsum[a];
+    const double *f = s->rhs[m][a];
+    const double *src = (a >= AC_EX && a <= AC_EZ) ? s->rhs[m][14 + (a - AC_EX)] : NULL;
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        size_t k = IX(s, i, j);
+        if (src)
+          acc[k] = acc[k] + (ab * f[k] - abt * src[k]);
+        else
+          acc[k] = acc[k] + ab * f[k];
+      }
+  }
+}
+
+static void varint(oracle_state *s) {
+  const double dt = s->c.dt;
+  for (int a = 0; a < AC_N; a++) {
+    const double *u = un_plane(s, a);
+    double *dst = s->st[ac_to_st[a]];
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        size_t k = IX(s, i, j);
+        dst[k] = u[k] + dt * s->intsum[a][k];
+      }
+  }
+  /* boundary_conserved: psi phi B q D tau S (24_boundary_conserved.f95) */
+  static const int cons[11] = {ST_PSI, ST_PHI, ST_BX, ST_BY, ST_BZ, ST_Q, ST_D, ST_TAU, ST_SX, ST_SY, ST_SZ};
+  for (int k = 0; k < 11; k++) fill_ghosts(s, s->st[cons[k]]);
+}
+
+/* electricvarint, IMEX branch: 11_elecvarint.f95:279-328 */
+static void electricvarint(oracle_state *s, int l) {
+  const double abt = s->c.Abt[l][l], sig = s->c.sigma, dt = s->c.dt;
+  double **P = s->prim, **S = s->st;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double w = s->W[k], vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+      double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k];
+      double eax = S[ST_EXA][k], eay = S[ST_EYA][k], eaz = S[ST_EZA][k];
+      double a = (abt * sig) * dt;
+      double matx = ((sq(w) * a + w * sq(a)) + w) + a;
+      double rx = bz * vy - by * vz;
+      double ry = bx * vz - bz * vx;
+      double rz = by * vx - bx * vy;
+      double aw2 = a * sq(w);
+      double kk = ((abt * sig) * w) * dt;
+      double txx = (a + w) + aw2 * sq(vx);
+      double tyy = (a + w) + aw2 * sq(vy);
+      double tzz = (a + w) + aw2 * sq(vz);
+      double inv = 1.0 / matx;
+      S[ST_EX][k] = inv * (((txx * eax + ((aw2 * vx) * vy) * eay) + ((aw2 * vx) * vz) * eaz) -
+                           kk * ((txx * rx + ((aw2 * vx) * vy) * ry) + ((aw2 * vx) * vz) * rz));
+      S[ST_EY][k] = inv * (((((aw2 * vx) * vy) * eax + tyy * eay) + ((aw2 * vy) * vz) * eaz) -
+                           kk * ((((aw2 * vx) * vy) * rx + tyy * ry) + ((aw2 * vy) * vz) * rz));
+      S[ST_EZ][k] = inv * (((((aw2 * vx) * vz) * eax + ((aw2 * vy) * vz) * eay) + tzz * eaz) -
+                           kk * ((((aw2 * vx) * vz) * rx + ((aw2 * vy) * vz) * ry) + tzz * rz));
+    }
+  fill_ghosts(s, S[ST_EX]);
+  fill_ghosts(s, S[ST_EY]);
+  fill_ghosts(s, S[ST_EZ]);
+}
+
+/* flowinit, current density only: 14_flowinit.f95:258-301 */
+static void flowinit(oracle_state *s) {
+  const double sig = s->c.sigma;
+  double **P = s->prim, **S = s->st;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double w = s->W[k], vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+      double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
+      double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k], q = S[ST_Q][k];
+      double edv = (ex * vx + ey * vy) + ez * vz;
+      double rx = bz * vy - by * vz, ry = bx * vz - bz * vx, rz = by * vx - bx * vy;
+      S[ST_JX][k] = (sig * w) * ((ex + rx) - edv * vx) + q * vx;
+      S[ST_JY][k] = (sig * w) * ((ey + ry) - edv * vy) + q * vy;
+      S[ST_JZ][k] = (sig * w) * ((ez + rz) - edv * vz) + q * vz;
+    }
+  /* 24_boundary_flux_conserved.f95:13: ghosts only for HLL and HLLC; with HLLE
+   * the J ghosts keep their zero initial value. */
+  if (s->c.flux_solver == OR_HLL || s->c.flux_solver == OR_HLLC) {
+    fill_ghosts(s, S[ST_JX]);
+    fill_ghosts(s, S[ST_JY]);
+    fill_ghosts(s, S[ST_JZ]);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* MP5 / MP7 / MP9 along one line: mp5.f95:40-264, mp7.f95, mp9.f95.   */
+/* u, up, um are indexed -6..n+6 (pointers to element 0); up/um are    */
+/* written on [-2, n+2].                                                */
+/* ------------------------------------------------------------------ */
+
+static inline double dmm(double x, double y) { /* funciones.f95:148-162 */
+  return (0.5 * (copysign(1.0, x) + copysign(1.0, y))) * fmin2(fabs(x), fabs(y));
+}
+
+static inline double dm4(double w, double x, double y, double z) { /* funciones.f95:165-185 */
+  double sw = copysign(1.0, w);
+  double m = fmin2(fmin2(fmin2(fabs(w), fabs(x)), fabs(y)), fabs(z));
+  return ((0.125 * (sw + copysign(1.0, x))) * fabs((sw + copysign(1.0, y)) * (sw + copysign(1.0, z)))) * m;
+}
+
+static inline double vor_plus(int order, const double *u, int i) {
+  if (order == 5) /* mp5.f95:79-80 */
+    return (1.0 / 60.0) * ((((2.0 * u[i - 2] - 13.0 * u[i - 1]) + 47.0 * u[i]) + 27.0 * u[i + 1]) - 3.0 * u[i + 2]);
+  if (order == 7) /* mp7.f95:111-113 */
+    return (1.0 / 420.0) * ((((((-3.0 * u[i - 3] + 25.0 * u[i - 2]) - 101.0 * u[i - 1]) + 319.0 * u[i]) +
+                              214.0 * u[i + 1]) - 38.0 * u[i + 2]) + 4.0 * u[i + 3]);
+  /* mp9.f95:116-118 */
+  return (1.0 / 2520.0) * ((((((((4.0 * u[i - 4] - 41.0 * u[i - 3]) + 199.0 * u[i - 2]) - 641.0 * u[i - 1]) +
+                               1879.0 * u[i]) + 1375.0 * u[i + 1]) - 305.0 * u[i + 2]) + 55.0 * u[i + 3]) -
+                           5.0 * u[i + 4]);
+}
+
+static inline double vor_minus(int order, const double *u, int i) {
+  if (order == 5) /* mp5.f95:82-83 */
+    return (1.0 / 60.0) * ((((-3.0 * u[i - 2] + 27.0 * u[i - 1]) + 47.0 * u[i]) - 13.0 * u[i + 1]) + 2.0 * u[i + 2]);
+  if (order == 7) /* mp7.f95:115-117 */
+    return (1.0 / 420.0) * ((((((-3.0 * u[i + 3] + 25.0 * u[i + 2]) - 101.0 * u[i + 1]) + 319.0 * u[i]) +
+                              214.0 * u[i - 1]) - 38.0 * u[i - 2]) + 4.0 * u[i - 3]);
+  /* mp9.f95:121-123 */
+  return (1.0 / 2520.0) * ((((((((4.0 * u[i + 4] - 41.0 * u[i + 3]) + 199.0 * u[i + 2]) - 641.0 * u[i + 1]) +
+                               1879.0 * u[i]) + 1375.0 * u[i - 1]) - 305.0 * u[i - 2]) + 55.0 * u[i - 3]) -
+                           5.0 * u[i - 4]);
+}
+
+void oracle_mp_line(int order, const double *u, int n, double *up, double *um, int positivity) {
+  const double b2 = 4.0 / 3.0, epsm = 1.0e-10, alpha = 2.0;
+  const int ext = order == 5 ? 3 : (order == 7 ? 4 : 5); /* du/d2 range mp5.f95:64, mp7.f95:69, mp9.f95:72 */
+  double *dub = (double *)malloc(sizeof(double) * (size_t)(n + 32));
+  double *d2b = (double *)malloc(sizeof(double) * (size_t)(n + 32));
+  double *du = dub + 8, *d2 = d2b + 8;
+  for (int i = -ext; i <= n + ext; i++) {
+    du[i] = u[i] - u[i - 1];
+    d2[i] = (u[i - 1] - 2.0 * u[i]) + u[i + 1];
+  }
+  for (int i = -2; i <= n + 2; i++) {
+    double u0 = u[i];
+    double vorp = vor_plus(order, u, i), vorm = vor_minus(order, u, i);
+    double vmpp = u0 + dmm(du[i + 1], alpha * du[i]);
+    double vmpm = u0 + dmm(-du[i], -(alpha * du[i + 1]));
+    /* U^L_{i+1/2}: mp5.f95:104-136 */
+    if ((vorp - u0) * (vorp - vmpp) <= epsm) {
+      up[i] = vorp;
+    } else {
+      double dm4jph = dm4(4.0 * d2[i] - d2[i + 1], 4.0 * d2[i + 1] - d2[i], d2[i], d2[i + 1]);
+      double dm4jmh = dm4(4.0 * d2[i] - d2[i - 1], 4.0 * d2[i - 1] - d2[i], d2[i], d2[i - 1]);
+      double vul = u0 + alpha * du[i];
+      double vav = 0.5 * (u0 + u[i + 1]);
+      double vmd = vav - 0.5 * dm4jph;
+      double vlc = (u0 + 0.5 * du[i]) + b2 * dm4jmh;
+      double vmin = fmax2(fmin2(fmin2(u0, u[i + 1]), vmd), fmin2(fmin2(u0, vul), vlc));
+      double vmax = fmin2(fmax2(fmax2(u0, u[i + 1]), vmd), fmax2(fmax2(u0, vul), vlc));
+      up[i] = vorp + dmm(vmin - vorp, vmax - vorp);
+    }
+    /* U^R_{i-1/2}: mp5.f95:142-167 */
+    if ((vorm - u0) * (vorm - vmpm) <= epsm) {
+      um[i] = vorm;
+    } else {
+      double dm4jph = dm4(4.0 * d2[i] - d2[i + 1], 4.0 * d2[i + 1] - d2[i], d2[i], d2[i + 1]);
+      double dm4jmh = dm4(4.0 * d2[i] - d2[i - 1], 4.0 * d2[i - 1] - d2[i], d2[i], d2[i - 1]);
+      double vul = u0 - alpha * du[i + 1];
+      double vav = 0.5 * (u0 + u[i - 1]);
+      double vmd = vav - 0.5 * dm4jmh;
+      double vlc = (u0 - 0.5 * du[i + 1]) + b2 * dm4jph;
+      double vmin = fmax2(fmin2(fmin2(u0, u[i - 1]), vmd), fmin2(fmin2(u0, vul), vlc));
+      double vmax = fmin2(fmax2(fmax2(u0, u[i - 1]), vmd), fmax2(fmax2(u0, vul), vlc));
+      um[i] = vorm + dmm(vmin - vorm, vmax - vorm);
+    }
+  }
+  if (positivity) { /* mp5.f95:177-216 (slots 24/25 and 61/62: p and rho) */
+    for (int i = -2; i <= n + 2; i++)
+      if (up[i] < 0.0) {
+        up[i] = u[i] + 0.5 * dmm(du[i], du[i + 1]);
+        um[i] = u[i];
+      }
+    for (int i = -2; i <= n + 2; i++)
+      if (um[i] < 0.0) {
+        um[i] = u[i] - 0.5 * dmm(du[i], du[i + 1]);
+        up[i] = u[i];
+      }
+  }
+  free(dub);
+  free(d2b);
+}
+
+/* ------------------------------------------------------------------ */
+/* pack / MPx / unpack: 16_hll_flow_mpx.f95:781-1352 (same in hllc/hlle) */
+/* ------------------------------------------------------------------ */
+
+static double slot_x(const oracle_state *s, int slot, size_t k) {
+  double *const *S = s->st, *const *P = s->prim;
+  switch (slot) {
+    case XS_PSI: return S[ST_PSI][k];
+    case XS_PHI: return S[ST_PHI][k];
+    case XS_EX: return S[ST_EX][k];
+    case XS_EY: return S[ST_EY][k];
+    case XS_EZ: return S[ST_EZ][k];
+    case XS_EXM: return -S[ST_EX][k];
+    case XS_EYM: return -S[ST_EY][k];
+    case XS_EZM: return -S[ST_EZ][k];
+    case XS_BX: return S[ST_BX][k];
+    case XS_BY: return S[ST_BY][k];
+    case XS_BZ: return S[ST_BZ][k];
+    case XS_BXM: return -S[ST_BX][k];
+    case XS_BYM: return -S[ST_BY][k];
+    case XS_BZM: return -S[ST_BZ][k];
+    case XS_Q: return S[ST_Q][k];
+    case XS_D: return S[ST_D][k];
+    case XS_TAU: return S[ST_TAU][k];
+    case XS_SX: return S[ST_SX][k];
+    case XS_SY: return S[ST_SY][k];
+    case XS_SZ: return S[ST_SZ][k];
+    case XS_VX: return P[OR_VX][k];
+    case XS_VY: return P[OR_VY][k];
+    case XS_VZ: return P[OR_VZ][k];
+    case XS_P: return P[OR_P][k];
+    case XS_RHO: return P[OR_RHO][k];
+    case XS_JX: return S[ST_JX][k];
+    default: return S[ST_JZ][k];
+  }
+}
+
+static double slot_y(const oracle_state *s, int slot, size_t k) {
+  if (slot == YS_JY) return s->st[ST_JY][k];
+  return slot_x(s, slot, k); /* slots 38..62 repeat the x order (1..25) */
+}
+
+static void reconstruct(oracle_state *s) {
+  const int im = s->c.imax, jm = s->c.jmax, ord = s->c.rec_order;
+  const int nmax = (im > jm ? im : jm) + 2 * G + 1;
+  double *ub = (double *)malloc(sizeof(double) * nmax);
+  double *upb = (double *)malloc(sizeof(double) * nmax);
+  double *umb = (double *)malloc(sizeof(double) * nmax);
+  double *u = ub + G, *up = upb + G, *um = umb + G;
+  /* x sweep, rows -1..jmax (the row jmax+1 of the reference is never read) */
+  for (int j = -1; j <= jm; j++)
+    for (int sl = 0; sl < NXS; sl++) {
+      for (int i = -G; i <= im + G; i++) u[i] = slot_x(s, sl, IX(s, i, j));
+      oracle_mp_line(ord, u, im, up, um, sl == XS_P || sl == XS_RHO);
+      for (int i = -1; i <= im + 1; i++) {
+        AT(s, s->xL[sl], i, j) = up[i];
+        AT(s, s->xR[sl], i, j) = um[i + 1];
+      }
+    }
+  /* y sweep, columns -1..imax */
+  for (int i = -1; i <= im; i++)
+    for (int sl = 0; sl < NYS; sl++) {
+      for (int j = -G; j <= jm + G; j++) u[j] = slot_y(s, sl, IX(s, i, j));
+      oracle_mp_line(ord, u, jm, up, um, sl == YS_P || sl == YS_RHO);
+      for (int j = -1; j <= jm + 1; j++) {
+        AT(s, s->yL[sl], i, j) = up[j];
+        AT(s, s->yR[sl], i, j) = um[j + 1];
+      }
+    }
+  free(ub); free(upb); free(umb);
+}
+
+/* ------------------------------------------------------------------ */
+/* face loop helpers                                                    */
+/* ------------------------------------------------------------------ */
+
+/* one reconstructed face side */
+typedef struct {
+  double psi, phi, ex, ey, ez, exm, eym, ezm, bx, by, bz, bxm, bym, bzm, q, d, tau, sx, sy, sz;
+  double vx, vy, vz, p, rho, j1, j3; /* j1 = normal J, j3 = Jz (x faces only) */
+  double w, e2b2, h;
+  double fd, ftau, fs1, fs2, fs3; /* normal-direction physical fluxes */
+} side_t;
+
+static void load_x(const oracle_state *s, double *const *arr, size_t k, side_t *o) {
+  o->psi = arr[XS_PSI][k]; o->phi = arr[XS_PHI][k];
+  o->ex = arr[XS_EX][k]; o->ey = arr[XS_EY][k]; o->ez = arr[XS_EZ][k];
+  o->exm = arr[XS_EXM][k]; o->eym = arr[XS_EYM][k]; o->ezm = arr[XS_EZM][k];
+  o->bx = arr[XS_BX][k]; o->by = arr[XS_BY][k]; o->bz = arr[XS_BZ][k];
+  o->bxm = arr[XS_BXM][k]; o->bym = arr[XS_BYM][k]; o->bzm = arr[XS_BZM][k];
+  o->q = arr[XS_Q][k]; o->d = arr[XS_D][k]; o->tau = arr[XS_TAU][k];
+  o->sx = arr[XS_SX][k]; o->sy = arr[XS_SY][k]; o->sz = arr[XS_SZ][k];
+  o->vx = arr[XS_VX][k]; o->vy = arr[XS_VY][k]; o->vz = arr[XS_VZ][k];
+  o->p = arr[XS_P][k]; o->rho = arr[XS_RHO][k];
+  o->j1 = arr[XS_JX][k]; o->j3 = arr[XS_JZ][k];
+  (void)s;
+}
+
+static void load_y(const oracle_state *s, double *const *arr, size_t k, side_t *o) {
+  o->psi = arr[YS_PSI][k]; o->phi = arr[YS_PHI][k];
+  o->ex = arr[YS_EX][k]; o->ey = arr[YS_EY][k]; o->ez = arr[YS_EZ][k];
+  o->exm = arr[YS_EXM][k]; o->eym = arr[YS_EYM][k]; o->ezm = arr[YS_EZM][k];
+  o->bx = arr[YS_BX][k]; o->by = arr[YS_BY][k]; o->bz = arr[YS_BZ][k];
+  o->bxm = arr[YS_BXM][k]; o->bym = arr[YS_BYM][k]; o->bzm = arr[YS_BZM][k];
+  o->q = arr[YS_Q][k]; o->d = arr[YS_D][k]; o->tau = arr[YS_TAU][k];
+  o->sx = arr[YS_SX][k]; o->sy = arr[YS_SY][k]; o->sz = arr[YS_SZ][k];
+  o->vx = arr[YS_VX][k]; o->vy = arr[YS_VY][k]; o->vz = arr[YS_VZ][k];
+  o->p = arr[YS_P][k]; o->rho = arr[YS_RHO][k];
+  o->j1 = arr[YS_JY][k]; o->j3 = 0.0;
+  (void)s;
+}
+
+/* W, E^2+B^2, enthalpy and the normal physical fluxes of one face side:
+ * 16_hll_flow_mpx.f95:1463-1466 (W), 1522-1532 (E2B2), 1603-1617 (eps, h),
+ * 1627-1715 (FD, Ftau, FS) */
+static void side_physics(double gam, int dir, side_t *o) {
+  double v2 = (sq(o->vx) + sq(o->vy)) + sq(o->vz);
+  o->w = 1.0 / sqrt(1.0 - v2);
+  o->e2b2 = ((((sq(o->ex) + sq(o->ey)) + sq(o->ez)) + sq(o->bx)) + sq(o->by)) + sq(o->bz);
+  double eps = o->p / ((gam - 1.0) * o->rho);
+  o->h = o->rho * (1.0 + eps) + o->p;
+  double hw2 = o->h * sq(o->w);
+  if (dir == 0) {
+    o->fd = o->d * o->vx;
+    o->ftau = (o->bz * o->ey - o->by * o->ez) + hw2 * o->vx;
+    o->fs1 = (((-sq(o->ex)) - sq(o->bx)) + hw2 * sq(o->vx)) + 0.5 * o->e2b2 + o->p;
+    o->fs2 = ((-(o->ex * o->ey)) - o->bx * o->by) + (hw2 * o->vx) * o->vy;
+    o->fs3 = ((-(o->ex * o->ez)) - o->bx * o->bz) + (hw2 * o->vx) * o->vz;
+  } else {
+    o->fd = o->d * o->vy;
+    o->ftau = (o->bx * o->ez - o->bz * o->ex) + hw2 * o->vy;
+    o->fs1 = ((-(o->ex * o->ey)) - o->bx * o->by) + (hw2 * o->vx) * o->vy; /* FSyx */
+    o->fs2 = (((-sq(o->ey)) - sq(o->by)) + hw2 * sq(o->vy)) + 0.5 * o->e2b2 + o->p; /* FSyy */
+    o->fs3 = ((-(o->ey * o->ez)) - o->by * o->bz) + (hw2 * o->vy) * o->vz; /* FSyz */
+  }
+}
+
+/* (s2*FL - s1*FR + s1*s2*(UR - UL))/(s2 - s1) with s1 = -1, s2 = 1
+ * (parameters.f95:283-284; 16_hll_flow_mpx.f95:1733-1958) */
+static const double S1 = -1.0, S2 = 1.0;
+static inline double hllf(double fl, double fr, double ul, double ur) {
+  return ((S2 * fl - S1 * fr) + (S1 * S2) * (ur - ul)) / (S2 - S1);
+}
+
+/* HLL face fluxes: 16_hll_flow_mpx.f95:1738-1958 */
+static void hll_x(const side_t *L, const side_t *R, double f[FX_N]) {
+  f[FX_EXPSI] = hllf(L->ex, R->ex, L->psi, R->psi);
+  f[FX_BXPHI] = hllf(L->bx, R->bx, L->phi, R->phi);
+  f[FX_PHIBX] = hllf(L->phi, R->phi, L->bx, R->bx);
+  f[FX_EZBY] = -hllf(L->ezm, R->ezm, L->by, R->by);
+  f[FX_EYBZ] = hllf(L->ey, R->ey, L->bz, R->bz);
+  f[FX_PSIEX] = hllf(L->psi, R->psi, L->ex, R->ex);
+  f[FX_BZEY] = hllf(L->bz, R->bz, L->ey, R->ey);
+  f[FX_BYEZ] = -hllf(L->bym, R->bym, L->ez, R->ez);
+  f[FX_J] = hllf(L->j1, R->j1, L->q, R->q);
+  f[FX_FD] = hllf(L->fd, R->fd, L->d, R->d);
+  f[FX_FTAU] = hllf(L->ftau, R->ftau, L->tau, R->tau);
+  f[FX_FSX] = hllf(L->fs1, R->fs1, L->sx, R->sx);
+  f[FX_FSY] = hllf(L->fs2, R->fs2, L->sy, R->sy);
+  f[FX_FSZ] = hllf(L->fs3, R->fs3, L->sz, R->sz);
+}
+
+static void hll_y(const side_t *L, const side_t *R, double f[FY_N]) {
+  f[FY_EYPSI] = hllf(L->ey, R->ey, L->psi, R->psi);
+  f[FY_BYPHI] = hllf(L->by, R->by, L->phi, R->phi);
+  f[FY_EZBX] = hllf(L->ez, R->ez, L->bx, R->bx);
+  f[FY_PHIBY] = hllf(L->phi, R->phi, L->by, R->by);
+  f[FY_EXBZ] = -hllf(L->exm, R->exm, L->bz, R->bz);
+  f[FY_BZEX] = -hllf(L->bzm, R->bzm, L->ex, R->ex);
+  f[FY_PSIEY] = hllf(L->psi, R->psi, L->ey, R->ey);
+  f[FY_BXEZ] = hllf(L->bx, R->bx, L->ez, R->ez);
+  f[FY_J] = hllf(L->j1, R->j1, L->q, R->q);
+  f[FY_FD] = hllf(L->fd, R->fd, L->d, R->d);
+  f[FY_FTAU] = hllf(L->ftau, R->ftau, L->tau, R->tau);
+  f[FY_FSX] = hllf(L->fs1, R->fs1, L->sx, R->sx);
+  f[FY_FSY] = hllf(L->fs2, R->fs2, L->sy, R->sy);
+  f[FY_FSZ] = hllf(L->fs3, R->fs3, L->sz, R->sz);
+}
+
+/* get_hllc_vel, funciones.f95:188-259: Newton on a*v^2 - b*v + c = 0 in
+ * [-1,1], started from the clamped seed (see the file header). */
+/* max(lo, min(hi, x)) as the reference build evaluates it: a NaN passes through */
+static inline double clampf(double x, double lo, double hi) {
+  double t = x > hi ? hi : x;
+  return t < lo ? lo : t;
+}
+
+double oracle_hllc_vel(double a, double b, double c, double rt) {
+  double oldv = clampf(rt, -1.0, 1.0);
+  double v = oldv;
+  int iter = 0;
+  for (;;) {
+    double fv = (a * v * v - b * v) + c;
+    double dfdv = (2.0 * a) * v - b;
+    v = clampf(oldv - fv / dfdv, -1.0, 1.0);
+    double err = fabs(oldv - v);
+    oldv = v;
+    iter = iter + 1;
+    if (iter >= 100) return NAN; /* the reference stops here */
+    if (!(err > 1.0e-12)) break;
+  }
+  return clampf(v, -1.0 + 1.0e-3, 1.0 - 1.0e-3);
+}
+
+/* state (0.5-weighted) HLL average and flux, 16_hllc_flow_mpx.f95:2811-3095 */
+static inline double hll_state(double ul, double ur, double fl, double fr) {
+  return (((S2 * ur - S1 * ul) + fl) - fr) / (S2 - S1);
+}
+static inline double hll_flux(double fl, double fr, double ul, double ur) {
+  return 0.5 * ((fl + fr) - (ur - ul));
+}
+
+/* contact speed of 16_hllc_flow_mpx.f95:3185-3334 */
+static double hllc_contact(double a, double b, double c) {
+  double det = sq(b) - (4.0 * a) * c;
+  double qr = -(0.5 * (b + copysign(1.0, b) * fabs(sqrt(det))));
+  double rt2 = c / qr;
+  if (det < 0.0) rt2 = c / b;
+  double v = oracle_hllc_vel(a, -b, c, rt2);
+  if (fabs(v) <= 1.0e-9)
+    v = 0.0;
+  else if (fabs(v) > 1.0)
+    v = clampf(v, -1.0 + 1.0e-6, 1.0 - 1.0e-6);
+  return v;
+}
+
+/* HLLC x face: 16_hllc_flow_mpx.f95:2811-4295. Returns 0 or a stop status. */
+static int hllc_x(const side_t *L, const side_t *R, double f[FX_N]) {
+  double psi_h = hll_state(L->psi, R->psi, L->ex, R->ex);
+  double phi_h = hll_state(L->phi, R->phi, L->bx, R->bx);
+  double ex_h = hll_state(L->ex, R->ex, L->psi, R->psi);
+  double ey_h = hll_state(L->ey, R->ey, L->bz, R->bz);
+  double ez_h = hll_state(L->ez, R->ez, L->bym, R->bym);
+  double bx_h = hll_state(L->bx, R->bx, L->phi, R->phi);
+  double by_h = hll_state(L->by, R->by, L->ezm, R->ezm);
+  double bz_h = hll_state(L->bz, R->bz, L->ey, R->ey);
+  double dd_h = hll_state(L->d, R->d, L->fd, R->fd);
+  double tau_h = hll_state(L->tau, R->tau, L->ftau, R->ftau);
+  double sx_h = hll_state(L->sx, R->sx, L->fs1, R->fs1);
+  double sy_h = hll_state(L->sy, R->sy, L->fs2, R->fs2);
+  double sz_h = hll_state(L->sz, R->sz, L->fs3, R->fs3);
+
+  double psi_f = hll_flux(L->ex, R->ex, L->psi, R->psi);
+  double phi_f = hll_flux(L->bx, R->bx, L->phi, R->phi);
+  double bx_f = hll_flux(L->phi, R->phi, L->bx, R->bx);
+  double by_f = hll_flux(L->ezm, R->ezm, L->by, R->by);
+  double bz_f = hll_flux(L->ey, R->ey, L->bz, R->bz);
+  double ex_f = hll_flux(L->psi, R->psi, L->ex, R->ex);
+  double ey_f = hll_flux(L->bz, R->bz, L->ey, R->ey);
+  double ez_f = hll_flux(L->bym, R->bym, L->ez, R->ez);
+  double q_f = hll_flux(L->j1, R->j1, L->q, R->q);
+  double dd_f = hll_flux(L->fd, R->fd, L->d, R->d);
+  double tau_f = hll_flux(L->ftau, R->ftau, L->tau, R->tau);
+  double sx_f = hll_flux(L->fs1, R->fs1, L->sx, R->sx);
+  double sy_f = hll_flux(L->fs2, R->fs2, L->sy, R->sy);
+  double sz_f = hll_flux(L->fs3, R->fs3, L->sz, R->sz);
+
+  /* star E, B, q are the HLL states on both sides (:3104-3118) */
+  double erx = bz_h * ey_h - by_h * ez_h;
+  double ery = bx_h * ez_h - bz_h * ex_h;
+  double erz = by_h * ex_h - bx_h * ey_h;
+  double e2p = sq(ey_h) + sq(ez_h);
+  double b2p = sq(by_h) + sq(bz_h);
+  double e2b2s = 0.5 * (((sq(ex_h) + sq(ey_h)) + sq(ez_h)) + ((sq(bx_h) + sq(by_h)) + sq(bz_h)));
+  double a = tau_f - erx;
+  double b = ((e2p + b2p) - tau_h) - sx_f;
+  double c = sx_h - erx;
+  double v = hllc_contact(a, b, c);
+  if (v != v) return 7;
+
+  double pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
+  double psiR = (S2 * psi_h - v * ex_f) / (S2 - v);
+  double psiL = (-(S1 * psi_h) + v * ex_f) / (v - S1);
+  double phiR = (S2 * phi_h - v * bx_f) / (S2 - v);
+  double phiL = (-(S1 * phi_h) + v * bx_f) / (v - S1);
+  double sxR = (S2 * sx_h - tau_f * v) / (S2 - v);
+  double sxL = (-(S1 * sx_h) + tau_f * v) / (v - S1);
+  double syR = ((((S2 * sy_h - sy_f) - ery * v) - ex_h * ey_h) - bx_h * by_h) / (S2 - v);
+  double syL = ((((-(S1 * sy_h) + sy_f) + ery * v) + ex_h * ey_h) + bx_h * by_h) / (v - S1);
+  double szR = ((((S2 * sz_h - sz_f) - erz * v) - ex_h * ez_h) - bx_h * bz_h) / (S2 - v);
+  double szL = ((((-(S1 * sz_h) + sz_f) + erz * v) + ex_h * ez_h) + bx_h * bz_h) / (v - S1);
+  double tauR = ((sxR - erx) / v - pstr) + e2b2s;
+  double tauL = ((sxL - erx) / v - pstr) + e2b2s;
+  double ddR = (S2 * dd_h - dd_f) / (S2 - v);
+  double ddL = (-(S1 * dd_h) + dd_f) / (v - S1);
+
+  double psifR, psifL, phifR, phifL, fdR, fdL, ftR, ftL, fsxR, fsxL, fsyR, fsyL, fszR, fszL;
+  if (v == 0.0) {
+    psifR = psifL = psi_f; phifR = phifL = phi_f;
+    fdR = fdL = dd_f; ftR = ftL = tau_f;
+    fsxR = fsxL = sx_f; fsyR = fsyL = sy_f; fszR = fszL = sz_f;
+  } else {
+    double cr = S2 * (v - S1), cl = S1 * (S2 - v), den = S2 - S1;
+    psifR = psi_f + (cr * (psiR - psiL)) / den;
+    psifL = psi_f - (cl * (psiR - psiL)) / den;
+    phifR = phi_f + (cr * (phiR - phiL)) / den;
+    phifL = phi_f - (cl * (phiR - phiL)) / den;
+    fdR = dd_f + (cr * (ddR - ddL)) / den;
+    fdL = dd_f - (cl * (ddR - ddL)) / den;
+    ftR = tau_f + (cr * (tauR - tauL)) / den;
+    ftL = tau_f - (cl * (tauR - tauL)) / den;
+    fsxR = sx_f + (cr * (sxR - sxL)) / den;
+    fsxL = sx_f - (cl * (sxR - sxL)) / den;
+    fsyR = sy_f + (cr * (syR - syL)) / den;
+    fsyL = sy_f - (cl * (syR - syL)) / den;
+    fszR = sz_f + (cr * (szR - szL)) / den;
+    fszL = sz_f - (cl * (szR - szL)) / den;
+  }
+  /* selection :3908-4295; E/B star fluxes are the HLL fluxes */
+  if (S1 < 0.0 && 0.0 < v) {
+    f[FX_EXPSI] = psifL; f[FX_BXPHI] = phifL; f[FX_J] = q_f;
+    f[FX_FD] = fdL; f[FX_FTAU] = ftL; f[FX_FSX] = fsxL; f[FX_FSY] = fsyL; f[FX_FSZ] = fszL;
+    f[FX_PHIBX] = bx_f; f[FX_EZBY] = -by_f; f[FX_EYBZ] = bz_f;
+    f[FX_PSIEX] = ex_f; f[FX_BZEY] = ey_f; f[FX_BYEZ] = -ez_f;
+  } else if (v < 0.0 && 0.0 < S2) {
+    f[FX_EXPSI] = psifR; f[FX_BXPHI] = phifR; f[FX_J] = q_f;
+    f[FX_FD] = fdR; f[FX_FTAU] = ftR; f[FX_FSX] = fsxR; f[FX_FSY] = fsyR; f[FX_FSZ] = fszR;
+    f[FX_PHIBX] = bx_f; f[FX_EZBY] = -by_f; f[FX_EYBZ] = bz_f;
+    f[FX_PSIEX] = ex_f; f[FX_BZEY] = ey_f; f[FX_BYEZ] = -ez_f;
+  } else if (v == 0.0) {
+    f[FX_EXPSI] = 0.5 * (psifR + psifL); f[FX_BXPHI] = 0.5 * (phifR + phifL);
+    f[FX_PHIBX] = 0.5 * (bx_f + bx_f); f[FX_EZBY] = -(0.5 * (by_f + by_f));
+    f[FX_EYBZ] = 0.5 * (bz_f + bz_f); f[FX_PSIEX] = 0.5 * (ex_f + ex_f);
+    f[FX_BZEY] = 0.5 * (ey_f + ey_f); f[FX_BYEZ] = -(0.5 * (ez_f + ez_f));
+    f[FX_J] = 0.5 * (q_f + q_f); f[FX_FD] = 0.5 * (fdR + fdL); f[FX_FTAU] = 0.5 * (ftR + ftL);
+    f[FX_FSX] = 0.5 * (fsxR + fsxL); f[FX_FSY] = 0.5 * (fsyR + fsyL); f[FX_FSZ] = 0.5 * (fszR + fszL);
+  } else {
+    return 8; /* "Revise Vxstar velocity" stop */
+  }
+  return 0;
+}
+
+/* HLLC y face: 16_hllc_flow_mpx.f95:2950-3095 (HLL), 3338-3489, 3536-3574,
+ * 3742-3901 (star), 4305-4606 (selection) */
+static int hllc_y(const side_t *L, const side_t *R, double f[FY_N]) {
+  double psi_h = hll_state(L->psi, R->psi, L->ey, R->ey);
+  double phi_h = hll_state(L->phi, R->phi, L->by, R->by);
+  double ey_h = hll_state(L->ey, R->ey, L->psi, R->psi);
+  double ez_h = hll_state(L->ez, R->ez, L->bx, R->bx);
+  double ex_h = hll_state(L->ex, R->ex, L->bzm, R->bzm);
+  double by_h = hll_state(L->by, R->by, L->phi, R->phi);
+  double bz_h = hll_state(L->bz, R->bz, L->exm, R->exm);
+  double bx_h = hll_state(L->bx, R->bx, L->ez, R->ez);
+  double dd_h = hll_state(L->d, R->d, L->fd, R->fd);
+  double tau_h = hll_state(L->tau, R->tau, L->ftau, R->ftau);
+  double sy_h = hll_state(L->sy, R->sy, L->fs2, R->fs2);
+  double sz_h = hll_state(L->sz, R->sz, L->fs3, R->fs3);
+  double sx_h = hll_state(L->sx, R->sx, L->fs1, R->fs1);
+
+  double psi_f = hll_flux(L->ey, R->ey, L->psi, R->psi);
+  double phi_f = hll_flux(L->by, R->by, L->phi, R->phi);
+  double by_f = hll_flux(L->phi, R->phi, L->by, R->by);
+  double bz_f = hll_flux(L->exm, R->exm, L->bz, R->bz);
+  double bx_f = hll_flux(L->ez, R->ez, L->bx, R->bx);
+  double ey_f = hll_flux(L->psi, R->psi, L->ey, R->ey);
+  double ez_f = hll_flux(L->bx, R->bx, L->ez, R->ez);
+  double ex_f = hll_flux(L->bzm, R->bzm, L->ex, R->ex);
+  double q_f = hll_flux(L->j1, R->j1, L->q, R->q);
+  double dd_f = hll_flux(L->fd, R->fd, L->d, R->d);
+  double tau_f = hll_flux(L->ftau, R->ftau, L->tau, R->tau);
+  double sy_f = hll_flux(L->fs2, R->fs2, L->sy, R->sy);
+  double sz_f = hll_flux(L->fs3, R->fs3, L->sz, R->sz);
+  double sx_f = hll_flux(L->fs1, R->fs1, L->sx, R->sx);
+
+  double erx = bz_h * ey_h - by_h * ez_h;
+  double ery = bx_h * ez_h - bz_h * ex_h;
+  double erz = by_h * ex_h - bx_h * ey_h;
+  double e2p = sq(ex_h) + sq(ez_h);
+  double b2p = sq(bx_h) + sq(bz_h);
+  double e2b2s = 0.5 * (((sq(ex_h) + sq(ey_h)) + sq(ez_h)) + ((sq(bx_h) + sq(by_h)) + sq(bz_h)));
+  double a = tau_f - ery;
+  double b = ((e2p + b2p) - tau_h) - sy_f;
+  double c = sy_h - ery;
+  double v = hllc_contact(a, b, c);
+  if (v != v) return 7;
+
+  double pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
+  double psiR = (S2 * psi_h - v * ey_f) / (S2 - v);
+  double psiL = (-(S1 * psi_h) + v * ey_f) / (v - S1);
+  double phiR = (S2 * phi_h - v * by_f) / (S2 - v);
+  double phiL = (-(S1 * phi_h) + v * by_f) / (v - S1);
+  double syR = (S2 * sy_h - tau_f * v) / (S2 - v);
+  double syL = (-(S1 * sy_h) + tau_f * v) / (v - S1);
+  double szR = ((((S2 * sz_h - sz_f) - erz * v) - ey_h * ez_h) - by_h * bz_h) / (S2 - v);
+  double szL = ((((-(S1 * sz_h) + sz_f) + erz * v) + ey_h * ez_h) + by_h * bz_h) / (v - S1);
+  double sxR = ((((S2 * sx_h - sx_f) - erx * v) - ey_h * ex_h) - by_h * bx_h) / (S2 - v);
+  double sxL = ((((-(S1 * sx_h) + sx_f) + erx * v) + ey_h * ex_h) + by_h * bx_h) / (v - S1);
+  double tauR = ((syR - ery) / v - pstr) + e2b2s;
+  double tauL = ((syL - ery) / v - pstr) + e2b2s;
+  double ddR = (S2 * dd_h - dd_f) / (S2 - v);
+  double ddL = (-(S1 * dd_h) + dd_f) / (v - S1);
+
+  double psifR, psifL, phifR, phifL, fdR, fdL, ftR, ftL, fsxR, fsxL, fsyR, fsyL, fszR, fszL;
+  if (v == 0.0) {
+    psifR = psifL = psi_f; phifR = phifL = phi_f;
+    fdR = fdL = dd_f; ftR = ftL = tau_f;
+    fsxR = fsxL = sx_f; fsyR = fsyL = sy_f; fszR = fszL = sz_f;
+  } else {
+    double cr = S2 * (v - S1), cl = S1 * (S2 - v), den = S2 - S1;
+    psifR = psi_f + (cr * (psiR - psiL)) / den;
+    psifL = psi_f - (cl * (psiR - psiL)) / den;
+    phifR = phi_f + (cr * (phiR - phiL)) / den;
+    phifL = phi_f - (cl * (phiR - phiL)) / den;
+    fdR = dd_f + (cr * (ddR - ddL)) / den;
+    fdL = dd_f - (cl * (ddR - ddL)) / den;
+    ftR = tau_f + (cr * (tauR - tauL)) / den;
+    ftL = tau_f - (cl * (tauR - tauL)) / den;
+    fsxR = sx_f + (cr * (sxR - sxL)) / den;
+    fsxL = sx_f - (cl * (sxR - sxL)) / den;
+    fsyR = sy_f + (cr * (syR - syL)) / den;
+    fsyL = sy_f - (cl * (syR - syL)) / den;
+    fszR = sz_f + (cr * (szR - szL)) / den;
+    fszL = sz_f - (cl * (szR - szL)) / den;
+  }
+  if (S1 < 0.0 && 0.0 < v) {
+    f[FY_EYPSI] = psifL; f[FY_BYPHI] = phifL; f[FY_J] = q_f;
+    f[FY_FD] = fdL; f[FY_FTAU] = ftL; f[FY_FSX] = fsxL; f[FY_FSY] = fsyL; f[FY_FSZ] = fszL;
+    f[FY_EZBX] = bx_f; f[FY_PHIBY] = by_f; f[FY_EXBZ] = -bz_f;
+    f[FY_BZEX] = -ex_f; f[FY_PSIEY] = ey_f; f[FY_BXEZ] = ez_f;
+  } else if (v < 0.0 && 0.0 < S2) {
+    f[FY_EYPSI] = psifR; f[FY_BYPHI] = phifR; f[FY_J] = q_f;
+    f[FY_FD] = fdR; f[FY_FTAU] = ftR; f[FY_FSX] = fsxR; f[FY_FSY] = fsyR; f[FY_FSZ] = fszR;
+    f[FY_EZBX] = bx_f; f[FY_PHIBY] = by_f; f[FY_EXBZ] = -bz_f;
+    f[FY_BZEX] = -ex_f; f[FY_PSIEY] = ey_f; f[FY_BXEZ] = ez_f;
+  } else if (v == 0.0) {
+    f[FY_EYPSI] = 0.5 * (psifR + psifL); f[FY_BYPHI] = 0.5 * (phifR + phifL);
+    f[FY_EZBX] = 0.5 * (bx_f + bx_f); f[FY_PHIBY] = 0.5 * (by_f + by_f);
+    f[FY_EXBZ] = -(0.5 * (bz_f + bz_f)); f[FY_BZEX] = -(0.5 * (ex_f + ex_f));
+    f[FY_PSIEY] = 0.5 * (ey_f + ey_f); f[FY_BXEZ] = 0.5 * (ez_f + ez_f);
+    f[FY_J] = 0.5 * (q_f + q_f); f[FY_FD] = 0.5 * (fdR + fdL); f[FY_FTAU] = 0.5 * (ftR + ftL);
+    f[FY_FSX] = 0.5 * (fsxR + fsxL); f[FY_FSY] = 0.5 * (fsyR + fsyL); f[FY_FSZ] = 0.5 * (fszR + fszL);
+  } else {
+    return 8;
+  }
+  return 0;
+}
+
+/* face loop: 16_hll_flow_mpx.f95:1358-1958, 16_hllc_flow_mpx.f95:2401-4612 */
+static int face_loop(oracle_state *s) {
+  const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
+  double **P = s->prim;
+  for (int j = -1; j <= jm; j++)
+    for (int i = -1; i <= im; i++) {
+      size_t k = IX(s, i, j);
+      side_t xl, xr, yl, yr;
+      load_x(s, s->xL, k, &xl);
+      load_x(s, s->xR, k, &xr);
+      load_y(s, s->yL, k, &yl);
+      load_y(s, s->yR, k, &yr);
+      /* superluminal fallback (16_hll_flow_mpx.f95:1392-1461): any of the
+       * four face speeds >= 1 replaces all of them by cell values, with the
+       * reference's (i-1,j) and (i,j-1) right states. */
+      double v2xl = (sq(xl.vx) + sq(xl.vy)) + sq(xl.vz);
+      double v2xr = (sq(xr.vx) + sq(xr.vy)) + sq(xr.vz);
+      double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
+      double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
+      if (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0) {
+        size_t kc = IX(s, i, j), km = IX(s, i - 1, j), kj = IX(s, i, j - 1);
+        xl.vx = P[OR_VX][kc]; xl.vy = P[OR_VY][kc]; xl.vz = P[OR_VZ][kc];
+        xr.vx = P[OR_VX][km]; xr.vy = P[OR_VY][km]; xr.vz = P[OR_VZ][km];
+        yl.vx = P[OR_VX][kc]; yl.vy = P[OR_VY][kc]; yl.vz = P[OR_VZ][kc];
+        yr.vx = P[OR_VX][kj]; yr.vy = P[OR_VY][kj]; yr.vz = P[OR_VZ][kj];
+      }
+      if (fs == OR_HLLC) { /* 16_hllc_flow_mpx.f95:2663-2693 */
+        double rc = 0.1 * P[OR_RHO][k];
+        if (yl.rho < 0.0) yl.rho = rc;
+        else if (yr.rho < 0.0) yr.rho = rc;
+        else if (xl.rho < 0.0) xl.rho = rc;
+        else if (xr.rho < 0.0) xr.rho = rc;
+      }
+      side_physics(s->c.gamma, 0, &xl);
+      side_physics(s->c.gamma, 0, &xr);
+      side_physics(s->c.gamma, 1, &yl);
+      side_physics(s->c.gamma, 1, &yr);
+      double fx[FX_N], fy[FY_N];
+      if (fs == OR_HLL) {
+        hll_x(&xl, &xr, fx);
+        hll_y(&yl, &yr, fy);
+      } else if (fs == OR_HLLC) {
+        int st = hllc_x(&xl, &xr, fx);
+        if (st) return st;
+        st = hllc_y(&yl, &yr, fy);
+        if (st) return st;
+      } else {
+        return 9; /* HLLE is not restated yet */
+      }
+      for (int f = 0; f < FX_N; f++) s->fx[f][k] = fx[f];
+      for (int f = 0; f < FY_N; f++) s->fy[f][k] = fy[f];
+    }
+  return 0;
+}
+
+/* fluxandsource, 2D IMEX branch: 17_fluxandsource.f95:442-542 */
+static void fluxandsource(oracle_state *s, int l) {
+  const double dx = s->c.dx, dy = s->c.dy, sig = s->c.sigma;
+  double **P = s->prim, **S = s->st, **X = s->fx, **Y = s->fy, **R = s->rhs[l];
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j), kw = IX(s, i - 1, j), ks = IX(s, i, j - 1);
+#define DXF(a) ((X[a][k] - X[a][kw]) / dx)
+#define DYF(a) ((Y[a][k] - Y[a][ks]) / dy)
+      double vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k], w = s->W[k];
+      double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
+      double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k], q = S[ST_Q][k];
+      double edv = (ex * vx + ey * vy) + ez * vz;
+      double rx = bz * vy - by * vz, ry = bx * vz - bz * vx, rz = by * vx - bx * vy;
+      R[0][k] = ((((-DXF(FX_EXPSI)) - DYF(FY_EYPSI)) - 0.0) + q) - s->c.kappa_psi * S[ST_PSI][k];
+      R[1][k] = (((-DXF(FX_BXPHI)) - DYF(FY_BYPHI)) - 0.0) - s->c.kappa_phi * S[ST_PHI][k];
+      R[2][k] = ((-DYF(FY_EZBX)) + 0.0) - DXF(FX_PHIBX);
+      R[3][k] = ((-0.0) + DXF(FX_EZBY)) - DYF(FY_PHIBY);
+      R[4][k] = ((-DXF(FX_EYBZ)) + DYF(FY_EXBZ)) - 0.0;
+      R[5][k] = ((DYF(FY_BZEX) - 0.0) - DXF(FX_PSIEX)) - q * vx;
+      R[6][k] = ((0.0 - DXF(FX_BZEY)) - DYF(FY_PSIEY)) - q * vy;
+      R[7][k] = ((DXF(FX_BYEZ) - DYF(FY_BXEZ)) - 0.0) - q * vz;
+      R[8][k] = ((-DXF(FX_J)) - DYF(FY_J)) - 0.0;
+      R[9][k] = ((-DXF(FX_FD)) - DYF(FY_FD)) - 0.0;
+      R[10][k] = ((-DXF(FX_FTAU)) - DYF(FY_FTAU)) - 0.0;
+      R[11][k] = ((-DXF(FX_FSX)) - DYF(FY_FSX)) - 0.0;
+      R[12][k] = ((-DXF(FX_FSY)) - DYF(FY_FSY)) - 0.0;
+      R[13][k] = ((-DXF(FX_FSZ)) - DYF(FY_FSZ)) - 0.0;
+      R[14][k] = (sig * w) * ((ex + rx) - edv * vx);
+      R[15][k] = (sig * w) * ((ey + ry) - edv * vy);
+      R[16][k] = (sig * w) * ((ez + rz) - edv * vz);
+#undef DXF
+#undef DYF
+    }
+}
+
+static void sumfin(oracle_state *s, int l) {
+  const double wb = s->c.wb[l], wbt = s->c.wbt[l];
+  for (int a = 0; a < AC_N; a++) {
+    double *acc = s->finsum[a];
+    const double *f = s->rhs[l][a];
+    const double *src = (a >= AC_EX && a <= AC_EZ) ? s->rhs[l][14 + (a - AC_EX)] : NULL;
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        size_t k = IX(s, i, j);
+        if (src)
+          acc[k] = acc[k] + (wb * f[k] - wbt * src[k]);
+        else
+          acc[k] = acc[k] + wb * f[k];
+      }
+  }
+}
+
+static void varfin(oracle_state *s) {
+  const double dt = s->c.dt;
+  for (int a = 0; a < AC_N; a++) {
+    double *u = (double *)un_plane(s, a);
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        size_t k = IX(s, i, j);
+        u[k] = u[k] + dt * s->finsum[a][k];
+      }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* varprimitivecardano, l_swap = 1: 12_varprimitive.f95:48-813          */
+/* in:  psi phi Bx By Bz Vx Vy Vz Ex Ey Ez D tau S..., see below        */
+/* ------------------------------------------------------------------ */
+
+/* in[14] = {D, tau, Sx, Sy, Sz, Ex, Ey, Ez, Bx, By, Bz, Vx0, Vy0, Vz0} plus
+ * the old rho and p through out[3], out[4] on entry.
+ * out[5] = {Vx, Vy, Vz, rho, p}; flags = fix-ups {rho, p, drift}.
+ * Returns 0 or a stop status. */
+int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags[3]) {
+  const double TOL = 1.0e-10;
+  const int nrmax = 40;
+  double D = in[0], tau = in[1], Sx = in[2], Sy = in[3], Sz = in[4];
+  double Ex = in[5], Ey = in[6], Ez = in[7], Bx = in[8], By = in[9], Bz = in[10];
+  double vx0 = in[11], vy0 = in[12], vz0 = in[13];
+  double rho_old = out[3], p_old = out[4];
+  flags[0] = flags[1] = flags[2] = 0;
+
+  double E2B2 = 0.5 * (((((sq(Ex) + sq(Ey)) + sq(Ez)) + sq(Bx)) + sq(By)) + sq(Bz));
+  double g1 = (gamma - 1.0) / gamma;
+  double V2c = (sq(vx0) + sq(vy0)) + sq(vz0);
+  if (V2c > 1.0) return 20; /* superluminal old velocity: the reference clamps with an undefined eps1 */
+
+  double C1 = (sq((Sx - Ey * Bz) + Ez * By) + sq((Sy - Ez * Bx) + Ex * Bz)) + sq((Sz - Ex * By) + Ey * Bx);
+  double C2 = tau - E2B2;
+  double a4 = C1 - sq(C2);
+  double a3 = ((2.0 * g1) * C2) * D;
+  double a2 = (sq(C2) - (2.0 * g1) * C1) - sq(g1) * sq(D);
+  double a1 = -(((2.0 * g1) * C2) * D);
+  double a0 = sq(g1) * (C1 + sq(D));
+  double b3 = a3 / a4, b2 = a2 / a4, b1 = a1 / a4, b0 = a0 / a4;
+  double d2 = b2 - (3.0 * sq(b3)) / 8.0;
+  double d1 = (b1 - (b3 * b2) / 2.0) + pw3(b3) / 8.0;
+  double d0 = ((b0 - (b3 * b1) / 4.0) + (sq(b3) * b2) / 16.0) - (3.0 * pw4(b3)) / 256.0;
+  double e = (20.0 * d2) / 8.0;
+  double f = -(d0 - 2.0 * sq(d2));
+  double g = -((sq(d1) + (4.0 * d2) * d0) - 4.0 * pw3(d2)) / 8.0;
+  double Q1 = (sq(e) - 3.0 * f) / 9.0;
+  double R1 = ((2.0 * pw3(e) - (9.0 * e) * f) + 27.0 * g) / 54.0;
+  double s, t, det, z1;
+  const double pi = 3.14159265358979323846264338327950288;
+  if (sq(R1) < pw3(Q1)) { /* :340-425 */
+    double T1 = acos(R1 / fabs(sqrt(pw3(Q1))));
+    double sQ = fabs(sqrt(Q1));
+    double za = (-((2.0 * sQ) * cos(T1 / 3.0))) - e / 3.0;
+    double sa = d2 + 2.0 * za, ta = ((sq(za) + (2.0 * d2) * za) + sq(d2)) - d0;
+    double zb = (-((2.0 * sQ) * cos((T1 + 2.0 * pi) / 3.0))) - e / 3.0;
+    double sb = d2 + 2.0 * zb, tb = ((sq(zb) + (2.0 * d2) * zb) + sq(d2)) - d0;
+    double zc = (-((2.0 * sQ) * cos((T1 - 2.0 * pi) / 3.0))) - e / 3.0;
+    double sc = d2 + 2.0 * zc, tc = ((sq(zc) + (2.0 * d2) * zc) + sq(d2)) - d0;
+    if (sa >= 0.0 && ta >= 0.0) { s = fabs(sqrt(sa)); t = fabs(sqrt(ta)); z1 = za; }
+    else if (sb >= 0.0 && tb >= 0.0) { s = fabs(sqrt(sb)); t = fabs(sqrt(tb)); z1 = zb; }
+    else if (sc >= 0.0 && tc >= 0.0) { s = fabs(sqrt(sc)); t = fabs(sqrt(tc)); z1 = zc; }
+    else return 21; /* "todo cardano IMAGINARIO" */
+    det = sq(s) - 4.0 * ((d2 + z1) - t);
+    if (det < 0.0) return 22;
+  } else { /* :427-452 */
+    double xx1 = fabs(R1) + fabs(sqrt(sq(R1) - pw3(Q1)));
+    double AA = -(copysign(1.0, R1) * fsign(fabs(pow(xx1, 1.0 / 3.0)), xx1));
+    double BB = AA != 0.0 ? Q1 / AA : 0.0;
+    z1 = (AA + BB) - e / 3.0;
+    s = fabs(sqrt(d2 + 2.0 * z1));
+    t = fabs(sqrt(((sq(z1) + (2.0 * d2) * z1) + sq(d2)) - d0));
+    det = sq(s) - 4.0 * ((d2 + z1) - t);
+  }
+  double Q2 = -(0.5 * ((-s) + copysign(1.0, -s) * fabs(sqrt(det))));
+  double y1 = Q2 - b3 / 4.0;
+  double y2 = (((d2 + z1) - t) / Q2) - b3 / 4.0;
+  double Wn = NAN;
+  if (y1 >= 1.0 && y2 >= 1.0) Wn = fmin2(y1, y2);
+  if (y1 <= 1.0 && y2 <= 1.0) return 23;
+  if (y1 >= 1.0 && y2 <= 1.0) Wn = y1;
+  if (y1 <= 1.0 && y2 >= 1.0) Wn = y2;
+  if (Wn != Wn) return 24;
+
+  /* Newton on W, :510-526 */
+  int nrw = 1;
+  double cha1 = 1.0;
+  while (cha1 > TOL && nrw < nrmax) {
+    double Wr = Wn - ((((pw4(Wn) + b3 * pw3(Wn)) + b2 * sq(Wn)) + b1 * Wn) + b0) /
+                         (((4.0 * pw3(Wn) + (3.0 * b3) * sq(Wn)) + (2.0 * b2) * Wn) + b1);
+    cha1 = fabs(Wr - Wn) / (0.5 * (Wr + Wn));
+    Wn = Wr;
+    nrw = nrw + 1;
+  }
+  double W = Wn;
+  double rho = D / W;
+  if (rho < 0.0) { flags[0] = 1; rho = fmax2(rho_old * 1.0e-1, rho); }
+  double h = ((tau - E2B2) - g1 * rho) / (sq(W) - g1);
+  double p = g1 * (h - rho);
+
+  /* pressure fixed point, :558-587 */
+  double pn = p;
+  int nr = 1;
+  double epsn = (((tau - E2B2) - D * W) + pn * (1.0 - sq(W))) / (D * W);
+  double cs2 = ((gamma * (gamma - 1.0)) * epsn) / (1.0 + gamma * epsn);
+  double cha = 1.0;
+  while (cha >= TOL && nr <= nrmax) {
+    double pr = pn - (((gamma - 1.0) * rho) * epsn - pn) / (V2c * cs2 - 1.0);
+    cha = fabs(pr - pn) / (0.5 * (pr + pn));
+    pn = pr;
+    nr = nr + 1;
+  }
+  if (pn < 0.0) { flags[1] = 1; pn = fmax2(p_old * 1.0e-4, pn); }
+  p = pn;
+  tau = (E2B2 + h * sq(W)) - p;
+  double den = (tau - E2B2) + p;
+  double vx = ((Sx - Ey * Bz) + Ez * By) / den;
+  double vy = ((Sy - Ez * Bx) + Ex * Bz) / den;
+  double vz = ((Sz - Ex * By) + Ey * Bx) / den;
+  V2c = (sq(vx) + sq(vy)) + sq(vz);
+  if (V2c >= 1.0) { /* drift branch :627-696 */
+    double Wsl = 1.0 / sqrt(1.0 - V2c);
+    double erb = ((((Ey * Bz - Ez * By) + Ez * Bx) - Ex * Bz) + Ex * By) - Ey * Bx;
+    double b2d = (sq(Bx) + sq(By)) + sq(Bz);
+    if (!(b2d / rho >= 10.0)) return 25;
+    double vd = erb / b2d;
+    double Wd = 1.0 / sqrt(1.0 - sq(vd));
+    rho = D / Wd;
+    h = ((tau - E2B2) - g1 * rho) / (sq(Wd) - g1);
+    p = g1 * (h - rho);
+    tau = (E2B2 + h * sq(Wsl)) - p;
+    den = (tau - E2B2) + p;
+    vx = ((Sx - Ey * Bz) + Ez * By) / den;
+    vy = ((Sy - Ez * Bx) + Ex * Bz) / den;
+    vz = ((Sz - Ex * By) + Ey * Bx) / den;
+    V2c = (sq(vx) + sq(vy)) + sq(vz);
+    flags[2] = 1;
+    if (V2c >= 1.0) return 26;
+  }
+  out[0] = vx; out[1] = vy; out[2] = vz; out[3] = rho; out[4] = p;
+  return 0;
+}
+
+static int c2p(oracle_state *s) {
+  double **P = s->prim;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double in[14] = {s->D[k], s->tau[k], s->Sx[k], s->Sy[k], s->Sz[k],
+                       P[OR_EX][k], P[OR_EY][k], P[OR_EZ][k], P[OR_BX][k], P[OR_BY][k], P[OR_BZ][k],
+                       P[OR_VX][k], P[OR_VY][k], P[OR_VZ][k]};
+      double out[5] = {0, 0, 0, P[OR_RHO][k], P[OR_P][k]};
+      int fl[3];
+      int st = oracle_c2p_point(s->c.gamma, in, out, fl);
+      if (st) return st;
+      s->cnt[0] += fl[0]; s->cnt[1] += fl[1]; s->cnt[2] += fl[2];
+      P[OR_VX][k] = out[0]; P[OR_VY][k] = out[1]; P[OR_VZ][k] = out[2];
+      P[OR_RHO][k] = out[3]; P[OR_P][k] = out[4];
+    }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* one time step: main_cueva_code.f95:47-332                            */
+/* ------------------------------------------------------------------ */
+
+int oracle_step(oracle_state *s, int nsteps) {
+  for (int h = 0; h < nsteps; h++) {
+    for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
+    startconvar(s);
+    cleanvarint(s);
+    s->cnt[0] = s->cnt[1] = s->cnt[2] = 0;
+    for (int l = 0; l < s->c.lmax; l++) {
+      for (int m = 0; m < l; m++) sumint(s, l, m);
+      varint(s);
+      electricvarint(s, l);
+      flowinit(s);
+      reconstruct(s);
+      int st = face_loop(s);
+      if (st) return st;
+      fluxandsource(s, l);
+      sumfin(s, l);
+    }
+    varfin(s);
+    int st = c2p(s);
+    if (st) return st;
+    for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
+  }
+  return 0;
+}

@@ -1,0 +1,167 @@
+! Harness that drives the CUEVA reference subroutines for the oracle.
+!
+! Test infrastructure only: it is linked with the reference units by
+! oracle/refbuild/build_ref.sh and its output becomes golden fixtures.
+!
+! It restates the per-step call sequence of the reference driver
+! (main_cueva_code.f95:47-356) with I/O and the commented-out adaptive dt
+! left out, and replaces the RESTART read (03_restart_primitive.f95:50-61)
+! by a binary read of the same 14 fields, so that seeded initial data reach
+! the reference bit for bit.
+!
+! usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug]
+!   INPUT      stream file: 14 arrays (-6:imax+6,-6:jmax+6) of real(8), x fastest,
+!              in the restart order psi phi Bx By Bz Vx Vy Vz Ex Ey Ez q rho p
+!   OUTPREFIX  each dump step h writes OUTPREFIX_hNNNNN.bin:
+!              int32 counters (contador_r, contador_p, contador_v), then 19 arrays:
+!              the 14 fields above followed by D tau Sx Sy Sz
+!   -debug     also write OUTPREFIX_stage.bin after step 1 with, for every stage l:
+!              W, Exast..Ezast, the 14 stage fields, Jx Jy Jz, the 28 face fluxes,
+!              and the 17 stage right-hand sides (see write_stage below)
+
+program cueva_ref_harness
+
+  use scalar
+  use parameters
+  use threevectors
+  use fourvectors
+
+  implicit none
+
+  character(len=1024) :: arg, infile, prefix, fname
+  integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out
+  integer, allocatable :: dumps(:)
+  logical             :: debug
+
+  nargs = command_argument_count()
+  if (nargs < 3) then
+     write(*,*) "usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug]"
+     stop 2
+  end if
+  call get_command_argument(1, infile)
+  call get_command_argument(2, arg)
+  read(arg,*) nsteps
+  call get_command_argument(3, prefix)
+  allocate(dumps(max(1,nargs)))
+  ndump = 0
+  debug = .false.
+  do iarg = 4, nargs
+     call get_command_argument(iarg, arg)
+     if (trim(arg) == "-debug") then
+        debug = .true.
+     else
+        ndump = ndump + 1
+        read(arg,*) dumps(ndump)
+     end if
+  end do
+
+  open(301, file=trim(prefix)//"_fixup_rho.txt", action="write")
+  open(302, file=trim(prefix)//"_fixup_p.txt",   action="write")
+  open(303, file=trim(prefix)//"_fixup_v.txt",   action="write")
+
+  open(newunit=u_in, file=trim(infile), access="stream", form="unformatted", status="old", action="read")
+  read(u_in) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
+             Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
+             q(:,:,1), rho(:,:,1), p(:,:,1)
+  close(u_in)
+
+  t_init = 0
+  call boundary
+  t_advance = (hini-1) * Delt
+
+  do h = 1, nsteps
+
+     call boundary
+     t_advance = t_advance + Delt
+
+     call startconvar
+     call cleanvarint
+     contador_r = 0
+     contador_p = 0
+     contador_v = 0
+     call swap1
+
+     do l = 1, lmax
+        do m = 1, l-1
+           call sumint
+        end do
+        call varint
+        call electricvarint
+        call boundary_conserved
+        call boundary_electric
+        call flowinit
+        call boundary_flux_conserved
+        if (flux_solver == 5) then
+           call hll_flow_mpx
+        else if (flux_solver == 6) then
+           call hlle_flow_mpx
+        else if (flux_solver == 7) then
+           call hllc_flow_mpx
+        else
+           write(*,*) "harness: flux_solver must be 5, 6 or 7"
+           stop 3
+        end if
+        call fluxandsource
+        call sumfin
+        if (debug .and. h == 1) call write_stage(l)
+     end do
+
+     l = lmax
+     call varfin
+     call electricfield
+     l_swap = 1
+     call varprimitivecardano
+     call boundary
+     t_init = 1
+
+     do kd = 1, ndump
+        if (dumps(kd) == h) then
+           write(fname,'(a,"_h",i5.5,".bin")') trim(prefix), h
+           open(newunit=u_out, file=trim(fname), access="stream", form="unformatted", &
+                status="replace", action="write")
+           write(u_out) contador_r, contador_p, contador_v
+           write(u_out) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
+                        Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
+                        q(:,:,1), rho(:,:,1), p(:,:,1), &
+                        D(:,:,1), tau(:,:,1), Sx(:,:,1), Sy(:,:,1), Sz(:,:,1)
+           close(u_out)
+        end if
+     end do
+  end do
+
+  close(301)
+  close(302)
+  close(303)
+
+contains
+
+  subroutine write_stage(ls)
+    integer, intent(in) :: ls
+    integer :: uo
+    character(len=1100) :: fn
+    write(fn,'(a,"_stage",i1,".bin")') trim(prefix), ls
+    open(newunit=uo, file=trim(fn), access="stream", form="unformatted", status="replace", action="write")
+    write(uo) W(:,:,1), Exast(:,:,1,ls), Eyast(:,:,1,ls), Ezast(:,:,1,ls)
+    write(uo) psiint(:,:,1,ls), phiint(:,:,1,ls), Bxint(:,:,1,ls), Byint(:,:,1,ls), Bzint(:,:,1,ls), &
+              Exint(:,:,1,ls), Eyint(:,:,1,ls), Ezint(:,:,1,ls), qint(:,:,1,ls), DDint(:,:,1,ls), &
+              tauint(:,:,1,ls), Sxint(:,:,1,ls), Syint(:,:,1,ls), Szint(:,:,1,ls)
+    write(uo) Jxint(:,:,1,ls), Jyint(:,:,1,ls), Jzint(:,:,1,ls)
+    ! x faces (14)
+    write(uo) Exlaxpsi(:,:,1,ls), Bxlaxphi(:,:,1,ls), philaxBx(:,:,1,ls), EzlaxBy(:,:,1,ls), &
+              EylaxBz(:,:,1,ls), psilaxEx(:,:,1,ls), BzlaxEy(:,:,1,ls), BylaxEz(:,:,1,ls), &
+              Jxlax(:,:,1,ls), FDxlax(:,:,1,ls), Ftauxlax(:,:,1,ls), FSxxlax(:,:,1,ls), &
+              FSxylax(:,:,1,ls), FSxzlax(:,:,1,ls)
+    ! y faces (14)
+    write(uo) Eylaxpsi(:,:,1,ls), Bylaxphi(:,:,1,ls), EzlaxBx(:,:,1,ls), philaxBy(:,:,1,ls), &
+              ExlaxBz(:,:,1,ls), BzlaxEx(:,:,1,ls), psilaxEy(:,:,1,ls), BxlaxEz(:,:,1,ls), &
+              Jylax(:,:,1,ls), FDylax(:,:,1,ls), Ftauylax(:,:,1,ls), FSyxlax(:,:,1,ls), &
+              FSyylax(:,:,1,ls), FSyzlax(:,:,1,ls)
+    ! stage right-hand sides (17)
+    write(uo) psiflux(:,:,1,ls), phiflux(:,:,1,ls), Bxflux(:,:,1,ls), Byflux(:,:,1,ls), Bzflux(:,:,1,ls), &
+              Exastflux(:,:,1,ls), Eyastflux(:,:,1,ls), Ezastflux(:,:,1,ls), qflux(:,:,1,ls), &
+              Dflux(:,:,1,ls), tauflux(:,:,1,ls), Sxflux(:,:,1,ls), Syflux(:,:,1,ls), Szflux(:,:,1,ls), &
+              Exastsour(:,:,1,ls), Eyastsour(:,:,1,ls), Ezastsour(:,:,1,ls)
+    close(uo)
+  end subroutine write_stage
+
+end program cueva_ref_harness
