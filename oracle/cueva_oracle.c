@@ -659,16 +659,19 @@ static void hll_y(const side_t *L, const side_t *R, double f[FY_N]) {
 }
 
 /* get_hllc_vel, funciones.f95:188-259: Newton on a*v^2 - b*v + c = 0 in
- * [-1,1], started from the clamped seed (see the file header). */
+ * [-1,1].  The reference evaluates the first Newton step at its iterate v
+ * before assigning it (funciones.f95:222-232); v0 is that point.  The seed
+ * mode passes v0 = clamp(rt); the reference-build mode passes the value the
+ * flang -O2 build leaves in that register (see hllc_v0_x/hllc_v0_y). */
 /* max(lo, min(hi, x)) as the reference build evaluates it: a NaN passes through */
 static inline double clampf(double x, double lo, double hi) {
   double t = x > hi ? hi : x;
   return t < lo ? lo : t;
 }
 
-double oracle_hllc_vel(double a, double b, double c, double rt) {
+double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
   double oldv = clampf(rt, -1.0, 1.0);
-  double v = oldv;
+  double v = v0;
   int iter = 0;
   for (;;) {
     double fv = (a * v * v - b * v) + c;
@@ -691,13 +694,14 @@ static inline double hll_flux(double fl, double fr, double ul, double ur) {
   return 0.5 * ((fl + fr) - (ur - ul));
 }
 
-/* contact speed of 16_hllc_flow_mpx.f95:3185-3334 */
-static double hllc_contact(double a, double b, double c) {
+/* contact speed of 16_hllc_flow_mpx.f95:3185-3334.  v0 is the point of the
+ * first Newton evaluation (see oracle_hllc_vel). */
+static double hllc_contact(double a, double b, double c, double v0) {
   double det = sq(b) - (4.0 * a) * c;
   double qr = -(0.5 * (b + copysign(1.0, b) * fabs(sqrt(det))));
   double rt2 = c / qr;
   if (det < 0.0) rt2 = c / b;
-  double v = oracle_hllc_vel(a, -b, c, rt2);
+  double v = oracle_hllc_vel(a, -b, c, rt2, v0 == v0 ? v0 : clampf(rt2, -1.0, 1.0));
   if (fabs(v) <= 1.0e-9)
     v = 0.0;
   else if (fabs(v) > 1.0)
@@ -706,6 +710,12 @@ static double hllc_contact(double a, double b, double c) {
 }
 
 /* HLLC x face: 16_hllc_flow_mpx.f95:2811-4295. Returns 0 or a stop status. */
+/* First Newton point of the x-face contact solve in the flang -O2 reference
+ * build: the register still holds Bzint_mp5_x_R(i,j).  For the y face it
+ * holds the negated b coefficient, i.e. the b argument itself. */
+static int g_hllc_seed_mode = 1; /* 1 reference build, 0 clamped seed */
+void oracle_set_hllc_seed_mode(int mode) { g_hllc_seed_mode = mode; }
+
 static int hllc_x(const side_t *L, const side_t *R, double f[FX_N]) {
   double psi_h = hll_state(L->psi, R->psi, L->ex, R->ex);
   double phi_h = hll_state(L->phi, R->phi, L->bx, R->bx);
@@ -746,7 +756,7 @@ static int hllc_x(const side_t *L, const side_t *R, double f[FX_N]) {
   double a = tau_f - erx;
   double b = ((e2p + b2p) - tau_h) - sx_f;
   double c = sx_h - erx;
-  double v = hllc_contact(a, b, c);
+  double v = hllc_contact(a, b, c, g_hllc_seed_mode ? R->bz : NAN);
   if (v != v) return 7;
 
   double pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
@@ -852,7 +862,7 @@ static int hllc_y(const side_t *L, const side_t *R, double f[FY_N]) {
   double a = tau_f - ery;
   double b = ((e2p + b2p) - tau_h) - sy_f;
   double c = sy_h - ery;
-  double v = hllc_contact(a, b, c);
+  double v = hllc_contact(a, b, c, g_hllc_seed_mode ? -b : NAN);
   if (v != v) return 7;
 
   double pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;

@@ -72,7 +72,9 @@ void oracle_get_rhs(const oracle_state *s, int l, int which, double *out);
 
 /* Pointwise/linewise pieces, exported for unit tests. */
 void oracle_mp_line(int order, const double *u, int n, double *up, double *um, int positivity);
-double oracle_hllc_vel(double a, double b, double c, double rt);
+double oracle_hllc_vel(double a, double b, double c, double rt, double v0);
+/* 1: first contact-speed Newton point as in the reference build; 0: clamped seed */
+void oracle_set_hllc_seed_mode(int mode);
 int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags[3]);
 
 #ifdef __cplusplus
