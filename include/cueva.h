@@ -1,0 +1,172 @@
+/*
+ * cueva.h -- C ABI of the MI355X (gfx950) implementation of one CUEVA 2D
+ * resistive relativistic MHD time step.
+ *
+ * The reference drives one step through zero-argument Fortran subroutines
+ * that share module globals (main_cueva_code.f95:47-356).  Each entry point
+ * below replaces one of those subroutines; the state they share lives on the
+ * GPU inside a cueva_ctx.  A Fortran host binds them with iso_c_binding
+ * (mhd-kh-instability_amd/fortran/cueva_gpu.f95); Python binds them with
+ * ctypes (mhd-kh-instability_amd/cueva/abi.py).
+ *
+ * Conventions
+ *  - Every function returns int: CUEVA_OK (0) or a CUEVA_E_* status.
+ *  - Stage and substep indices are 1-based, as in the reference (l, m).
+ *  - Host field arrays use the reference layout (-6:imax+6, -6:jmax+6), x
+ *    fastest, ld = imax+13 doubles between rows; a set of 14 "prims" planes
+ *    is in the restart order psi phi Bx By Bz Vx Vy Vz Ex Ey Ez q rho p
+ *    (22_writer.f95:1116-1172).
+ *  - Calls on one ctx are ordered on the ctx's HIP stream; only
+ *    cueva_download*, cueva_c2p counters, cueva_reduce_globals and
+ *    cueva_sync wait for the device.  A ctx is not thread safe.
+ */
+#ifndef CUEVA_H
+#define CUEVA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUEVA_ABI_VERSION 1
+
+enum cueva_status {
+  CUEVA_OK = 0,
+  CUEVA_E_ARG = 1,           /* invalid argument or configuration           */
+  CUEVA_E_HIP = 2,           /* HIP runtime error                           */
+  CUEVA_E_NOMEM = 3,         /* device allocation failed                    */
+  CUEVA_E_HLLC_VEL = 7,      /* get_hllc_vel did not converge / NaN contact  */
+  CUEVA_E_HLLC_STAR = 8,     /* "Revise Vxstar velocity" (NaN contact speed) */
+  CUEVA_E_HLLE_NAN = 9,      /* HLLE wave speed is NaN                      */
+  CUEVA_E_C2P_OLDV = 20,     /* superluminal velocity entering Cardano       */
+  CUEVA_E_C2P_IMAG = 21,     /* no admissible resolvent root                 */
+  CUEVA_E_C2P_DET = 22,      /* det < 0 in the Cardano quadratic             */
+  CUEVA_E_C2P_NOROOT = 23,   /* both quartic roots below 1                   */
+  CUEVA_E_C2P_NAN = 24,      /* no root selected (NaN)                       */
+  CUEVA_E_C2P_SUPERLUM = 25, /* superluminal v and magnetisation < 10        */
+  CUEVA_E_C2P_DRIFT = 26     /* drift-velocity fix still superluminal        */
+};
+
+/* Boundary map per axis (the BOUND modes of 04_boundary.f95 are pairs of these:
+ * BOUND 1 = copy/copy, BOUND 2 = periodic/periodic, BOUND 6 = copy x / periodic y;
+ * the Kelvin-Helmholtz setup uses periodic x / copy y). */
+enum { CUEVA_BC_COPY = 0, CUEVA_BC_PERIODIC = 1 };
+
+/* Riemann solver + MPx reconstruction (parameters.f95:21 flux_solver 5/6/7). */
+enum { CUEVA_HLL = 5, CUEVA_HLLE = 6, CUEVA_HLLC = 7 };
+
+/* Arithmetic of the fluxes: plain FP64, or FP32 reconstruction/Riemann/flux
+ * difference with FP64 primitive recovery, E solve and sums. */
+enum { CUEVA_FP64 = 0, CUEVA_MIXED = 1 };
+
+/* Primitive recovery: Cardano quartic (12_varprimitive.f95), or a learned
+ * initial guess refined by <= 2 FP64 Newton steps with Cardano fallback. */
+enum { CUEVA_C2P_CARDANO = 0, CUEVA_C2P_LEARNED = 1 };
+
+/* Ghost-fill targets of cueva_boundary (04_boundary.f95, 24_boundary_conserved.f95,
+ * 25_boundary_electric.f95, 24_boundary_flux_conserved.f95). */
+enum { CUEVA_BND_PRIM = 0, CUEVA_BND_CONS = 1, CUEVA_BND_ELEC = 2, CUEVA_BND_FLUX = 3 };
+
+/* Plain-old-data mirror of the compile-time parameters the path reads
+ * (parameters.f95:10-334).  Fill with cueva_params_default, then override. */
+typedef struct cueva_params {
+  int32_t imax, jmax;     /* cells per axis (parameters.f95:110-111)           */
+  int32_t order;          /* 2 = IMEX SSP2(2,2,2), 4 = SSP3(4,3,3) (:10)        */
+  int32_t flux_solver;    /* CUEVA_HLL / CUEVA_HLLE / CUEVA_HLLC (:21)          */
+  int32_t rec_mp;         /* 0 MP5, 1 MP7, 2 MP9 (:33)                          */
+  int32_t bc_x, bc_y;     /* CUEVA_BC_* per axis                                */
+  int32_t precision;      /* CUEVA_FP64 / CUEVA_MIXED                           */
+  int32_t c2p_mode;       /* CUEVA_C2P_*                                        */
+  int32_t hllc_seed;      /* 1: first contact-speed Newton point as the flang
+                             build of the reference; 0: the clamped seed       */
+  int32_t reserved[6];
+  double longx, longy;    /* domain lengths Lx, Ly (:174-175)                   */
+  double cfl;             /* CFL number (:309)                                  */
+  double sigma;           /* conductivity sigma_0 (:196)                        */
+  double gamma;           /* adiabatic index (:148)                             */
+  double dt;              /* time step; <= 0 selects CFL*min(Delx,Dely,1) (:312) */
+  double reserved_d[6];
+} cueva_params;
+
+typedef struct cueva_ctx cueva_ctx;
+
+/* version / build information */
+int cueva_abi_version(void);
+
+/* Defaults: 128x256, SSP2(2,2,2), HLL+MP5, copy/periodic, FP64, Cardano. */
+int cueva_params_default(cueva_params *p);
+
+/* Grid spacings and time step as the reference derives them
+ * (parameters.f95:306-312, 325-334). out = {Delx, Dely, Delt, kappa_psi, kappa_phi}. */
+int cueva_derived(const cueva_params *p, double out[5]);
+
+/* Allocate device state for one grid on the current HIP device.
+ * Replaces the module arrays of scalar.f95 / threevectors.f95 / fourvectors.f95. */
+int cueva_create(const cueva_params *p, cueva_ctx **ctx);
+int cueva_destroy(cueva_ctx *ctx);
+/* Wait for all work queued on ctx. */
+int cueva_sync(cueva_ctx *ctx);
+/* Device bytes held by ctx. */
+int cueva_device_bytes(const cueva_ctx *ctx, int64_t *bytes);
+
+/* Host <-> device.  prims14 holds 14 planes of (imax+13)*(jmax+13) doubles.
+ * Upload replaces the RESTART read (03_restart_primitive.f95:50-61). */
+int cueva_upload_prims(cueva_ctx *ctx, const double *prims14);
+/* Download the 14 planes and, if cons5 != NULL, the evolved D tau Sx Sy Sz. */
+int cueva_download(cueva_ctx *ctx, double *prims14, double *cons5);
+
+/* ---- one entry point per reference subroutine of the step ---------------- */
+/* boundary / boundary_conserved / boundary_electric / boundary_flux_conserved
+ * (04_boundary.f95:1, 24_boundary_conserved.f95:2, 25_boundary_electric.f95:1,
+ *  24_boundary_flux_conserved.f95:2); kind = CUEVA_BND_*, l = stage (1-based). */
+int cueva_boundary(cueva_ctx *ctx, int kind, int l);
+int cueva_startconvar(cueva_ctx *ctx);             /* 06_conserved_var.f95:5   */
+int cueva_cleanvarint(cueva_ctx *ctx);             /* 07_clean_var.f95:5       */
+int cueva_swap1(cueva_ctx *ctx);                   /* 08_swap1.f95:2           */
+int cueva_sumint(cueva_ctx *ctx, int l, int m);    /* 09_sumint.f95:5          */
+int cueva_varint(cueva_ctx *ctx, int l);           /* 10_varint.f95:5          */
+int cueva_electricvarint(cueva_ctx *ctx, int l);   /* 11_elecvarint.f95:3      */
+int cueva_flowinit(cueva_ctx *ctx, int l);         /* 14_flowinit.f95          */
+/* hll_flow_mpx / hlle_flow_mpx / hllc_flow_mpx per params.flux_solver
+ * (16_hll_flow_mpx.f95, 16_hlle_flow_mpx.f95, 16_hllc_flow_mpx.f95). */
+int cueva_flux(cueva_ctx *ctx, int l);
+int cueva_fluxandsource(cueva_ctx *ctx, int l);    /* 17_fluxandsource.f95:6   */
+int cueva_sumfin(cueva_ctx *ctx, int l);           /* 18_sumfin.f95:5          */
+int cueva_varfin(cueva_ctx *ctx);                  /* 19_varfin.f95:5          */
+int cueva_electricfield(cueva_ctx *ctx);           /* 20_electricfield.f95     */
+/* varprimitivecardano with l_swap = 1 (12_varprimitive.f95:6).  Waits for the
+ * device; counters = {contador_r, contador_p, contador_v} of this call;
+ * fallback = cells that fell back to Cardano in the learned mode (else 0). */
+int cueva_c2p(cueva_ctx *ctx, int l_swap, int32_t counters[3], int32_t *fallback);
+
+/* The whole step, fused into few kernels: same results bit for bit as the
+ * sequence boundary, startconvar, ..., varprimitivecardano, boundary. */
+int cueva_step(cueva_ctx *ctx, int nsteps);
+/* Fix-up counters and Cardano fallbacks accumulated by cueva_step since the
+ * last call (waits for the device). */
+int cueva_step_counters(cueva_ctx *ctx, int64_t counters[3], int64_t *fallback);
+
+/* FP64 global integrals of 22_writer.f95:821-1000 over i=0..imax, j=0..jmax,
+ * each times Delx*Dely: out = {psi, phi, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz,
+ * E2, B2, v2, rho, D, q, p, W}.  Waits for the device. */
+int cueva_reduce_globals(cueva_ctx *ctx, double out[19]);
+
+/* Largest signal speed max|v| + ... used for an adaptive dt (new; the reference
+ * uses the fixed dt of parameters.f95:312). Waits for the device. */
+int cueva_max_speed(cueva_ctx *ctx, double *vmax);
+
+/* Seeded Kelvin-Helmholtz initial data on the host (no reference counterpart:
+ * the reference has no such setup).  Periodic x in [-Lx/2, Lx/2], outflow y in
+ * [-Ly/2, Ly/2]; rho = p = 1, vx = v0 tanh(y/0.01) + U(-1e-4,1e-4) noise,
+ * vy = 0.01 sin(2 pi x / Lx) exp(-y^2/0.01), B = (bx0, 0, bz0), E = -v x B.
+ * rotated != 0 writes the same data in the frame x' = y, y' = -x (so that
+ * the reference's BOUND 6 applies) with imax' = jmax, jmax' = imax, i.e. the
+ * caller's imax/jmax are the rotated grid's. */
+int cueva_kh_init(int imax, int jmax, double lx, double ly, int rotated, double v0,
+                  double bx0, double bz0, uint64_t seed, double *prims14);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,478 @@
+// C ABI of the gfx950 CUEVA step (declarations and citations: include/cueva.h).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+
+#include "cueva.h"
+#include "cueva_kernels.hpp"
+
+using namespace cueva;
+
+struct cueva_ctx {
+  cueva_params prm;
+  Geom g;
+  Scheme S;
+  Planes P;
+  hipStream_t stream;
+  double* base;
+  int64_t bytes;
+  DevStatus* dst;
+  int rec_order;
+  int err;  // sticky host-side error
+};
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    hipError_t e_ = (x);                            \
+    if (e_ != hipSuccess) return CUEVA_E_HIP;       \
+  } while (0)
+
+static inline int is_pow2(double d) {
+  int e;
+  double m = frexp(d, &e);
+  return m == 0.5;
+}
+
+extern "C" int cueva_abi_version(void) { return CUEVA_ABI_VERSION; }
+
+extern "C" int cueva_params_default(cueva_params* p) {
+  if (!p) return CUEVA_E_ARG;
+  memset(p, 0, sizeof(*p));
+  p->imax = 128;
+  p->jmax = 256;
+  p->order = 2;
+  p->flux_solver = CUEVA_HLL;
+  p->rec_mp = 0;
+  p->bc_x = CUEVA_BC_PERIODIC;
+  p->bc_y = CUEVA_BC_COPY;
+  p->precision = CUEVA_FP64;
+  p->c2p_mode = CUEVA_C2P_CARDANO;
+  p->hllc_seed = 1;
+  p->longx = 1.0;
+  p->longy = 2.0;
+  p->cfl = 0.4;
+  p->sigma = 1.0e3;
+  p->gamma = 4.0 / 3.0;
+  p->dt = 0.0;
+  return CUEVA_OK;
+}
+
+// parameters.f95:306-334: Delx = Longx/real(imax) (real() is exact here),
+// Delt = CFL*min(Delx,Dely,Delz) with Delz = 1, kappa = 1/cp^2,
+// cp = sqrt(min(Delx,Dely)/alpha), alpha = 1.
+extern "C" int cueva_derived(const cueva_params* p, double out[5]) {
+  if (!p || !out) return CUEVA_E_ARG;
+  if (p->imax < 8 || p->jmax < 8 || p->imax > (1 << 24) || p->jmax > (1 << 24)) return CUEVA_E_ARG;
+  double dx = p->longx / (double)(float)p->imax;
+  double dy = p->longy / (double)(float)p->jmax;
+  double m = dy < dx ? dy : dx;
+  double dt = p->dt > 0.0 ? p->dt : p->cfl * (1.0 < m ? 1.0 : m);
+  double cp = sqrt(m / 1.0);
+  out[0] = dx;
+  out[1] = dy;
+  out[2] = dt;
+  out[3] = 1.0 / (cp * cp);
+  out[4] = 1.0 / (cp * cp);
+  return CUEVA_OK;
+}
+
+// Butcher tables of imex_scheme.f95:40-49 (SSP2(2,2,2)) and :280-291 (SSP3(4,3,3))
+static void imex_tables(int order, Scheme& S) {
+  memset(S.wb, 0, sizeof(S.wb));
+  memset(S.wbt, 0, sizeof(S.wbt));
+  memset(S.Ab, 0, sizeof(S.Ab));
+  memset(S.Abt, 0, sizeof(S.Abt));
+  if (order == 2) {
+    const double g = 1.0 - 1.0 / sqrt(2.0);  // parameters.f95:240
+    S.lmax = 2;
+    S.wb[0] = 0.5; S.wb[1] = 0.5;
+    S.wbt[0] = 0.5; S.wbt[1] = 0.5;
+    S.Ab[1][0] = 1.0;
+    S.Abt[0][0] = g;
+    S.Abt[1][0] = 1.0 - 2.0 * g;
+    S.Abt[1][1] = g;
+  } else {
+    const double a1 = 0.24169426078821, b1 = 0.06042356519705, e1 = 0.12915286960590;  // :242-244
+    S.lmax = 4;
+    S.wb[1] = 1.0 / 6.0; S.wb[2] = 1.0 / 6.0; S.wb[3] = 2.0 / 3.0;
+    S.wbt[1] = 1.0 / 6.0; S.wbt[2] = 1.0 / 6.0; S.wbt[3] = 2.0 / 3.0;
+    S.Ab[2][1] = 1.0;
+    S.Ab[3][1] = 0.25; S.Ab[3][2] = 0.25;
+    S.Abt[0][0] = a1;
+    S.Abt[1][0] = -1.0 * a1; S.Abt[1][1] = a1;
+    S.Abt[2][1] = 1.0 - a1; S.Abt[2][2] = a1;
+    S.Abt[3][0] = b1; S.Abt[3][1] = e1; S.Abt[3][2] = ((0.5 - b1) - e1) - a1; S.Abt[3][3] = a1;
+  }
+}
+
+static int check_params(const cueva_params* p) {
+  if (!p) return CUEVA_E_ARG;
+  if (p->imax < 8 || p->jmax < 8 || p->imax > 65536 || p->jmax > 65536) return CUEVA_E_ARG;
+  if (p->order != 2 && p->order != 4) return CUEVA_E_ARG;
+  if (p->flux_solver != CUEVA_HLL && p->flux_solver != CUEVA_HLLC && p->flux_solver != CUEVA_HLLE)
+    return CUEVA_E_ARG;
+  if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
+  if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
+  if (p->precision != CUEVA_FP64) return CUEVA_E_ARG;
+  if (p->c2p_mode != CUEVA_C2P_CARDANO) return CUEVA_E_ARG;
+  if (p->flux_solver == CUEVA_HLLE) return CUEVA_E_ARG;  // not built yet
+  if (!(p->longx > 0.0) || !(p->longy > 0.0) || !(p->cfl > 0.0) || !(p->sigma >= 0.0)) return CUEVA_E_ARG;
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
+  if (!out) return CUEVA_E_ARG;
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  cueva_ctx* c = new (std::nothrow) cueva_ctx;
+  if (!c) return CUEVA_E_NOMEM;
+  memset(c, 0, sizeof(*c));
+  c->prm = *p;
+  Geom& g = c->g;
+  g.imax = p->imax;
+  g.jmax = p->jmax;
+  g.pitch = ((p->imax + 16) + 7) / 8 * 8;
+  g.rows = p->jmax + 13;
+  g.plane = ((int64_t)g.pitch * g.rows + 63) / 64 * 64;
+  g.bcx = p->bc_x;
+  g.bcy = p->bc_y;
+  double d[5];
+  cueva_derived(p, d);
+  Scheme& S = c->S;
+  S.dx = d[0]; S.dy = d[1]; S.dt = d[2]; S.kpsi = d[3]; S.kphi = d[4];
+  S.inv_dx = 1.0 / S.dx; S.inv_dy = 1.0 / S.dy;
+  S.div_exact = is_pow2(S.dx) && is_pow2(S.dy);
+  S.sigma = p->sigma;
+  S.gamma = p->gamma;
+  S.flux_solver = p->flux_solver;
+  S.order = p->order;
+  S.hllc_seed_ref = p->hllc_seed;
+  imex_tables(p->order, S);
+  c->rec_order = 5 + 2 * p->rec_mp;
+
+  const int nplanes = P_N + C_N + S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N;
+  c->bytes = (int64_t)nplanes * g.plane * (int64_t)sizeof(double);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CUEVA_E_HIP; }
+  if (hipMalloc(&c->base, (size_t)c->bytes) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return CUEVA_E_NOMEM;
+  }
+  if (hipMalloc(&c->dst, sizeof(DevStatus)) != hipSuccess) {
+    hipFree(c->base);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return CUEVA_E_NOMEM;
+  }
+  double* q = c->base;
+  auto take = [&]() { double* r = q; q += g.plane; return r; };
+  for (int k = 0; k < P_N; k++) c->P.p[k] = take();
+  for (int k = 0; k < C_N; k++) c->P.c[k] = take();
+  for (int k = 0; k < S_N; k++) c->P.s[k] = take();
+  for (int k = 0; k < FX_N; k++) c->P.fx[k] = take();
+  for (int k = 0; k < FY_N; k++) c->P.fy[k] = take();
+  for (int l = 0; l < MAXL; l++)
+    for (int k = 0; k < R_N; k++) c->P.rhs[l][k] = take();
+  for (int k = 0; k < A_N; k++) c->P.ints[k] = take();
+  for (int k = 0; k < A_N; k++) c->P.fins[k] = take();
+  if (hipMemsetAsync(c->base, 0, (size_t)c->bytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    hipFree(c->dst);
+    hipFree(c->base);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return CUEVA_E_HIP;
+  }
+  *out = c;
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_destroy(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  hipStreamSynchronize(c->stream);
+  hipFree(c->dst);
+  hipFree(c->base);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_sync(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_device_bytes(const cueva_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return CUEVA_E_ARG;
+  *bytes = c->bytes;
+  return CUEVA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host <-> device
+// ---------------------------------------------------------------------------
+static int copy_planes_h2d(cueva_ctx* c, double* const* dst, int n, const double* host) {
+  const Geom& g = c->g;
+  const size_t hw = (size_t)(g.imax + 13), hh = (size_t)(g.jmax + 13);
+  for (int k = 0; k < n; k++) {
+    HIPCHK(hipMemcpy2DAsync(dst[k] + 2, (size_t)g.pitch * sizeof(double), host + (size_t)k * hw * hh,
+                            hw * sizeof(double), hw * sizeof(double), hh, hipMemcpyHostToDevice, c->stream));
+  }
+  return CUEVA_OK;
+}
+
+static int copy_planes_d2h(cueva_ctx* c, double* const* src, int n, double* host) {
+  const Geom& g = c->g;
+  const size_t hw = (size_t)(g.imax + 13), hh = (size_t)(g.jmax + 13);
+  for (int k = 0; k < n; k++) {
+    HIPCHK(hipMemcpy2DAsync(host + (size_t)k * hw * hh, hw * sizeof(double), src[k] + 2,
+                            (size_t)g.pitch * sizeof(double), hw * sizeof(double), hh, hipMemcpyDeviceToHost,
+                            c->stream));
+  }
+  return CUEVA_OK;
+}
+
+static dim3 interior_grid(const Geom& g, dim3 blk) {
+  return dim3((unsigned)((g.imax + 1 + blk.x - 1) / blk.x), (unsigned)((g.jmax + 1 + blk.y - 1) / blk.y));
+}
+
+static int launch_ok() { return hipGetLastError() == hipSuccess ? CUEVA_OK : CUEVA_E_HIP; }
+
+static int fill_ghosts(cueva_ctx* c, double* const* planes, int n) {
+  PlaneList pl;
+  memset(&pl, 0, sizeof(pl));
+  for (int start = 0; start < n; start += 20) {
+    pl.n = n - start < 20 ? n - start : 20;
+    for (int k = 0; k < pl.n; k++) pl.a[k] = planes[start + k];
+    int64_t cnt = ghost_count(c->g);
+    hipLaunchKernelGGL(k_fill_ghosts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, c->g, pl);
+    if (launch_ok()) return CUEVA_E_HIP;
+  }
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_upload_prims(cueva_ctx* c, const double* prims14) {
+  if (!c || !prims14) return CUEVA_E_ARG;
+  int rc = copy_planes_h2d(c, c->P.p, P_N, prims14);
+  if (rc) return rc;
+  return fill_ghosts(c, c->P.p, P_N);
+}
+
+extern "C" int cueva_download(cueva_ctx* c, double* prims14, double* cons5) {
+  if (!c) return CUEVA_E_ARG;
+  int rc;
+  if (prims14 && (rc = copy_planes_d2h(c, c->P.p, P_N, prims14))) return rc;
+  if (cons5 && (rc = copy_planes_d2h(c, c->P.c, 5, cons5))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one entry point per reference subroutine
+// ---------------------------------------------------------------------------
+static const dim3 BLK(64, 4);
+
+extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
+  if (!c) return CUEVA_E_ARG;
+  (void)l;
+  double* list[20];
+  int n = 0;
+  switch (kind) {
+    case CUEVA_BND_PRIM:
+      for (int k = 0; k < P_N; k++) list[n++] = c->P.p[k];
+      break;
+    case CUEVA_BND_CONS: {
+      static const int cons[11] = {S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ};
+      for (int k = 0; k < 11; k++) list[n++] = c->P.s[cons[k]];
+      break;
+    }
+    case CUEVA_BND_ELEC:
+      list[n++] = c->P.s[S_EX]; list[n++] = c->P.s[S_EY]; list[n++] = c->P.s[S_EZ];
+      break;
+    case CUEVA_BND_FLUX:
+      // guarded by flux_solver == 5 .or. 7 (24_boundary_flux_conserved.f95:13)
+      if (c->prm.flux_solver == CUEVA_HLLE) return CUEVA_OK;
+      list[n++] = c->P.s[S_JX]; list[n++] = c->P.s[S_JY]; list[n++] = c->P.s[S_JZ];
+      break;
+    default:
+      return CUEVA_E_ARG;
+  }
+  return fill_ghosts(c, list, n);
+}
+
+extern "C" int cueva_startconvar(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_startconvar, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S);
+  return launch_ok();
+}
+
+extern "C" int cueva_cleanvarint(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_cleanvarint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P);
+  if (launch_ok()) return CUEVA_E_HIP;
+  HIPCHK(hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream));  // contador_* = 0
+  return CUEVA_OK;
+}
+
+// swap1 copies the primitives into stage-1 arrays that only REC_PRIM=1 and
+// l_swap=2 read (08_swap1.f95); with the flags of this path it has no effect.
+extern "C" int cueva_swap1(cueva_ctx* c) { return c ? CUEVA_OK : CUEVA_E_ARG; }
+
+static int stage_ok(const cueva_ctx* c, int l) { return c && l >= 1 && l <= c->S.lmax; }
+
+extern "C" int cueva_sumint(cueva_ctx* c, int l, int m) {
+  if (!stage_ok(c, l) || m < 1 || m >= l) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_sumint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1, m - 1);
+  return launch_ok();
+}
+
+extern "C" int cueva_varint(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_varint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S);
+  if (launch_ok()) return CUEVA_E_HIP;
+  return cueva_boundary(c, CUEVA_BND_CONS, l);  // 10_varint.f95:72
+}
+
+extern "C" int cueva_electricvarint(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_electricvarint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1);
+  if (launch_ok()) return CUEVA_E_HIP;
+  return cueva_boundary(c, CUEVA_BND_ELEC, l);  // 11_elecvarint.f95:402
+}
+
+extern "C" int cueva_flowinit(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_flowinit, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S);
+  return launch_ok();
+}
+
+template <int ORD>
+static int launch_faces_ord(cueva_ctx* c) {
+  dim3 grid((unsigned)((c->g.imax + 2 + BLK.x - 1) / BLK.x), (unsigned)((c->g.jmax + 2 + BLK.y - 1) / BLK.y));
+  if (c->prm.flux_solver == CUEVA_HLLC)
+    hipLaunchKernelGGL((k_faces<ORD, 7>), grid, BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+  else
+    hipLaunchKernelGGL((k_faces<ORD, 5>), grid, BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+  return launch_ok();
+}
+
+static int launch_faces(cueva_ctx* c) {
+  switch (c->rec_order) {
+    case 5: return launch_faces_ord<5>(c);
+    case 7: return launch_faces_ord<7>(c);
+    default: return launch_faces_ord<9>(c);
+  }
+}
+
+extern "C" int cueva_flux(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  // the MPx solvers refresh the stage ghosts first (16_hll_flow_mpx.f95:739-741)
+  int rc;
+  if ((rc = cueva_boundary(c, CUEVA_BND_CONS, l))) return rc;
+  if ((rc = cueva_boundary(c, CUEVA_BND_ELEC, l))) return rc;
+  if ((rc = cueva_boundary(c, CUEVA_BND_FLUX, l))) return rc;
+  return launch_faces(c);
+}
+
+extern "C" int cueva_fluxandsource(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_fluxandsource, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1);
+  return launch_ok();
+}
+
+extern "C" int cueva_sumfin(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_sumfin, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1);
+  return launch_ok();
+}
+
+extern "C" int cueva_varfin(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_varfin, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, 0);
+  return launch_ok();
+}
+
+extern "C" int cueva_electricfield(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_varfin, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, 1);
+  return launch_ok();
+}
+
+static int read_status(cueva_ctx* c, DevStatus* h) {
+  HIPCHK(hipMemcpyAsync(h, c->dst, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t* fallback) {
+  if (!c || l_swap != 1) return CUEVA_E_ARG;
+  hipLaunchKernelGGL(k_c2p, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+  if (launch_ok()) return CUEVA_E_HIP;
+  DevStatus h;
+  int rc = read_status(c, &h);
+  if (rc) return rc;
+  if (counters) for (int k = 0; k < 3; k++) counters[k] = (int32_t)h.cnt[k];
+  if (fallback) *fallback = (int32_t)h.fallback;
+  return h.err ? h.err : CUEVA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused step
+// ---------------------------------------------------------------------------
+extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
+  if (!c || nsteps < 0) return CUEVA_E_ARG;
+  const Geom& g = c->g;
+  const int halo = (c->rec_order - 1) / 2 + 1;  // stencil radius K plus the face offset
+  for (int h = 0; h < nsteps; h++) {
+    for (int l = 0; l < c->S.lmax; l++) {
+      dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
+              (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
+      hipLaunchKernelGGL(k_stage_fused, gs, BLK, 0, c->stream, c->g, c->P, c->S, l, halo);
+      if (launch_ok()) return CUEVA_E_HIP;
+      int rc = launch_faces(c);
+      if (rc) return rc;
+      if (l < c->S.lmax - 1) {
+        hipLaunchKernelGGL(k_fluxandsource, interior_grid(g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l);
+      } else {
+        hipLaunchKernelGGL(k_final_fused, interior_grid(g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+      }
+      if (launch_ok()) return CUEVA_E_HIP;
+    }
+    int rc = fill_ghosts(c, c->P.p, P_N);
+    if (rc) return rc;
+  }
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_step_counters(cueva_ctx* c, int64_t counters[3], int64_t* fallback) {
+  if (!c) return CUEVA_E_ARG;
+  DevStatus h;
+  int rc = read_status(c, &h);
+  if (rc) return rc;
+  if (counters) for (int k = 0; k < 3; k++) counters[k] = (int64_t)h.cnt[k];
+  if (fallback) *fallback = (int64_t)h.fallback;
+  DevStatus z;
+  memset(&z, 0, sizeof(z));
+  z.err = h.err; z.err_i = h.err_i; z.err_j = h.err_j;
+  HIPCHK(hipMemcpyAsync(c->dst, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h.err ? h.err : CUEVA_OK;
+}
+
+extern "C" int cueva_reduce_globals(cueva_ctx* c, double out[19]) {
+  (void)c; (void)out;
+  return CUEVA_E_ARG;  // built in the reductions unit
+}
+
+extern "C" int cueva_max_speed(cueva_ctx* c, double* vmax) {
+  (void)c; (void)vmax;
+  return CUEVA_E_ARG;
+}

@@ -1,0 +1,493 @@
+// Kernels of the CUEVA step for gfx950 (FP64 path).
+//
+// Two ways through one step share these kernels:
+//  - one launch per reference subroutine (the drop-in entry points of
+//    include/cueva.h), with the reference's accumulators materialised;
+//  - cueva_step, which computes the stage state directly at every point the
+//    stencils read (ghosts through the boundary index map), forms the IMEX
+//    sums on the fly and fuses the last stage with varfin + primitive
+//    recovery.  Both apply the same operations in the same order.
+#pragma once
+
+#include "cueva_device.hpp"
+
+namespace cueva {
+
+// ---- field planes --------------------------------------------------------
+// primitive/EM planes, restart order (22_writer.f95:1116-1172)
+enum { P_PSI, P_PHI, P_BX, P_BY, P_BZ, P_VX, P_VY, P_VZ, P_EX, P_EY, P_EZ, P_Q, P_RHO, P_P, P_N };
+// startconvar outputs (06_conserved_var.f95)
+enum { C_D, C_TAU, C_SX, C_SY, C_SZ, C_W, C_N };
+// stage planes (the l-indexed arrays of fourvectors.f95, one stage at a time)
+enum { S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_EXA, S_EYA, S_EZA, S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ,
+       S_EX, S_EY, S_EZ, S_JX, S_JY, S_JZ, S_N };
+// accumulated fields of sumint/sumfin (09_sumint.f95, 18_sumfin.f95)
+enum { A_PSI, A_PHI, A_BX, A_BY, A_BZ, A_EX, A_EY, A_EZ, A_Q, A_D, A_TAU, A_SX, A_SY, A_SZ, A_N };
+// stage right-hand side: 14 flux divergences (A_* order) + 3 stiff sources
+enum { R_SEX = 14, R_SEY = 15, R_SEZ = 16, R_N = 17 };
+
+constexpr int MAXL = 4;
+
+struct Planes {
+  double* p[P_N];
+  double* c[C_N];
+  double* s[S_N];
+  double* fx[FX_N];
+  double* fy[FY_N];
+  double* rhs[MAXL][R_N];
+  double* ints[A_N];
+  double* fins[A_N];
+};
+
+struct Scheme {
+  double dx, dy, dt, sigma, kpsi, kphi, gamma;
+  double inv_dx, inv_dy;
+  int div_exact;  // dx, dy powers of two: x/d == x*(1/d)
+  int lmax, flux_solver, order;
+  double wb[MAXL], wbt[MAXL], Ab[MAXL][MAXL], Abt[MAXL][MAXL];
+  int hllc_seed_ref;
+};
+
+// status words in device memory
+struct DevStatus {
+  int err;          // first error code (0 none)
+  int err_i, err_j; // where
+  int pad;
+  unsigned long long cnt[3];  // contador_r, contador_p, contador_v
+  unsigned long long fallback;
+};
+
+__device__ inline void report(DevStatus* st, int code, int i, int j) {
+  if (atomicCAS(&st->err, 0, code) == 0) {
+    st->err_i = i;
+    st->err_j = j;
+  }
+}
+
+// U^n of accumulated field a (in A_* order) at offset k
+__device__ inline double un_value(const Planes& P, int a, int64_t k) {
+  switch (a) {
+    case A_PSI: return P.p[P_PSI][k];
+    case A_PHI: return P.p[P_PHI][k];
+    case A_BX: return P.p[P_BX][k];
+    case A_BY: return P.p[P_BY][k];
+    case A_BZ: return P.p[P_BZ][k];
+    case A_EX: return P.p[P_EX][k];
+    case A_EY: return P.p[P_EY][k];
+    case A_EZ: return P.p[P_EZ][k];
+    case A_Q: return P.p[P_Q][k];
+    case A_D: return P.c[C_D][k];
+    case A_TAU: return P.c[C_TAU][k];
+    case A_SX: return P.c[C_SX][k];
+    case A_SY: return P.c[C_SY][k];
+    default: return P.c[C_SZ][k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill: every ghost point copies the interior point of the index map
+// ---------------------------------------------------------------------------
+struct PlaneList {
+  double* a[20];
+  int n;
+};
+
+__global__ void k_fill_ghosts(Geom g, PlaneList pl) {
+  const int im = g.imax, jm = g.jmax;
+  const int64_t nb = 6LL * (im + 13);  // one horizontal band
+  const int64_t nv = 6LL * (jm + 1);   // one vertical band
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j;
+  if (t < nb) { j = -6 + (int)(t / (im + 13)); i = -6 + (int)(t % (im + 13)); }
+  else if (t < 2 * nb) { t -= nb; j = jm + 1 + (int)(t / (im + 13)); i = -6 + (int)(t % (im + 13)); }
+  else if (t < 2 * nb + nv) { t -= 2 * nb; i = -6 + (int)(t / (jm + 1)); j = (int)(t % (jm + 1)); }
+  else if (t < 2 * nb + 2 * nv) { t -= 2 * nb + nv; i = im + 1 + (int)(t / (jm + 1)); j = (int)(t % (jm + 1)); }
+  else return;
+  int64_t dst = g.at(i, j);
+  int64_t src = g.at(src_index(i, im, g.bcx), src_index(j, jm, g.bcy));
+  for (int k = 0; k < pl.n; k++) pl.a[k][dst] = pl.a[k][src];
+}
+
+inline int64_t ghost_count(const Geom& g) { return 12LL * (g.imax + 13) + 12LL * (g.jmax + 1); }
+
+// ---------------------------------------------------------------------------
+// pointwise kernels over the interior [0,imax] x [0,jmax]
+// ---------------------------------------------------------------------------
+#define INTERIOR_IJ                                         \
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;      \
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;      \
+  if (i > g.imax || j > g.jmax) return;                     \
+  const int64_t k = g.at(i, j);
+
+// startconvar, 06_conserved_var.f95:77-140
+__global__ void k_startconvar(Geom g, Planes P, Scheme S) {
+  INTERIOR_IJ
+  Cons c = prim_to_cons(S.gamma, P.p[P_VX][k], P.p[P_VY][k], P.p[P_VZ][k], P.p[P_EX][k], P.p[P_EY][k],
+                        P.p[P_EZ][k], P.p[P_BX][k], P.p[P_BY][k], P.p[P_BZ][k], P.p[P_RHO][k], P.p[P_P][k]);
+  P.c[C_D][k] = c.d; P.c[C_TAU][k] = c.tau;
+  P.c[C_SX][k] = c.sx; P.c[C_SY][k] = c.sy; P.c[C_SZ][k] = c.sz; P.c[C_W][k] = c.w;
+}
+
+// cleanvarint, 07_clean_var.f95:18-60
+__global__ void k_cleanvarint(Geom g, Planes P) {
+  INTERIOR_IJ
+  for (int a = 0; a < A_N; a++) { P.ints[a][k] = 0.0; P.fins[a][k] = 0.0; }
+}
+
+// sumint, 09_sumint.f95:18-61 (stage l, substep m; 0-based here)
+__global__ void k_sumint(Geom g, Planes P, Scheme S, int l, int m) {
+  INTERIOR_IJ
+  const double ab = S.Ab[l][m], abt = S.Abt[l][m];
+  for (int a = 0; a < A_N; a++) {
+    double f = P.rhs[m][a][k];
+    if (a >= A_EX && a <= A_EZ)
+      P.ints[a][k] = P.ints[a][k] + (ab * f - abt * P.rhs[m][R_SEX + (a - A_EX)][k]);
+    else
+      P.ints[a][k] = P.ints[a][k] + ab * f;
+  }
+}
+
+// varint, 10_varint.f95:18-63
+__global__ void k_varint(Geom g, Planes P, Scheme S) {
+  INTERIOR_IJ
+  static constexpr int dst[A_N] = {S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_EXA, S_EYA, S_EZA,
+                                   S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ};
+  for (int a = 0; a < A_N; a++) P.s[dst[a]][k] = un_value(P, a, k) + S.dt * P.ints[a][k];
+}
+
+// electricvarint (IMEX), 11_elecvarint.f95:279-328
+__global__ void k_electricvarint(Geom g, Planes P, Scheme S, int l) {
+  INTERIOR_IJ
+  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
+  double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
+  double ex, ey, ez;
+  e_solve(S.Abt[l][l], S.sigma, S.dt, w, vx, vy, vz, P.s[S_BX][k], P.s[S_BY][k], P.s[S_BZ][k],
+          P.s[S_EXA][k], P.s[S_EYA][k], P.s[S_EZA][k], ex, ey, ez);
+  P.s[S_EX][k] = ex; P.s[S_EY][k] = ey; P.s[S_EZ][k] = ez;
+}
+
+// flowinit (current density), 14_flowinit.f95:258-301
+__global__ void k_flowinit(Geom g, Planes P, Scheme S) {
+  INTERIOR_IJ
+  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
+  double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
+  double sx, sy, sz;
+  ohm_source(S.sigma, w, vx, vy, vz, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k], P.s[S_BY][k],
+             P.s[S_BZ][k], sx, sy, sz);
+  double q = P.s[S_Q][k];
+  P.s[S_JX][k] = sx + q * vx;
+  P.s[S_JY][k] = sy + q * vy;
+  P.s[S_JZ][k] = sz + q * vz;
+}
+
+// sumfin, 18_sumfin.f95:20-60
+__global__ void k_sumfin(Geom g, Planes P, Scheme S, int l) {
+  INTERIOR_IJ
+  const double wb = S.wb[l], wbt = S.wbt[l];
+  for (int a = 0; a < A_N; a++) {
+    double f = P.rhs[l][a][k];
+    if (a >= A_EX && a <= A_EZ)
+      P.fins[a][k] = P.fins[a][k] + (wb * f - wbt * P.rhs[l][R_SEX + (a - A_EX)][k]);
+    else
+      P.fins[a][k] = P.fins[a][k] + wb * f;
+  }
+}
+
+// varfin (19_varfin.f95:35-50) and electricfield (20_electricfield.f95:157-163);
+// which = 0: the 11 non-E fields, 1: E
+__global__ void k_varfin(Geom g, Planes P, Scheme S, int which) {
+  INTERIOR_IJ
+  for (int a = 0; a < A_N; a++) {
+    bool e = (a >= A_EX && a <= A_EZ);
+    if (e != (which == 1)) continue;
+    double* u;
+    switch (a) {
+      case A_PSI: u = P.p[P_PSI]; break;
+      case A_PHI: u = P.p[P_PHI]; break;
+      case A_BX: u = P.p[P_BX]; break;
+      case A_BY: u = P.p[P_BY]; break;
+      case A_BZ: u = P.p[P_BZ]; break;
+      case A_EX: u = P.p[P_EX]; break;
+      case A_EY: u = P.p[P_EY]; break;
+      case A_EZ: u = P.p[P_EZ]; break;
+      case A_Q: u = P.p[P_Q]; break;
+      case A_D: u = P.c[C_D]; break;
+      case A_TAU: u = P.c[C_TAU]; break;
+      case A_SX: u = P.c[C_SX]; break;
+      case A_SY: u = P.c[C_SY]; break;
+      default: u = P.c[C_SZ]; break;
+    }
+    u[k] = u[k] + S.dt * P.fins[a][k];
+  }
+}
+
+// primitive recovery over the interior; reads D tau S and E B of the new step
+__device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevStatus* st, int i, int j,
+                                 int64_t k) {
+  C2PIn in;
+  in.d = P.c[C_D][k]; in.tau = P.c[C_TAU][k];
+  in.sx = P.c[C_SX][k]; in.sy = P.c[C_SY][k]; in.sz = P.c[C_SZ][k];
+  in.ex = P.p[P_EX][k]; in.ey = P.p[P_EY][k]; in.ez = P.p[P_EZ][k];
+  in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
+  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
+  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
+  C2POut o;
+  int code = c2p_cardano_full(S.gamma, in, o);
+  if (code) { report(st, code, i, j); return; }
+  if (o.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
+  if (o.fix_p) atomicAdd(&st->cnt[1], 1ULL);
+  if (o.fix_v) atomicAdd(&st->cnt[2], 1ULL);
+  P.p[P_VX][k] = o.vx; P.p[P_VY][k] = o.vy; P.p[P_VZ][k] = o.vz;
+  P.p[P_RHO][k] = o.rho; P.p[P_P][k] = o.p;
+  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
+}
+
+__global__ void k_c2p(Geom g, Planes P, Scheme S, DevStatus* st) {
+  INTERIOR_IJ
+  c2p_store(g, P, S, st, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// faces: reconstruction + Riemann solve for the face pair (i,j), i.e. the x
+// face between (i,j),(i+1,j) and the y face between (i,j),(i,j+1), for
+// i in [-1,imax], j in [-1,jmax] (16_*_flow_mpx.f95 face loops)
+// ---------------------------------------------------------------------------
+template <int ORD>
+__device__ inline void face_rec(const double* __restrict__ a, int64_t c, int64_t st, double& L, double& R) {
+  constexpr int K = MPc<ORD>::K;
+  double v[2 * K + 2];
+#pragma unroll
+  for (int t = 0; t < 2 * K + 2; t++) v[t] = a[c + (int64_t)(t - K) * st];
+  mp_face<ORD>(v, L, R);
+}
+template <int ORD>
+__device__ inline void face_rec_pos(const double* __restrict__ a, int64_t c, int64_t st, double& L, double& R) {
+  constexpr int K = MPc<ORD>::K;
+  double v[2 * K + 2];
+#pragma unroll
+  for (int t = 0; t < 2 * K + 2; t++) v[t] = a[c + (int64_t)(t - K) * st];
+  mp_face_pos<ORD>(v, L, R);
+}
+
+// reconstruct every non-kinematic quantity of one face (slots 1-20, 26/27/63)
+template <int ORD, int DIR>
+__device__ inline void face_fields(const Planes& P, int64_t c, int64_t st, Side<double>& L, Side<double>& R) {
+  face_rec<ORD>(P.s[S_PSI], c, st, L.psi, R.psi);
+  face_rec<ORD>(P.s[S_PHI], c, st, L.phi, R.phi);
+  face_rec<ORD>(P.s[S_EX], c, st, L.ex, R.ex);
+  face_rec<ORD>(P.s[S_EY], c, st, L.ey, R.ey);
+  face_rec<ORD>(P.s[S_EZ], c, st, L.ez, R.ez);
+  face_rec<ORD>(P.s[S_BX], c, st, L.bx, R.bx);
+  face_rec<ORD>(P.s[S_BY], c, st, L.by, R.by);
+  face_rec<ORD>(P.s[S_BZ], c, st, L.bz, R.bz);
+  face_rec<ORD>(P.s[S_Q], c, st, L.q, R.q);
+  face_rec<ORD>(P.s[S_D], c, st, L.d, R.d);
+  face_rec<ORD>(P.s[S_TAU], c, st, L.tau, R.tau);
+  face_rec<ORD>(P.s[S_SX], c, st, L.sx, R.sx);
+  face_rec<ORD>(P.s[S_SY], c, st, L.sy, R.sy);
+  face_rec<ORD>(P.s[S_SZ], c, st, L.sz, R.sz);
+  face_rec<ORD>(P.s[DIR == 0 ? S_JX : S_JY], c, st, L.jn, R.jn);
+}
+
+template <int ORD, int SOLVER>
+__global__ void __launch_bounds__(256) k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
+  if (i > g.imax || j > g.jmax) return;
+  const int64_t c = g.at(i, j);
+  const int64_t sx = 1, sy = g.pitch;
+  Side<double> xl, xr, yl, yr;
+  // kinematics of both faces first: the fallbacks couple them
+  face_rec<ORD>(P.p[P_VX], c, sx, xl.vx, xr.vx);
+  face_rec<ORD>(P.p[P_VY], c, sx, xl.vy, xr.vy);
+  face_rec<ORD>(P.p[P_VZ], c, sx, xl.vz, xr.vz);
+  face_rec<ORD>(P.p[P_VX], c, sy, yl.vx, yr.vx);
+  face_rec<ORD>(P.p[P_VY], c, sy, yl.vy, yr.vy);
+  face_rec<ORD>(P.p[P_VZ], c, sy, yl.vz, yr.vz);
+  face_rec_pos<ORD>(P.p[P_P], c, sx, xl.p, xr.p);
+  face_rec_pos<ORD>(P.p[P_RHO], c, sx, xl.rho, xr.rho);
+  face_rec_pos<ORD>(P.p[P_P], c, sy, yl.p, yr.p);
+  face_rec_pos<ORD>(P.p[P_RHO], c, sy, yl.rho, yr.rho);
+  // superluminal fallback, 16_hll_flow_mpx.f95:1392-1461 (same in hllc/hlle)
+  double v2xl = (sq(xl.vx) + sq(xl.vy)) + sq(xl.vz);
+  double v2xr = (sq(xr.vx) + sq(xr.vy)) + sq(xr.vz);
+  double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
+  double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
+  if (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0) {
+    const int64_t cm = g.at(i - 1, j), cj = g.at(i, j - 1);
+    xl.vx = P.p[P_VX][c]; xl.vy = P.p[P_VY][c]; xl.vz = P.p[P_VZ][c];
+    xr.vx = P.p[P_VX][cm]; xr.vy = P.p[P_VY][cm]; xr.vz = P.p[P_VZ][cm];
+    yl.vx = P.p[P_VX][c]; yl.vy = P.p[P_VY][c]; yl.vz = P.p[P_VZ][c];
+    yr.vx = P.p[P_VX][cj]; yr.vy = P.p[P_VY][cj]; yr.vz = P.p[P_VZ][cj];
+  }
+  if (SOLVER == 7) {  // density fix-up, 16_hllc_flow_mpx.f95:2663-2693
+    double rc = 0.1 * P.p[P_RHO][c];
+    if (yl.rho < 0.0) yl.rho = rc;
+    else if (yr.rho < 0.0) yr.rho = rc;
+    else if (xl.rho < 0.0) xl.rho = rc;
+    else if (xr.rho < 0.0) xr.rho = rc;
+  }
+  // x face
+  {
+    face_fields<ORD, 0>(P, c, sx, xl, xr);
+    side_physics<0>(S.gamma, xl);
+    side_physics<0>(S.gamma, xr);
+    double f[FX_N];
+    if (SOLVER == 7) {
+      int code = hllc_x(xl, xr, f, S.hllc_seed_ref != 0);
+      if (code) report(st, code, i, j);
+    } else {
+      hll_x(xl, xr, f);
+    }
+#pragma unroll
+    for (int t = 0; t < FX_N; t++) P.fx[t][c] = f[t];
+  }
+  // y face
+  {
+    face_fields<ORD, 1>(P, c, sy, yl, yr);
+    side_physics<1>(S.gamma, yl);
+    side_physics<1>(S.gamma, yr);
+    double f[FY_N];
+    if (SOLVER == 7) {
+      int code = hllc_y(yl, yr, f, S.hllc_seed_ref != 0);
+      if (code) report(st, code, i, j);
+    } else {
+      hll_y(yl, yr, f);
+    }
+#pragma unroll
+    for (int t = 0; t < FY_N; t++) P.fy[t][c] = f[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flux differences and stiff sources, 17_fluxandsource.f95:442-542
+// ---------------------------------------------------------------------------
+struct Rhs { double r[R_N]; };
+
+__device__ inline double ddx(const Scheme& S, double a) { return S.div_exact ? a * S.inv_dx : a / S.dx; }
+__device__ inline double ddy(const Scheme& S, double a) { return S.div_exact ? a * S.inv_dy : a / S.dy; }
+
+__device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k) {
+  const int64_t kw = g.at(i - 1, j), ks = g.at(i, j - 1);
+#define DXF(a) ddx(S, P.fx[a][k] - P.fx[a][kw])
+#define DYF(a) ddy(S, P.fy[a][k] - P.fy[a][ks])
+  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
+  double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
+  double q = P.s[S_Q][k];
+  Rhs o;
+  o.r[0] = ((((-DXF(FX_EXPSI)) - DYF(FY_EYPSI)) - 0.0) + q) - S.kpsi * P.s[S_PSI][k];
+  o.r[1] = (((-DXF(FX_BXPHI)) - DYF(FY_BYPHI)) - 0.0) - S.kphi * P.s[S_PHI][k];
+  o.r[2] = ((-DYF(FY_EZBX)) + 0.0) - DXF(FX_PHIBX);
+  o.r[3] = ((-0.0) + DXF(FX_EZBY)) - DYF(FY_PHIBY);
+  o.r[4] = ((-DXF(FX_EYBZ)) + DYF(FY_EXBZ)) - 0.0;
+  o.r[5] = ((DYF(FY_BZEX) - 0.0) - DXF(FX_PSIEX)) - q * vx;
+  o.r[6] = ((0.0 - DXF(FX_BZEY)) - DYF(FY_PSIEY)) - q * vy;
+  o.r[7] = ((DXF(FX_BYEZ) - DYF(FY_BXEZ)) - 0.0) - q * vz;
+  o.r[8] = ((-DXF(FX_J)) - DYF(FY_J)) - 0.0;
+  o.r[9] = ((-DXF(FX_FD)) - DYF(FY_FD)) - 0.0;
+  o.r[10] = ((-DXF(FX_FTAU)) - DYF(FY_FTAU)) - 0.0;
+  o.r[11] = ((-DXF(FX_FSX)) - DYF(FY_FSX)) - 0.0;
+  o.r[12] = ((-DXF(FX_FSY)) - DYF(FY_FSY)) - 0.0;
+  o.r[13] = ((-DXF(FX_FSZ)) - DYF(FY_FSZ)) - 0.0;
+#undef DXF
+#undef DYF
+  ohm_source(S.sigma, w, vx, vy, vz, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k], P.s[S_BY][k],
+             P.s[S_BZ][k], o.r[R_SEX], o.r[R_SEY], o.r[R_SEZ]);
+  return o;
+}
+
+__global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l) {
+  INTERIOR_IJ
+  Rhs o = flux_source(g, P, S, i, j, k);
+#pragma unroll
+  for (int t = 0; t < R_N; t++) P.rhs[l][t][k] = o.r[t];
+}
+
+// ---------------------------------------------------------------------------
+// fused step pieces
+// ---------------------------------------------------------------------------
+
+// Stage state at every point the stencils read: the point's ghost map gives
+// the interior point whose values the reference's ghost fills copy.
+// Runs sumint (cumulative, as the reference never resets the sums within a
+// step), varint, electricvarint and flowinit at that interior point.
+__global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - halo;
+  const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - halo;
+  if (i > g.imax + halo || j > g.jmax + halo) return;
+  const int si = src_index(i, g.imax, g.bcx), sj = src_index(j, g.jmax, g.bcy);
+  const int64_t k = g.at(i, j), ks = g.at(si, sj);
+  const bool ghost = (si != i) || (sj != j);
+  double vx = P.p[P_VX][ks], vy = P.p[P_VY][ks], vz = P.p[P_VZ][ks];
+  double ex0 = P.p[P_EX][ks], ey0 = P.p[P_EY][ks], ez0 = P.p[P_EZ][ks];
+  double bx0 = P.p[P_BX][ks], by0 = P.p[P_BY][ks], bz0 = P.p[P_BZ][ks];
+  Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, P.p[P_RHO][ks], P.p[P_P][ks]);
+  double un[A_N] = {P.p[P_PSI][ks], P.p[P_PHI][ks], bx0, by0, bz0, ex0, ey0, ez0, P.p[P_Q][ks],
+                    cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
+  double stv[A_N];
+#pragma unroll
+  for (int a = 0; a < A_N; a++) {
+    double acc = 0.0;
+    for (int lp = 1; lp <= l; lp++)
+      for (int m = 0; m < lp; m++) {
+        double f = P.rhs[m][a][ks];
+        if (a >= A_EX && a <= A_EZ)
+          acc = acc + (S.Ab[lp][m] * f - S.Abt[lp][m] * P.rhs[m][R_SEX + (a - A_EX)][ks]);
+        else
+          acc = acc + S.Ab[lp][m] * f;
+      }
+    stv[a] = un[a] + S.dt * acc;
+  }
+  double ex, ey, ez;
+  e_solve(S.Abt[l][l], S.sigma, S.dt, cs.w, vx, vy, vz, stv[A_BX], stv[A_BY], stv[A_BZ], stv[A_EX], stv[A_EY],
+          stv[A_EZ], ex, ey, ez);
+  double jx, jy, jz;
+  ohm_source(S.sigma, cs.w, vx, vy, vz, ex, ey, ez, stv[A_BX], stv[A_BY], stv[A_BZ], jx, jy, jz);
+  jx = jx + stv[A_Q] * vx;
+  jy = jy + stv[A_Q] * vy;
+  jz = jz + stv[A_Q] * vz;
+  if (ghost && S.flux_solver == 6) jx = jy = jz = 0.0;  // 24_boundary_flux_conserved.f95:13
+  P.s[S_PSI][k] = stv[A_PSI]; P.s[S_PHI][k] = stv[A_PHI];
+  P.s[S_BX][k] = stv[A_BX]; P.s[S_BY][k] = stv[A_BY]; P.s[S_BZ][k] = stv[A_BZ];
+  P.s[S_EX][k] = ex; P.s[S_EY][k] = ey; P.s[S_EZ][k] = ez;
+  P.s[S_Q][k] = stv[A_Q]; P.s[S_D][k] = stv[A_D]; P.s[S_TAU][k] = stv[A_TAU];
+  P.s[S_SX][k] = stv[A_SX]; P.s[S_SY][k] = stv[A_SY]; P.s[S_SZ][k] = stv[A_SZ];
+  P.s[S_JX][k] = jx; P.s[S_JY][k] = jy; P.s[S_JZ][k] = jz;
+}
+
+// Last stage + sumfin + varfin + electricfield + varprimitivecardano.
+__global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
+  INTERIOR_IJ
+  const int L = S.lmax - 1;
+  Rhs last = flux_source(g, P, S, i, j, k);
+  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
+  double ex0 = P.p[P_EX][k], ey0 = P.p[P_EY][k], ez0 = P.p[P_EZ][k];
+  double bx0 = P.p[P_BX][k], by0 = P.p[P_BY][k], bz0 = P.p[P_BZ][k];
+  Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, P.p[P_RHO][k], P.p[P_P][k]);
+  double un[A_N] = {P.p[P_PSI][k], P.p[P_PHI][k], bx0, by0, bz0, ex0, ey0, ez0, P.p[P_Q][k],
+                    cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
+  double nu[A_N];
+#pragma unroll
+  for (int a = 0; a < A_N; a++) {
+    double acc = 0.0;
+    for (int l = 0; l <= L; l++) {
+      double f = l == L ? last.r[a] : P.rhs[l][a][k];
+      if (a >= A_EX && a <= A_EZ) {
+        double s = l == L ? last.r[R_SEX + (a - A_EX)] : P.rhs[l][R_SEX + (a - A_EX)][k];
+        acc = acc + (S.wb[l] * f - S.wbt[l] * s);
+      } else {
+        acc = acc + S.wb[l] * f;
+      }
+    }
+    nu[a] = un[a] + S.dt * acc;
+  }
+  P.p[P_PSI][k] = nu[A_PSI]; P.p[P_PHI][k] = nu[A_PHI];
+  P.p[P_BX][k] = nu[A_BX]; P.p[P_BY][k] = nu[A_BY]; P.p[P_BZ][k] = nu[A_BZ];
+  P.p[P_EX][k] = nu[A_EX]; P.p[P_EY][k] = nu[A_EY]; P.p[P_EZ][k] = nu[A_EZ];
+  P.p[P_Q][k] = nu[A_Q];
+  P.c[C_D][k] = nu[A_D]; P.c[C_TAU][k] = nu[A_TAU];
+  P.c[C_SX][k] = nu[A_SX]; P.c[C_SY][k] = nu[A_SY]; P.c[C_SZ][k] = nu[A_SZ];
+  c2p_store(g, P, S, st, i, j, k);
+}
+
+}  // namespace cueva

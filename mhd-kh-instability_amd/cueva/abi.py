@@ -1,0 +1,231 @@
+"""ctypes binding of the C ABI in include/cueva.h.
+
+This is the Python host side of the drop-in boundary: every method maps to
+one ``extern "C"`` entry point, which replaces one subroutine of the
+reference time step (main_cueva_code.f95:47-356).  Loading fails loudly when
+the HIP library has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libcueva.so")
+
+HLL, HLLE, HLLC = 5, 6, 7
+BC_COPY, BC_PERIODIC = 0, 1
+FP64, MIXED = 0, 1
+C2P_CARDANO, C2P_LEARNED = 0, 1
+BND_PRIM, BND_CONS, BND_ELEC, BND_FLUX = 0, 1, 2, 3
+
+# field order of the 14 host planes (restart order, 22_writer.f95:1116-1172)
+PRIM_NAMES = ("psi", "phi", "Bx", "By", "Bz", "Vx", "Vy", "Vz", "Ex", "Ey", "Ez", "q", "rho", "p")
+CONS_NAMES = ("D", "tau", "Sx", "Sy", "Sz")
+
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory",
+    7: "get_hllc_vel: no convergence", 8: "HLLC contact speed is NaN", 9: "HLLE wave speed is NaN",
+    20: "superluminal velocity entering Cardano", 21: "Cardano: no admissible resolvent root",
+    22: "Cardano: det < 0", 23: "Cardano: both roots below 1", 24: "Cardano: no root selected",
+    25: "superluminal velocity, magnetisation below 10", 26: "drift velocity still superluminal",
+}
+
+
+class CuevaError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__("cueva %s failed: %s (status %d)" % (where, STATUS.get(code, "?"), code))
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("imax", ctypes.c_int32), ("jmax", ctypes.c_int32), ("order", ctypes.c_int32),
+        ("flux_solver", ctypes.c_int32), ("rec_mp", ctypes.c_int32),
+        ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("c2p_mode", ctypes.c_int32), ("hllc_seed", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("longx", ctypes.c_double), ("longy", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("sigma", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
+        ("reserved_d", ctypes.c_double * 6),
+    ]
+
+
+# every symbol include/cueva.h declares, with (restype, argtypes)
+_P = ctypes.POINTER
+_c = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+SIGNATURES = {
+    "cueva_abi_version": (_i, []),
+    "cueva_params_default": (_i, [_P(Params)]),
+    "cueva_derived": (_i, [_P(Params), _P(_d)]),
+    "cueva_create": (_i, [_P(Params), _P(_c)]),
+    "cueva_destroy": (_i, [_c]),
+    "cueva_sync": (_i, [_c]),
+    "cueva_device_bytes": (_i, [_c, _P(ctypes.c_int64)]),
+    "cueva_upload_prims": (_i, [_c, _P(_d)]),
+    "cueva_download": (_i, [_c, _P(_d), _P(_d)]),
+    "cueva_boundary": (_i, [_c, _i, _i]),
+    "cueva_startconvar": (_i, [_c]),
+    "cueva_cleanvarint": (_i, [_c]),
+    "cueva_swap1": (_i, [_c]),
+    "cueva_sumint": (_i, [_c, _i, _i]),
+    "cueva_varint": (_i, [_c, _i]),
+    "cueva_electricvarint": (_i, [_c, _i]),
+    "cueva_flowinit": (_i, [_c, _i]),
+    "cueva_flux": (_i, [_c, _i]),
+    "cueva_fluxandsource": (_i, [_c, _i]),
+    "cueva_sumfin": (_i, [_c, _i]),
+    "cueva_varfin": (_i, [_c]),
+    "cueva_electricfield": (_i, [_c]),
+    "cueva_c2p": (_i, [_c, _i, _P(ctypes.c_int32), _P(ctypes.c_int32)]),
+    "cueva_step": (_i, [_c, _i]),
+    "cueva_step_counters": (_i, [_c, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    "cueva_reduce_globals": (_i, [_c, _P(_d)]),
+    "cueva_max_speed": (_i, [_c, _P(_d)]),
+    "cueva_kh_init": (_i, [_i, _i, _d, _d, _i, _d, _d, _d, ctypes.c_uint64, _P(_d)]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libcueva.so (built by cueva.build.build_hip)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libcueva.so not built (%s); run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(_d))
+
+
+def check(code, where):
+    if code != 0:
+        raise CuevaError(code, where)
+
+
+def default_params(**kw):
+    lib = load()
+    p = Params()
+    check(lib.cueva_params_default(ctypes.byref(p)), "params_default")
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def derived(p):
+    out = (ctypes.c_double * 5)()
+    check(load().cueva_derived(ctypes.byref(p), out), "derived")
+    return dict(zip(("dx", "dy", "dt", "kappa_psi", "kappa_phi"), list(out)))
+
+
+def plane_shape(p):
+    return (p.jmax + 13, p.imax + 13)
+
+
+def kh_init(imax, jmax, lx, ly, rotated, v0=0.5, bx0=0.01, bz0=0.0, seed=20261202):
+    """Seeded KH primitives, shape (14, jmax+13, imax+13) (cueva_kh_init)."""
+    out = np.zeros((14, jmax + 13, imax + 13))
+    check(load().cueva_kh_init(imax, jmax, lx, ly, int(rotated), v0, bx0, bz0, seed, _dp(out)), "kh_init")
+    return out
+
+
+class Solver:
+    """One device-resident grid (cueva_ctx)."""
+
+    def __init__(self, params):
+        self.lib = load()
+        self.p = params
+        self.ctx = ctypes.c_void_p()
+        check(self.lib.cueva_create(ctypes.byref(params), ctypes.byref(self.ctx)), "create")
+        self.shape = plane_shape(params)
+
+    def close(self):
+        if self.ctx:
+            self.lib.cueva_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- data
+    def upload(self, prims):
+        prims = np.ascontiguousarray(prims, dtype=np.float64)
+        assert prims.shape == (14,) + self.shape, prims.shape
+        check(self.lib.cueva_upload_prims(self.ctx, _dp(prims)), "upload_prims")
+
+    def download(self, cons=True):
+        prims = np.zeros((14,) + self.shape)
+        c5 = np.zeros((5,) + self.shape) if cons else None
+        check(self.lib.cueva_download(self.ctx, _dp(prims), _dp(c5) if cons else None), "download")
+        return (prims, c5) if cons else prims
+
+    def device_bytes(self):
+        b = ctypes.c_int64()
+        check(self.lib.cueva_device_bytes(self.ctx, ctypes.byref(b)), "device_bytes")
+        return b.value
+
+    def sync(self):
+        check(self.lib.cueva_sync(self.ctx), "sync")
+
+    # -- fused step
+    def step(self, n=1):
+        check(self.lib.cueva_step(self.ctx, int(n)), "step")
+
+    def step_counters(self):
+        c = (ctypes.c_int64 * 3)()
+        fb = ctypes.c_int64()
+        check(self.lib.cueva_step_counters(self.ctx, c, ctypes.byref(fb)), "step_counters")
+        return list(c), fb.value
+
+    # -- per-subroutine entry points, in the reference's call order
+    def step_by_routines(self):
+        """One step as the reference driver sequences it (main_cueva_code.f95:53-328);
+        returns the fix-up counters of the primitive recovery."""
+        L = self.lib
+        x = self.ctx
+        lmax = 2 if self.p.order == 2 else 4
+        check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
+        check(L.cueva_startconvar(x), "startconvar")
+        check(L.cueva_cleanvarint(x), "cleanvarint")
+        check(L.cueva_swap1(x), "swap1")
+        for l in range(1, lmax + 1):
+            for m in range(1, l):
+                check(L.cueva_sumint(x, l, m), "sumint")
+            check(L.cueva_varint(x, l), "varint")
+            check(L.cueva_electricvarint(x, l), "electricvarint")
+            check(L.cueva_boundary(x, BND_CONS, l), "boundary_conserved")
+            check(L.cueva_boundary(x, BND_ELEC, l), "boundary_electric")
+            check(L.cueva_flowinit(x, l), "flowinit")
+            check(L.cueva_boundary(x, BND_FLUX, l), "boundary_flux_conserved")
+            check(L.cueva_flux(x, l), "flux")
+            check(L.cueva_fluxandsource(x, l), "fluxandsource")
+            check(L.cueva_sumfin(x, l), "sumfin")
+        check(L.cueva_varfin(x), "varfin")
+        check(L.cueva_electricfield(x), "electricfield")
+        cnt = (ctypes.c_int32 * 3)()
+        fb = ctypes.c_int32()
+        check(L.cueva_c2p(x, 1, cnt, ctypes.byref(fb)), "varprimitivecardano")
+        check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
+        return list(cnt)

@@ -1,0 +1,71 @@
+"""Build steps of the package: the gfx950 HIP library and the Fortran driver.
+
+The library is built in-tree into ``mhd-kh-instability_amd/lib/libcueva.so``.
+Every HIP source is compiled with ``-ffp-contract=off`` (separately rounded
+multiply and add, as in the reference build) and with packed FP32 disabled.
+"""
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FC_CUEVA", "/opt/rocm/lib/llvm/bin/flang")
+ARCH = os.environ.get("CUEVA_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd, cwd=None):
+    res = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("build step failed: %s\n%s\n%s" % (" ".join(cmd), res.stdout[-4000:], res.stderr[-4000:]))
+    return res
+
+
+def build_hip(verbose=False):
+    """Compile libcueva.so (kernels, C ABI, host KH generator)."""
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcueva.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, "cueva.hip"), os.path.join(CSRC, "kh_init.cpp")]
+    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-I", INCLUDE, "-I", CSRC, "-o", tmp] + srcs
+    _run(cmd)
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
+def build_fortran(verbose=False):
+    """Compile the Fortran host driver against libcueva.so (needs flang)."""
+    fdir = os.path.join(PKG, "fortran")
+    if not os.path.isdir(fdir) or not os.path.exists(FC):
+        return None
+    out = os.path.join(LIB, "cueva_gpu_driver")
+    bdir = os.path.join(LIB, "_fbuild")
+    os.makedirs(bdir, exist_ok=True)
+    srcs = ["parameters_kh.f95", "cueva_gpu.f95", "cueva_shims.f95", "main_cueva_gpu.f95"]
+    objs = []
+    for s in srcs:
+        o = os.path.join(bdir, s.replace(".f95", ".o"))
+        _run([FC, "-O2", "-c", os.path.join(fdir, s), "-J", bdir, "-o", o])
+        objs.append(o)
+    _run([FC, "-O2"] + objs + ["-L", LIB, "-lcueva", "-Wl,-rpath,$ORIGIN", "-o", out])
+    if verbose:
+        print("built", out)
+    return out
+
+
+def build_all(verbose=False):
+    build_hip(verbose)
+    build_fortran(verbose)
