@@ -156,6 +156,25 @@ int cueva_reduce_globals(cueva_ctx *ctx, double out[19]);
  * uses the fixed dt of parameters.f95:312). Waits for the device. */
 int cueva_max_speed(cueva_ctx *ctx, double *vmax);
 
+/* Replace the conserved D tau Sx Sy Sz (5 planes, host layout) that the
+ * primitive recovery reads; with cueva_upload_prims this loads a restart
+ * state given in conserved form.  No reference counterpart (the reference
+ * restarts from primitives only). */
+int cueva_upload_cons(cueva_ctx *ctx, const double *cons5);
+
+/* Copy one device plane to the host (host layout).  group: 0 primitives
+ * (14, restart order), 1 D tau Sx Sy Sz W, 2 stage planes, 3 x-face fluxes,
+ * 4 y-face fluxes, 5+l stage-l right-hand sides (17).  For tests. */
+int cueva_download_plane(cueva_ctx *ctx, int group, int index, double *out);
+
+/* Kernel timing on the ctx stream: while on, every launch is bracketed by
+ * HIP events; cueva_kernel_times sums them per kernel class
+ * {stage, faces, fluxsource, final, ghosts, other} in ms, with launch counts. */
+int cueva_set_profiling(cueva_ctx *ctx, int on);
+int cueva_kernel_times(cueva_ctx *ctx, double ms[6], int64_t launches[6]);
+/* Milliseconds of nsteps fused steps timed with HIP events on the ctx stream. */
+int cueva_time_steps(cueva_ctx *ctx, int nsteps, double *ms);
+
 /* Seeded Kelvin-Helmholtz initial data on the host (no reference counterpart:
  * the reference has no such setup).  Periodic x in [-Lx/2, Lx/2], outflow y in
  * [-Ly/2, Ly/2]; rho = p = 1, vx = v0 tanh(y/0.01) + U(-1e-4,1e-4) noise,

@@ -24,7 +24,43 @@ struct cueva_ctx {
   DevStatus* dst;
   int rec_order;
   int err;  // sticky host-side error
+  // kernel timing (cueva_set_profiling)
+  int prof;
+  int nev, capev;
+  hipEvent_t* ev;  // pairs (start, stop)
+  int* evcls;
 };
+
+enum { K_STAGE = 0, K_FACES = 1, K_FLUXSRC = 2, K_FINAL = 3, K_GHOSTS = 4, K_OTHER = 5, K_NCLS = 6 };
+
+static void prof_begin(cueva_ctx* c, int cls) {
+  if (!c->prof) return;
+  if (c->nev == c->capev) {
+    int cap = c->capev ? 2 * c->capev : 256;
+    hipEvent_t* ev = (hipEvent_t*)realloc(c->ev, sizeof(hipEvent_t) * 2 * cap);
+    int* cl = (int*)realloc(c->evcls, sizeof(int) * cap);
+    if (!ev || !cl) { c->prof = 0; return; }
+    for (int k = 2 * c->capev; k < 2 * cap; k++) hipEventCreate(&ev[k]);
+    c->ev = ev;
+    c->evcls = cl;
+    c->capev = cap;
+  }
+  c->evcls[c->nev] = cls;
+  hipEventRecord(c->ev[2 * c->nev], c->stream);
+}
+
+static void prof_end(cueva_ctx* c) {
+  if (!c->prof) return;
+  hipEventRecord(c->ev[2 * c->nev + 1], c->stream);
+  c->nev++;
+}
+
+#define LAUNCH(cls, kern, grid, blk, ...)                                   \
+  do {                                                                    \
+    prof_begin(c, cls);                                                   \
+    hipLaunchKernelGGL(kern, grid, blk, 0, c->stream, __VA_ARGS__);       \
+    prof_end(c);                                                          \
+  } while (0)
 
 #define HIPCHK(x)                                   \
   do {                                              \
@@ -197,6 +233,9 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
 extern "C" int cueva_destroy(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
   hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 2 * c->capev; k++) hipEventDestroy(c->ev[k]);
+  free(c->ev);
+  free(c->evcls);
   hipFree(c->dst);
   hipFree(c->base);
   hipStreamDestroy(c->stream);
@@ -253,7 +292,7 @@ static int fill_ghosts(cueva_ctx* c, double* const* planes, int n) {
     pl.n = n - start < 20 ? n - start : 20;
     for (int k = 0; k < pl.n; k++) pl.a[k] = planes[start + k];
     int64_t cnt = ghost_count(c->g);
-    hipLaunchKernelGGL(k_fill_ghosts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, c->g, pl);
+    LAUNCH(K_GHOSTS, k_fill_ghosts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), c->g, pl);
     if (launch_ok()) return CUEVA_E_HIP;
   }
   return CUEVA_OK;
@@ -310,13 +349,13 @@ extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
 
 extern "C" int cueva_startconvar(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_startconvar, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S);
+  LAUNCH(K_OTHER, k_startconvar, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S);
   return launch_ok();
 }
 
 extern "C" int cueva_cleanvarint(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_cleanvarint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P);
+  LAUNCH(K_OTHER, k_cleanvarint, interior_grid(c->g, BLK), BLK, c->g, c->P);
   if (launch_ok()) return CUEVA_E_HIP;
   HIPCHK(hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream));  // contador_* = 0
   return CUEVA_OK;
@@ -330,27 +369,27 @@ static int stage_ok(const cueva_ctx* c, int l) { return c && l >= 1 && l <= c->S
 
 extern "C" int cueva_sumint(cueva_ctx* c, int l, int m) {
   if (!stage_ok(c, l) || m < 1 || m >= l) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_sumint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1, m - 1);
+  LAUNCH(K_OTHER, k_sumint, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1, m - 1);
   return launch_ok();
 }
 
 extern "C" int cueva_varint(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_varint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S);
+  LAUNCH(K_OTHER, k_varint, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S);
   if (launch_ok()) return CUEVA_E_HIP;
   return cueva_boundary(c, CUEVA_BND_CONS, l);  // 10_varint.f95:72
 }
 
 extern "C" int cueva_electricvarint(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_electricvarint, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1);
+  LAUNCH(K_OTHER, k_electricvarint, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
   if (launch_ok()) return CUEVA_E_HIP;
   return cueva_boundary(c, CUEVA_BND_ELEC, l);  // 11_elecvarint.f95:402
 }
 
 extern "C" int cueva_flowinit(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_flowinit, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S);
+  LAUNCH(K_OTHER, k_flowinit, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S);
   return launch_ok();
 }
 
@@ -358,9 +397,9 @@ template <int ORD>
 static int launch_faces_ord(cueva_ctx* c) {
   dim3 grid((unsigned)((c->g.imax + 2 + BLK.x - 1) / BLK.x), (unsigned)((c->g.jmax + 2 + BLK.y - 1) / BLK.y));
   if (c->prm.flux_solver == CUEVA_HLLC)
-    hipLaunchKernelGGL((k_faces<ORD, 7>), grid, BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, c->dst);
   else
-    hipLaunchKernelGGL((k_faces<ORD, 5>), grid, BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 5>), grid, BLK, c->g, c->P, c->S, c->dst);
   return launch_ok();
 }
 
@@ -384,25 +423,25 @@ extern "C" int cueva_flux(cueva_ctx* c, int l) {
 
 extern "C" int cueva_fluxandsource(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_fluxandsource, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1);
+  LAUNCH(K_FLUXSRC, k_fluxandsource, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
   return launch_ok();
 }
 
 extern "C" int cueva_sumfin(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_sumfin, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l - 1);
+  LAUNCH(K_OTHER, k_sumfin, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
   return launch_ok();
 }
 
 extern "C" int cueva_varfin(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_varfin, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, 0);
+  LAUNCH(K_OTHER, k_varfin, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, 0);
   return launch_ok();
 }
 
 extern "C" int cueva_electricfield(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_varfin, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, 1);
+  LAUNCH(K_OTHER, k_varfin, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, 1);
   return launch_ok();
 }
 
@@ -414,7 +453,7 @@ static int read_status(cueva_ctx* c, DevStatus* h) {
 
 extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t* fallback) {
   if (!c || l_swap != 1) return CUEVA_E_ARG;
-  hipLaunchKernelGGL(k_c2p, interior_grid(c->g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+  LAUNCH(K_OTHER, k_c2p, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
   if (launch_ok()) return CUEVA_E_HIP;
   DevStatus h;
   int rc = read_status(c, &h);
@@ -435,14 +474,14 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
     for (int l = 0; l < c->S.lmax; l++) {
       dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
               (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
-      hipLaunchKernelGGL(k_stage_fused, gs, BLK, 0, c->stream, c->g, c->P, c->S, l, halo);
+      LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo);
       if (launch_ok()) return CUEVA_E_HIP;
       int rc = launch_faces(c);
       if (rc) return rc;
       if (l < c->S.lmax - 1) {
-        hipLaunchKernelGGL(k_fluxandsource, interior_grid(g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, l);
+        LAUNCH(K_FLUXSRC, k_fluxandsource, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
       } else {
-        hipLaunchKernelGGL(k_final_fused, interior_grid(g, BLK), BLK, 0, c->stream, c->g, c->P, c->S, c->dst);
+        LAUNCH(K_FINAL, k_final_fused, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
       }
       if (launch_ok()) return CUEVA_E_HIP;
     }
@@ -465,6 +504,66 @@ extern "C" int cueva_step_counters(cueva_ctx* c, int64_t counters[3], int64_t* f
   HIPCHK(hipMemcpyAsync(c->dst, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return h.err ? h.err : CUEVA_OK;
+}
+
+extern "C" int cueva_upload_cons(cueva_ctx* c, const double* cons5) {
+  if (!c || !cons5) return CUEVA_E_ARG;
+  return copy_planes_h2d(c, c->P.c, 5, cons5);
+}
+
+extern "C" int cueva_download_plane(cueva_ctx* c, int group, int index, double* out) {
+  if (!c || !out || index < 0) return CUEVA_E_ARG;
+  double* src;
+  if (group == 0 && index < P_N) src = c->P.p[index];
+  else if (group == 1 && index < C_N) src = c->P.c[index];
+  else if (group == 2 && index < S_N) src = c->P.s[index];
+  else if (group == 3 && index < FX_N) src = c->P.fx[index];
+  else if (group == 4 && index < FY_N) src = c->P.fy[index];
+  else if (group >= 5 && group < 5 + MAXL && index < R_N) src = c->P.rhs[group - 5][index];
+  else return CUEVA_E_ARG;
+  int rc = copy_planes_d2h(c, &src, 1, out);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_set_profiling(cueva_ctx* c, int on) {
+  if (!c) return CUEVA_E_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->prof = on ? 1 : 0;
+  c->nev = 0;
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_kernel_times(cueva_ctx* c, double ms[6], int64_t launches[6]) {
+  if (!c || !ms || !launches) return CUEVA_E_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < K_NCLS; k++) { ms[k] = 0.0; launches[k] = 0; }
+  for (int e = 0; e < c->nev; e++) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[2 * e], c->ev[2 * e + 1]));
+    ms[c->evcls[e]] += (double)t;
+    launches[c->evcls[e]] += 1;
+  }
+  c->nev = 0;
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_time_steps(cueva_ctx* c, int nsteps, double* ms) {
+  if (!c || !ms || nsteps < 0) return CUEVA_E_ARG;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, c->stream));
+  int rc = cueva_step(c, nsteps);
+  HIPCHK(hipEventRecord(b, c->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *ms = (double)t;
+  return rc;
 }
 
 extern "C" int cueva_reduce_globals(cueva_ctx* c, double out[19]) {

@@ -84,7 +84,14 @@ SIGNATURES = {
     "cueva_reduce_globals": (_i, [_c, _P(_d)]),
     "cueva_max_speed": (_i, [_c, _P(_d)]),
     "cueva_kh_init": (_i, [_i, _i, _d, _d, _i, _d, _d, _d, ctypes.c_uint64, _P(_d)]),
+    "cueva_upload_cons": (_i, [_c, _P(_d)]),
+    "cueva_download_plane": (_i, [_c, _i, _i, _P(_d)]),
+    "cueva_set_profiling": (_i, [_c, _i]),
+    "cueva_kernel_times": (_i, [_c, _P(_d), _P(ctypes.c_int64)]),
+    "cueva_time_steps": (_i, [_c, _i, _P(_d)]),
 }
+
+KERNEL_CLASSES = ("stage", "faces", "fluxsource", "final", "ghosts", "other")
 
 _lib = None
 
@@ -188,6 +195,31 @@ class Solver:
 
     def sync(self):
         check(self.lib.cueva_sync(self.ctx), "sync")
+
+    def upload_cons(self, cons):
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        assert cons.shape == (5,) + self.shape, cons.shape
+        check(self.lib.cueva_upload_cons(self.ctx, _dp(cons)), "upload_cons")
+
+    def plane(self, group, index):
+        out = np.zeros(self.shape)
+        check(self.lib.cueva_download_plane(self.ctx, group, index, _dp(out)), "download_plane")
+        return out
+
+    # -- timing
+    def set_profiling(self, on):
+        check(self.lib.cueva_set_profiling(self.ctx, int(bool(on))), "set_profiling")
+
+    def kernel_times(self):
+        ms = (ctypes.c_double * 6)()
+        n = (ctypes.c_int64 * 6)()
+        check(self.lib.cueva_kernel_times(self.ctx, ms, n), "kernel_times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        check(self.lib.cueva_time_steps(self.ctx, int(n), ctypes.byref(ms)), "time_steps")
+        return ms.value
 
     # -- fused step
     def step(self, n=1):
