@@ -80,7 +80,9 @@ typedef struct cueva_params {
   int32_t c2p_mode;       /* CUEVA_C2P_*                                        */
   int32_t hllc_seed;      /* 1: first contact-speed Newton point as the flang
                              build of the reference; 0: the clamped seed       */
-  int32_t reserved[6];
+  int32_t crmath;         /* 1: correctly rounded acos/cos/pow in the Cardano
+                             step (double-double); 0: device math library      */
+  int32_t reserved[5];
   double longx, longy;    /* domain lengths Lx, Ly (:174-175)                   */
   double cfl;             /* CFL number (:309)                                  */
   double sigma;           /* conductivity sigma_0 (:196)                        */

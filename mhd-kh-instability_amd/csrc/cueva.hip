@@ -89,6 +89,7 @@ extern "C" int cueva_params_default(cueva_params* p) {
   p->precision = CUEVA_FP64;
   p->c2p_mode = CUEVA_C2P_CARDANO;
   p->hllc_seed = 1;
+  p->crmath = 1;
   p->longx = 1.0;
   p->longy = 2.0;
   p->cfl = 0.4;
@@ -189,6 +190,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   S.flux_solver = p->flux_solver;
   S.order = p->order;
   S.hllc_seed_ref = p->hllc_seed;
+  S.crmath = p->crmath;
   imex_tables(p->order, S);
   c->rec_order = 5 + 2 * p->rec_mp;
 

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cueva_crmath.hpp"
+
 namespace cueva {
 
 constexpr int GH = 6;  // ghost layers (scalar.f95:9)
@@ -584,7 +586,7 @@ __device__ inline Quartic c2p_quartic(double gamma, const C2PIn& in) {
 }
 
 // Ferrari-Cardano root (12_varprimitive.f95:297-500); returns 0 or a status.
-__device__ inline int c2p_cardano(const Quartic& qd, double& Wn) {
+__device__ inline int c2p_cardano(const Quartic& qd, double& Wn, bool crm) {
   const double pi = 3.14159265358979323846264338327950288;
   double d2 = qd.d2, d1 = qd.d1, d0 = qd.d0;
   double e = (20.0 * d2) / 8.0;
@@ -594,13 +596,18 @@ __device__ inline int c2p_cardano(const Quartic& qd, double& Wn) {
   double R1 = ((2.0 * pw3(e) - (9.0 * e) * f) + 27.0 * g) / 54.0;
   double s, t, det, z1;
   if (sq(R1) < pw3(Q1)) {
-    double T1 = acos(R1 / fabs(sqrt(pw3(Q1))));
+    double ac = R1 / fabs(sqrt(pw3(Q1)));
+    double T1 = crm ? cr::acos_cr(ac) : acos(ac);
     double sQ = fabs(sqrt(Q1));
-    double za = (-((2.0 * sQ) * cos(T1 / 3.0))) - e / 3.0;
+    double ta0 = T1 / 3.0, tb0 = (T1 + 2.0 * pi) / 3.0, tc0 = (T1 - 2.0 * pi) / 3.0;
+    double ca = crm ? cr::cos_cr(ta0) : cos(ta0);
+    double cb = crm ? cr::cos_cr(tb0) : cos(tb0);
+    double cc = crm ? cr::cos_cr(tc0) : cos(tc0);
+    double za = (-((2.0 * sQ) * ca)) - e / 3.0;
     double sa = d2 + 2.0 * za, ta = ((sq(za) + (2.0 * d2) * za) + sq(d2)) - d0;
-    double zb = (-((2.0 * sQ) * cos((T1 + 2.0 * pi) / 3.0))) - e / 3.0;
+    double zb = (-((2.0 * sQ) * cb)) - e / 3.0;
     double sb = d2 + 2.0 * zb, tb = ((sq(zb) + (2.0 * d2) * zb) + sq(d2)) - d0;
-    double zc = (-((2.0 * sQ) * cos((T1 - 2.0 * pi) / 3.0))) - e / 3.0;
+    double zc = (-((2.0 * sQ) * cc)) - e / 3.0;
     double sc = d2 + 2.0 * zc, tc = ((sq(zc) + (2.0 * d2) * zc) + sq(d2)) - d0;
     if (sa >= 0.0 && ta >= 0.0) { s = fabs(sqrt(sa)); t = fabs(sqrt(ta)); z1 = za; }
     else if (sb >= 0.0 && tb >= 0.0) { s = fabs(sqrt(sb)); t = fabs(sqrt(tb)); z1 = zb; }
@@ -610,7 +617,8 @@ __device__ inline int c2p_cardano(const Quartic& qd, double& Wn) {
     if (det < 0.0) return 22;
   } else {
     double xx1 = fabs(R1) + fabs(sqrt(sq(R1) - pw3(Q1)));
-    double AA = -(copysign(1.0, R1) * copysign(fabs(pow(xx1, 1.0 / 3.0)), xx1));
+    double rt3 = crm ? cr::pow_third_cr(xx1) : pow(xx1, 1.0 / 3.0);
+    double AA = -(copysign(1.0, R1) * copysign(fabs(rt3), xx1));
     double BB = AA != 0.0 ? Q1 / AA : 0.0;
     z1 = (AA + BB) - e / 3.0;
     s = fabs(sqrt(d2 + 2.0 * z1));
@@ -709,11 +717,11 @@ __device__ inline int c2p_finish(double gamma, const C2PIn& in, const Quartic& q
   return 0;
 }
 
-__device__ inline int c2p_cardano_full(double gamma, const C2PIn& in, C2POut& o) {
+__device__ inline int c2p_cardano_full(double gamma, const C2PIn& in, C2POut& o, bool crm) {
   Quartic qd = c2p_quartic(gamma, in);
   if (qd.v2old > 1.0) return 20;
   double Wn;
-  int st = c2p_cardano(qd, Wn);
+  int st = c2p_cardano(qd, Wn, crm);
   if (st) return st;
   return c2p_finish(gamma, in, qd, Wn, true, o);
 }

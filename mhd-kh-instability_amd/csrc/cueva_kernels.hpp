@@ -46,6 +46,7 @@ struct Scheme {
   int lmax, flux_solver, order;
   double wb[MAXL], wbt[MAXL], Ab[MAXL][MAXL], Abt[MAXL][MAXL];
   int hllc_seed_ref;
+  int crmath;
 };
 
 // status words in device memory
@@ -232,7 +233,7 @@ __device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevS
   in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
   in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
   C2POut o;
-  int code = c2p_cardano_full(S.gamma, in, o);
+  int code = c2p_cardano_full(S.gamma, in, o, S.crmath != 0);
   if (code) { report(st, code, i, j); return; }
   if (o.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
   if (o.fix_p) atomicAdd(&st->cnt[1], 1ULL);

@@ -43,7 +43,8 @@ class Params(ctypes.Structure):
         ("imax", ctypes.c_int32), ("jmax", ctypes.c_int32), ("order", ctypes.c_int32),
         ("flux_solver", ctypes.c_int32), ("rec_mp", ctypes.c_int32),
         ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("precision", ctypes.c_int32),
-        ("c2p_mode", ctypes.c_int32), ("hllc_seed", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6),
+        ("c2p_mode", ctypes.c_int32), ("hllc_seed", ctypes.c_int32), ("crmath", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
         ("longx", ctypes.c_double), ("longy", ctypes.c_double), ("cfl", ctypes.c_double),
         ("sigma", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
         ("reserved_d", ctypes.c_double * 6),

@@ -48,6 +48,7 @@ def load():
         lib.oracle_hllc_vel.argtypes = [ctypes.c_double] * 5
         lib.oracle_c2p_point.argtypes = [ctypes.c_double, _P, _P, ctypes.POINTER(ctypes.c_int)]
         lib.oracle_set_hllc_seed_mode.argtypes = [ctypes.c_int]
+        lib.oracle_set_libm_mode.argtypes = [ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -59,8 +60,10 @@ def _dp(a):
 class Oracle:
     """CPU restatement of the step for one grid; mirrors cueva.Solver's data calls."""
 
-    def __init__(self, params):
-        """params: a cueva.Params (the same parameters the HIP path gets)."""
+    def __init__(self, params, libm=None):
+        """params: a cueva.Params (the same parameters the HIP path gets).
+        libm: "c" (C library, as the reference build) or "cr" (correctly rounded);
+        default follows params.crmath."""
         lib = load()
         self.lib = lib
         self.c = Cfg()
@@ -71,6 +74,9 @@ class Oracle:
         if rc:
             raise ValueError("oracle config rejected (%d)" % rc)
         lib.oracle_set_hllc_seed_mode(int(params.hllc_seed))
+        if libm is None:
+            libm = "cr" if params.crmath else "c"
+        lib.oracle_set_libm_mode(1 if libm == "cr" else 0)
         self.s = lib.oracle_create(ctypes.byref(self.c))
         self.shape = (params.jmax + 13, params.imax + 13)
 

@@ -23,8 +23,22 @@
 #include "cueva_oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Elementary functions of the Cardano step (12_varprimitive.f95:342-434).
+ * Mode 0 calls the C library, as the reference build does (and so matches it
+ * bit for bit on the machine the fixtures were made on).  Mode 1 returns the
+ * correctly rounded result, computed in quad precision (libquadmath) and
+ * rounded once to double. */
+static int g_libm_mode = 0;
+void oracle_set_libm_mode(int mode) { g_libm_mode = mode; }
+static double o_acos(double x) { return g_libm_mode ? (double)acosq((__float128)x) : acos(x); }
+static double o_cos(double x) { return g_libm_mode ? (double)cosq((__float128)x) : cos(x); }
+static double o_pow(double x, double y) {
+  return g_libm_mode ? (double)powq((__float128)x, (__float128)y) : pow(x, y);
+}
 
 #define G 6
 #define NXS 27 /* x-sweep slots 1..27   (16_hll_flow_mpx.f95:784-917)  */
@@ -1088,13 +1102,13 @@ int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags
   double s, t, det, z1;
   const double pi = 3.14159265358979323846264338327950288;
   if (sq(R1) < pw3(Q1)) { /* :340-425 */
-    double T1 = acos(R1 / fabs(sqrt(pw3(Q1))));
+    double T1 = o_acos(R1 / fabs(sqrt(pw3(Q1))));
     double sQ = fabs(sqrt(Q1));
-    double za = (-((2.0 * sQ) * cos(T1 / 3.0))) - e / 3.0;
+    double za = (-((2.0 * sQ) * o_cos(T1 / 3.0))) - e / 3.0;
     double sa = d2 + 2.0 * za, ta = ((sq(za) + (2.0 * d2) * za) + sq(d2)) - d0;
-    double zb = (-((2.0 * sQ) * cos((T1 + 2.0 * pi) / 3.0))) - e / 3.0;
+    double zb = (-((2.0 * sQ) * o_cos((T1 + 2.0 * pi) / 3.0))) - e / 3.0;
     double sb = d2 + 2.0 * zb, tb = ((sq(zb) + (2.0 * d2) * zb) + sq(d2)) - d0;
-    double zc = (-((2.0 * sQ) * cos((T1 - 2.0 * pi) / 3.0))) - e / 3.0;
+    double zc = (-((2.0 * sQ) * o_cos((T1 - 2.0 * pi) / 3.0))) - e / 3.0;
     double sc = d2 + 2.0 * zc, tc = ((sq(zc) + (2.0 * d2) * zc) + sq(d2)) - d0;
     if (sa >= 0.0 && ta >= 0.0) { s = fabs(sqrt(sa)); t = fabs(sqrt(ta)); z1 = za; }
     else if (sb >= 0.0 && tb >= 0.0) { s = fabs(sqrt(sb)); t = fabs(sqrt(tb)); z1 = zb; }
@@ -1104,7 +1118,7 @@ int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags
     if (det < 0.0) return 22;
   } else { /* :427-452 */
     double xx1 = fabs(R1) + fabs(sqrt(sq(R1) - pw3(Q1)));
-    double AA = -(copysign(1.0, R1) * fsign(fabs(pow(xx1, 1.0 / 3.0)), xx1));
+    double AA = -(copysign(1.0, R1) * fsign(fabs(o_pow(xx1, 1.0 / 3.0)), xx1));
     double BB = AA != 0.0 ? Q1 / AA : 0.0;
     z1 = (AA + BB) - e / 3.0;
     s = fabs(sqrt(d2 + 2.0 * z1));

@@ -75,6 +75,8 @@ void oracle_mp_line(int order, const double *u, int n, double *up, double *um, i
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0);
 /* 1: first contact-speed Newton point as in the reference build; 0: clamped seed */
 void oracle_set_hllc_seed_mode(int mode);
+/* 0: acos/cos/pow from the C library (as the reference build); 1: correctly rounded */
+void oracle_set_libm_mode(int mode);
 int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags[3]);
 
 #ifdef __cplusplus

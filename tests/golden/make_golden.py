@@ -82,6 +82,20 @@ def main(only=None):
                 }
                 if d in full:
                     arrays["h%d" % d] = np.ascontiguousarray(inner)
+            # the reference's own sensitivity to the C library: the same binary
+            # with glibc's SSE2 code paths instead of its FMA ones
+            spread_dir = os.path.join(tmp, "nofma")
+            os.makedirs(spread_dir)
+            subprocess.run([exe, inp, str(nsteps), os.path.join(spread_dir, "out")] + [str(d) for d in dumps],
+                           capture_output=True, text=True,
+                           env=dict(os.environ, GLIBC_TUNABLES="glibc.cpu.hwcaps=-AVX2,-FMA"))
+            for d in dumps:
+                a = np.frombuffer(open(os.path.join(tmp, "out_h%05d.bin" % d), "rb").read()[12:], dtype="<f8")
+                b = np.frombuffer(open(os.path.join(spread_dir, "out_h%05d.bin" % d), "rb").read()[12:], dtype="<f8")
+                a = a.reshape(19, jmax + 13, imax + 13)[:, 6:6 + jmax + 1, 6:6 + imax + 1]
+                b = b.reshape(19, jmax + 13, imax + 13)[:, 6:6 + jmax + 1, 6:6 + imax + 1]
+                meta["dumps"][str(d)]["libm_spread"] = {
+                    f: float(np.abs(b[k] - a[k]).sum() / max(np.abs(a[k]).sum(), 1e-300)) for k, f in enumerate(FIELDS)}
             if stop is not None:
                 assert stopped, "%s: the reference was expected to stop" % name
                 assert not os.path.exists(os.path.join(tmp, "out_h%05d.bin" % dumps[-1]).replace(

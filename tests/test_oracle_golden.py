@@ -54,7 +54,7 @@ def run_case(name):
     meta, arrays = load_case(name)
     p, prims = problem(meta)
     assert digest(prims) == meta["init_sha256"], "initial-data generator changed"
-    o = Oracle(p)
+    o = Oracle(p, libm="c")  # the C library, as the reference build
     o.upload(prims)
     done = 0
     for d in sorted(int(k) for k in meta["dumps"]):
