@@ -157,7 +157,6 @@ static int check_params(const cueva_params* p) {
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
   if (p->precision != CUEVA_FP64) return CUEVA_E_ARG;
   if (p->c2p_mode != CUEVA_C2P_CARDANO) return CUEVA_E_ARG;
-  if (p->flux_solver == CUEVA_HLLE) return CUEVA_E_ARG;  // not built yet
   if (!(p->longx > 0.0) || !(p->longy > 0.0) || !(p->cfl > 0.0) || !(p->sigma >= 0.0)) return CUEVA_E_ARG;
   return CUEVA_OK;
 }
@@ -400,6 +399,8 @@ static int launch_faces_ord(cueva_ctx* c) {
   dim3 grid((unsigned)((c->g.imax + 2 + BLK.x - 1) / BLK.x), (unsigned)((c->g.jmax + 2 + BLK.y - 1) / BLK.y));
   if (c->prm.flux_solver == CUEVA_HLLC)
     LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, c->dst);
+  else if (c->prm.flux_solver == CUEVA_HLLE)
+    LAUNCH(K_FACES, (k_faces<ORD, 6>), grid, BLK, c->g, c->P, c->S, c->dst);
   else
     LAUNCH(K_FACES, (k_faces<ORD, 5>), grid, BLK, c->g, c->P, c->S, c->dst);
   return launch_ok();

@@ -184,10 +184,15 @@ template <typename T> struct Side {
 };
 
 // W, E^2+B^2, enthalpy and normal fluxes (16_hll_flow_mpx.f95:1463-1466,
-// 1522-1532, 1603-1617, 1627-1715).  DIR 0 = x face, 1 = y face.
-template <int DIR, typename T> __device__ inline void side_physics(T gam, Side<T>& o) {
-  T v2 = (sq(o.vx) + sq(o.vy)) + sq(o.vz);
-  o.w = T(1) / sqrt(T(1) - v2);
+// 1522-1532, 1603-1617, 1627-1715).  DIR 0 = x face, 1 = y face.  HLLE_W:
+// W as 16_hlle_flow_mpx.f95:2302-2305 forms it (squares subtracted one by one).
+template <int DIR, bool HLLE_W = false, typename T> __device__ inline void side_physics(T gam, Side<T>& o) {
+  if constexpr (HLLE_W) {
+    o.w = T(1) / sqrt(((T(1) - sq(o.vx)) - sq(o.vy)) - sq(o.vz));
+  } else {
+    T v2 = (sq(o.vx) + sq(o.vy)) + sq(o.vz);
+    o.w = T(1) / sqrt(T(1) - v2);
+  }
   o.e2b2 = ((((sq(o.ex) + sq(o.ey)) + sq(o.ez)) + sq(o.bx)) + sq(o.by)) + sq(o.bz);
   T eps = o.p / ((gam - T(1)) * o.rho);
   o.h = o.rho * (T(1) + eps) + o.p;
@@ -265,7 +270,8 @@ template <typename T> __device__ inline void hll_y(const Side<T>& L, const Side<
 // [-1,1].  The reference evaluates its first step at the iterate before
 // assigning it; v0 is that point (clamp(rt) in the seed mode).  Tolerances
 // are the reference's for double and scaled to the precision for float.
-template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, int& fail) {
+// vlast receives the last iterate before the final clamp.
+template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, int& fail, T& vlast) {
   const T minerr = sizeof(T) == 8 ? T(1.0e-12) : T(2.0e-6);
   T oldv = clampf(rt, T(-1), T(1));
   T v = v0;
@@ -277,20 +283,22 @@ template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, in
     T err = fab(oldv - v);
     oldv = v;
     iter = iter + 1;
-    if (iter >= 100) { fail = 1; return v; }
+    if (iter >= 100) { fail = 1; vlast = v; return v; }
     if (!(err > minerr)) break;
   }
+  vlast = v;
   return clampf(v, T(-1) + T(1.0e-3), T(1) - T(1.0e-3));
 }
 
 // contact speed, 16_hllc_flow_mpx.f95:3185-3334
 template <typename T> __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail) {
+  T vlast;
   T det = sq(b) - (T(4) * a) * c;
   T qr = -(T(0.5) * (b + csign(b) * fab(sqrt(det))));
   T rt2 = c / qr;
   if (det < T(0)) rt2 = c / b;
   T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
-  T v = hllc_vel(a, -b, c, rt2, v0, fail);
+  T v = hllc_vel(a, -b, c, rt2, v0, fail, vlast);
   if (fab(v) <= T(1.0e-9))
     v = T(0);
   else if (fab(v) > T(1))
@@ -493,6 +501,182 @@ template <typename T> __device__ inline int hllc_y(const Side<T>& L, const Side<
     f[FY_FSX] = T(0.5) * (fsxR + fsxL); f[FY_FSY] = T(0.5) * (fsyR + fsyL); f[FY_FSZ] = T(0.5) * (fszR + fszL);
   } else {
     return 8;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// HLLE, 16_hlle_flow_mpx.f95:2258-4618: the electromagnetic subsystem is HLL
+// with s = -1, +1; q, D, tau, S use relativistic sound-speed wave estimates
+// (sa, sb) and an HLLC-structured contact for D, tau, S.
+// ---------------------------------------------------------------------------
+
+// wave estimates of one side, 16_hlle_flow_mpx.f95:2715-2740: the root
+// holds -vn*(1-cs^2), the first power of the normal speed
+template <typename T> __device__ inline void hlle_lambda(T gam, const Side<T>& o, T vn, T& lm, T& lp) {
+  T cs = sqrt((gam * o.p) / o.h);
+  T cs2 = sq(cs);
+  T v2 = (sq(o.vx) + sq(o.vy)) + sq(o.vz);
+  T root = sqrt((T(1) - v2) * ((T(1) - v2 * cs2) - vn * (T(1) - cs2)));
+  T den = T(1) - cs2 * v2;
+  lp = (vn * (T(1) - cs2) + cs * root) / den;
+  lm = (vn * (T(1) - cs2) - cs * root) / den;
+}
+
+// min/max over the sides and the clamp to |s| < 1 (:2789-2818); 9 = NaN stop
+template <int DIR, typename T>
+__device__ inline int hlle_speeds(T gam, const Side<T>& L, const Side<T>& R, T& sa, T& sb) {
+  T lmL, lpL, lmR, lpR;
+  hlle_lambda(gam, L, DIR == 0 ? L.vx : L.vy, lmL, lpL);
+  hlle_lambda(gam, R, DIR == 0 ? R.vx : R.vy, lmR, lpR);
+  T a = lmL < lmR ? lmL : lmR;
+  T b = lpL > lpR ? lpL : lpR;
+  if (a != a || b != b) return 9;
+  const T e2 = T(1.0e-9);
+  if (fab(a) > T(1)) a = clampf(a, T(-1) + e2, T(1) - e2);
+  if (fab(b) > T(1)) b = clampf(b, T(-1) + e2, T(1) - e2);
+  sa = a;
+  sb = b;
+  return 0;
+}
+
+template <typename T> __device__ inline T hlle_state(T a, T b, T ul, T ur, T fl, T fr) {
+  return (((b * ur - a * ul) + fl) - fr) / (b - a);
+}
+template <typename T> __device__ inline T hlle_flux(T a, T b, T fl, T fr, T ul, T ur) {
+  return ((b * fl - a * fr) + (a * b) * (ur - ul)) / (b - a);
+}
+
+// (E x B)_y of the y face's HLL states: the first Newton point of the x
+// contact solve in the flang -O2 reference build (still in the register)
+template <typename T> __device__ inline T hlle_y_ery(const Side<T>& L, const Side<T>& R) {
+  T ez_h = hll_state(L.ez, R.ez, L.bx, R.bx);
+  T ex_h = hll_state(L.ex, R.ex, -L.bz, -R.bz);
+  T bz_h = hll_state(L.bz, R.bz, -L.ex, -R.ex);
+  T bx_h = hll_state(L.bx, R.bx, L.ez, R.ez);
+  return bx_h * ez_h - bz_h * ex_h;
+}
+
+// One HLLE face.  DIR 0: x face (16_hlle_flow_mpx.f95:2520-2862, 3225-3290,
+// 3336-3412, 3459-3540, 4021-4175); DIR 1: y face (:2560-2708, 2893-3040,
+// 3292-3334, 3413-3458, 3573-3654, 4413-4500).  vref: first Newton point
+// when seed_ref; vlast: the solve's last iterate (the y solve's first point).
+// A face whose waves all leave one side takes the reference's never-assigned
+// (zero) one-sided scalars (:4021-4033, 4143-4160, which only its 1D branch
+// sets at :222-475).
+template <int DIR, typename T>
+__device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f, bool seed_ref, T vref, T& vlast) {
+  T e[3], b[3];  // electromagnetic HLL states (x, y, z)
+  if constexpr (DIR == 0) {
+    e[0] = hll_state(L.ex, R.ex, L.psi, R.psi);
+    e[1] = hll_state(L.ey, R.ey, L.bz, R.bz);
+    e[2] = hll_state(L.ez, R.ez, -L.by, -R.by);
+    b[0] = hll_state(L.bx, R.bx, L.phi, R.phi);
+    b[1] = hll_state(L.by, R.by, -L.ez, -R.ez);
+    b[2] = hll_state(L.bz, R.bz, L.ey, R.ey);
+    f[FX_EXPSI] = hll_flux(L.ex, R.ex, L.psi, R.psi);
+    f[FX_BXPHI] = hll_flux(L.bx, R.bx, L.phi, R.phi);
+    f[FX_PHIBX] = hll_flux(L.phi, R.phi, L.bx, R.bx);
+    f[FX_EZBY] = -hll_flux(-L.ez, -R.ez, L.by, R.by);
+    f[FX_EYBZ] = hll_flux(L.ey, R.ey, L.bz, R.bz);
+    f[FX_PSIEX] = hll_flux(L.psi, R.psi, L.ex, R.ex);
+    f[FX_BZEY] = hll_flux(L.bz, R.bz, L.ey, R.ey);
+    f[FX_BYEZ] = -hll_flux(-L.by, -R.by, L.ez, R.ez);
+  } else {
+    e[1] = hll_state(L.ey, R.ey, L.psi, R.psi);
+    e[2] = hll_state(L.ez, R.ez, L.bx, R.bx);
+    e[0] = hll_state(L.ex, R.ex, -L.bz, -R.bz);
+    b[1] = hll_state(L.by, R.by, L.phi, R.phi);
+    b[2] = hll_state(L.bz, R.bz, -L.ex, -R.ex);
+    b[0] = hll_state(L.bx, R.bx, L.ez, R.ez);
+    f[FY_EYPSI] = hll_flux(L.ey, R.ey, L.psi, R.psi);
+    f[FY_BYPHI] = hll_flux(L.by, R.by, L.phi, R.phi);
+    f[FY_PHIBY] = hll_flux(L.phi, R.phi, L.by, R.by);
+    f[FY_EXBZ] = -hll_flux(-L.ex, -R.ex, L.bz, R.bz);
+    f[FY_EZBX] = hll_flux(L.ez, R.ez, L.bx, R.bx);
+    f[FY_PSIEY] = hll_flux(L.psi, R.psi, L.ey, R.ey);
+    f[FY_BXEZ] = hll_flux(L.bx, R.bx, L.ez, R.ez);
+    f[FY_BZEX] = -hll_flux(-L.bz, -R.bz, L.ex, R.ex);
+  }
+  T sa, sb;
+  int code = hlle_speeds<DIR>(gam, L, R, sa, sb);
+  if (code) return code;
+  constexpr int n = DIR, t1 = DIR == 0 ? 1 : 2, t2 = DIR == 0 ? 2 : 0;
+  T q_f = hlle_flux(sa, sb, L.jn, R.jn, L.q, R.q);
+  T dd_h = hlle_state(sa, sb, L.d, R.d, L.fd, R.fd);
+  T tau_h = hlle_state(sa, sb, L.tau, R.tau, L.ftau, R.ftau);
+  T s_h[3], s_f[3];
+  s_h[0] = hlle_state(sa, sb, L.sx, R.sx, L.fs1, R.fs1);
+  s_h[1] = hlle_state(sa, sb, L.sy, R.sy, L.fs2, R.fs2);
+  s_h[2] = hlle_state(sa, sb, L.sz, R.sz, L.fs3, R.fs3);
+  T dd_f = hlle_flux(sa, sb, L.fd, R.fd, L.d, R.d);
+  T tau_f = hlle_flux(sa, sb, L.ftau, R.ftau, L.tau, R.tau);
+  s_f[0] = hlle_flux(sa, sb, L.fs1, R.fs1, L.sx, R.sx);
+  s_f[1] = hlle_flux(sa, sb, L.fs2, R.fs2, L.sy, R.sy);
+  s_f[2] = hlle_flux(sa, sb, L.fs3, R.fs3, L.sz, R.sz);
+  T er[3];
+  er[0] = b[2] * e[1] - b[1] * e[2];
+  er[1] = b[0] * e[2] - b[2] * e[0];
+  er[2] = b[1] * e[0] - b[0] * e[1];
+  T e2b2s = T(0.5) * (((sq(e[0]) + sq(e[1])) + sq(e[2])) + ((sq(b[0]) + sq(b[1])) + sq(b[2])));
+  constexpr int p1 = DIR == 0 ? 1 : 0;  // the transverse pair is (y,z) or (x,z)
+  T ca = tau_f - er[n];
+  T cb = (((sq(e[p1]) + sq(e[2])) + (sq(b[p1]) + sq(b[2]))) - tau_h) - s_f[n];
+  T cc = s_h[n] - er[n];
+  int fail = 0;
+  T v;
+  {
+    T det = sq(cb) - (T(4) * ca) * cc;
+    T qr = -(T(0.5) * (cb + csign(cb) * fab(sqrt(det))));
+    T rt2 = cc / qr;
+    if (det < T(0)) rt2 = cc / cb;
+    T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
+    v = hllc_vel(ca, -cb, cc, rt2, v0, fail, vlast);
+    if (fab(v) <= T(1.0e-9))
+      v = T(0);
+    else if (fab(v) > T(1))
+      v = clampf(v, T(-1) + T(1.0e-6), T(1) - T(1.0e-6));
+  }
+  if (fail || v != v) return 7;
+  T pstr = (s_f[n] + (sq(e[n]) + sq(b[n]))) - (tau_f - er[n]) * v;
+  T sR[3], sL[3];
+  sR[n] = (sb * s_h[n] - tau_f * v) / (sb - v);
+  sL[n] = (-(sa * s_h[n]) + tau_f * v) / (v - sa);
+  sR[t1] = ((((sb * s_h[t1] - s_f[t1]) - er[t1] * v) - e[n] * e[t1]) - b[n] * b[t1]) / (sb - v);
+  sL[t1] = ((((-(sa * s_h[t1]) + s_f[t1]) + er[t1] * v) + e[n] * e[t1]) + b[n] * b[t1]) / (v - sa);
+  sR[t2] = ((((sb * s_h[t2] - s_f[t2]) - er[t2] * v) - e[n] * e[t2]) - b[n] * b[t2]) / (sb - v);
+  sL[t2] = ((((-(sa * s_h[t2]) + s_f[t2]) + er[t2] * v) + e[n] * e[t2]) + b[n] * b[t2]) / (v - sa);
+  T tauR = ((sR[n] - er[n]) / v - pstr) + e2b2s;
+  T tauL = ((sL[n] - er[n]) / v - pstr) + e2b2s;
+  T ddR = (sb * dd_h - dd_f) / (sb - v);
+  T ddL = (-(sa * dd_h) + dd_f) / (v - sa);
+  // flux slots J, FD, Ftau, FSx, FSy, FSz are contiguous in both enums
+  T* o = f + (DIR == 0 ? FX_J : FY_J);
+  if (sa >= T(0)) {
+    o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = T(0);
+  } else if (v == T(0)) {  // L branch with the HLL fluxes
+    o[0] = q_f; o[1] = dd_f; o[2] = tau_f; o[3] = s_f[0]; o[4] = s_f[1]; o[5] = s_f[2];
+  } else {
+    T den = sb - sa;
+    if (sa <= T(0) && T(0) <= v) {
+      T cl = sa * (sb - v);
+      o[0] = q_f;
+      o[1] = dd_f - (cl * (ddR - ddL)) / den;
+      o[2] = tau_f - (cl * (tauR - tauL)) / den;
+#pragma unroll
+      for (int k = 0; k < 3; k++) o[3 + k] = s_f[k] - (cl * (sR[k] - sL[k])) / den;
+    } else if (v <= T(0) && T(0) <= sb) {
+      T cr = sb * (v - sa);
+      o[0] = q_f;
+      o[1] = dd_f + (cr * (ddR - ddL)) / den;
+      o[2] = tau_f + (cr * (tauR - tauL)) / den;
+#pragma unroll
+      for (int k = 0; k < 3; k++) o[3 + k] = s_f[k] + (cr * (sR[k] - sL[k])) / den;
+    } else if (sb < T(0)) {
+      o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = T(0);
+    } else {
+      return 8;
+    }
   }
   return 0;
 }

@@ -309,17 +309,20 @@ __global__ void __launch_bounds__(256) k_faces(Geom g, Planes P, Scheme S, DevSt
   face_rec_pos<ORD>(P.p[P_RHO], c, sx, xl.rho, xr.rho);
   face_rec_pos<ORD>(P.p[P_P], c, sy, yl.p, yr.p);
   face_rec_pos<ORD>(P.p[P_RHO], c, sy, yl.rho, yr.rho);
-  // superluminal fallback, 16_hll_flow_mpx.f95:1392-1461 (same in hllc/hlle)
-  double v2xl = (sq(xl.vx) + sq(xl.vy)) + sq(xl.vz);
-  double v2xr = (sq(xr.vx) + sq(xr.vy)) + sq(xr.vz);
-  double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
-  double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
-  if (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0) {
-    const int64_t cm = g.at(i - 1, j), cj = g.at(i, j - 1);
-    xl.vx = P.p[P_VX][c]; xl.vy = P.p[P_VY][c]; xl.vz = P.p[P_VZ][c];
-    xr.vx = P.p[P_VX][cm]; xr.vy = P.p[P_VY][cm]; xr.vz = P.p[P_VZ][cm];
-    yl.vx = P.p[P_VX][c]; yl.vy = P.p[P_VY][c]; yl.vz = P.p[P_VZ][c];
-    yr.vx = P.p[P_VX][cj]; yr.vy = P.p[P_VY][cj]; yr.vz = P.p[P_VZ][cj];
+  // superluminal fallback, 16_hll_flow_mpx.f95:1392-1461 (same in hllc;
+  // hlle only prints, 16_hlle_flow_mpx.f95:2270-2300)
+  if (SOLVER != 6) {
+    double v2xl = (sq(xl.vx) + sq(xl.vy)) + sq(xl.vz);
+    double v2xr = (sq(xr.vx) + sq(xr.vy)) + sq(xr.vz);
+    double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
+    double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
+    if (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0) {
+      const int64_t cm = g.at(i - 1, j), cj = g.at(i, j - 1);
+      xl.vx = P.p[P_VX][c]; xl.vy = P.p[P_VY][c]; xl.vz = P.p[P_VZ][c];
+      xr.vx = P.p[P_VX][cm]; xr.vy = P.p[P_VY][cm]; xr.vz = P.p[P_VZ][cm];
+      yl.vx = P.p[P_VX][c]; yl.vy = P.p[P_VY][c]; yl.vz = P.p[P_VZ][c];
+      yr.vx = P.p[P_VX][cj]; yr.vy = P.p[P_VY][cj]; yr.vz = P.p[P_VZ][cj];
+    }
   }
   if (SOLVER == 7) {  // density fix-up, 16_hllc_flow_mpx.f95:2663-2693
     double rc = 0.1 * P.p[P_RHO][c];
@@ -328,14 +331,28 @@ __global__ void __launch_bounds__(256) k_faces(Geom g, Planes P, Scheme S, DevSt
     else if (xl.rho < 0.0) xl.rho = rc;
     else if (xr.rho < 0.0) xr.rho = rc;
   }
+  constexpr bool HW = SOLVER == 6;
+  // HLLE: first Newton point of the x contact solve in the reference build
+  // is the y face's (E x B)_y; that of the y solve is the x solve's last iterate
+  double seed_x = 0.0, seed_y = 0.0;
+  if (SOLVER == 6 && S.hllc_seed_ref) {
+    face_rec<ORD>(P.s[S_EX], c, sy, yl.ex, yr.ex);
+    face_rec<ORD>(P.s[S_EZ], c, sy, yl.ez, yr.ez);
+    face_rec<ORD>(P.s[S_BX], c, sy, yl.bx, yr.bx);
+    face_rec<ORD>(P.s[S_BZ], c, sy, yl.bz, yr.bz);
+    seed_x = hlle_y_ery(yl, yr);
+  }
   // x face
   {
     face_fields<ORD, 0>(P, c, sx, xl, xr);
-    side_physics<0>(S.gamma, xl);
-    side_physics<0>(S.gamma, xr);
+    side_physics<0, HW>(S.gamma, xl);
+    side_physics<0, HW>(S.gamma, xr);
     double f[FX_N];
     if (SOLVER == 7) {
       int code = hllc_x(xl, xr, f, S.hllc_seed_ref != 0);
+      if (code) report(st, code, i, j);
+    } else if (SOLVER == 6) {
+      int code = hlle_face<0>(S.gamma, xl, xr, f, S.hllc_seed_ref != 0, seed_x, seed_y);
       if (code) report(st, code, i, j);
     } else {
       hll_x(xl, xr, f);
@@ -346,11 +363,15 @@ __global__ void __launch_bounds__(256) k_faces(Geom g, Planes P, Scheme S, DevSt
   // y face
   {
     face_fields<ORD, 1>(P, c, sy, yl, yr);
-    side_physics<1>(S.gamma, yl);
-    side_physics<1>(S.gamma, yr);
+    side_physics<1, HW>(S.gamma, yl);
+    side_physics<1, HW>(S.gamma, yr);
     double f[FY_N];
     if (SOLVER == 7) {
       int code = hllc_y(yl, yr, f, S.hllc_seed_ref != 0);
+      if (code) report(st, code, i, j);
+    } else if (SOLVER == 6) {
+      double unused;
+      int code = hlle_face<1>(S.gamma, yl, yr, f, S.hllc_seed_ref != 0, seed_y, unused);
       if (code) report(st, code, i, j);
     } else {
       hll_y(yl, yr, f);

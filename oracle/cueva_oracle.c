@@ -608,9 +608,10 @@ static void load_y(const oracle_state *s, double *const *arr, size_t k, side_t *
 /* W, E^2+B^2, enthalpy and the normal physical fluxes of one face side:
  * 16_hll_flow_mpx.f95:1463-1466 (W), 1522-1532 (E2B2), 1603-1617 (eps, h),
  * 1627-1715 (FD, Ftau, FS) */
-static void side_physics(double gam, int dir, side_t *o) {
+static void side_physics(double gam, int dir, int hlle, side_t *o) {
   double v2 = (sq(o->vx) + sq(o->vy)) + sq(o->vz);
-  o->w = 1.0 / sqrt(1.0 - v2);
+  /* HLLE subtracts the squares one by one (16_hlle_flow_mpx.f95:2302-2305) */
+  o->w = hlle ? 1.0 / sqrt(((1.0 - sq(o->vx)) - sq(o->vy)) - sq(o->vz)) : 1.0 / sqrt(1.0 - v2);
   o->e2b2 = ((((sq(o->ex) + sq(o->ey)) + sq(o->ez)) + sq(o->bx)) + sq(o->by)) + sq(o->bz);
   double eps = o->p / ((gam - 1.0) * o->rho);
   o->h = o->rho * (1.0 + eps) + o->p;
@@ -683,6 +684,9 @@ static inline double clampf(double x, double lo, double hi) {
   return t < lo ? lo : t;
 }
 
+/* last Newton iterate of the latest hllc_vel call, before the final clamp */
+static double g_hllc_last_iterate = 0.0;
+
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
   double oldv = clampf(rt, -1.0, 1.0);
   double v = v0;
@@ -697,6 +701,7 @@ double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
     if (iter >= 100) return NAN; /* the reference stops here */
     if (!(err > 1.0e-12)) break;
   }
+  g_hllc_last_iterate = v;
   return clampf(v, -1.0 + 1.0e-3, 1.0 - 1.0e-3);
 }
 
@@ -940,7 +945,220 @@ static int hllc_y(const side_t *L, const side_t *R, double f[FY_N]) {
   return 0;
 }
 
-/* face loop: 16_hll_flow_mpx.f95:1358-1958, 16_hllc_flow_mpx.f95:2401-4612 */
+/* ------------------------------------------------------------------ */
+/* HLLE: 16_hlle_flow_mpx.f95:2258-4618.  The electromagnetic subsystem */
+/* is HLL with s = -1, +1; q, D, tau, S use relativistic sound-speed    */
+/* wave estimates and an HLLC-structured contact for D, tau, S.         */
+/* ------------------------------------------------------------------ */
+
+/* relativistic sound-speed wave estimates, 16_hlle_flow_mpx.f95:2715-2818
+ * (x) and :2893-2996 (y), including the reference's -vn*(1-cs^2) term
+ * under the root (a first power of the normal speed). */
+static void hlle_lambda(double gam, const side_t *o, double vn, double *lm, double *lp) {
+  double cs = sqrt((gam * o->p) / o->h);
+  double cs2 = sq(cs);
+  double v2 = (sq(o->vx) + sq(o->vy)) + sq(o->vz);
+  double root = sqrt((1.0 - v2) * ((1.0 - v2 * cs2) - vn * (1.0 - cs2)));
+  double den = 1.0 - cs2 * v2;
+  *lp = (vn * (1.0 - cs2) + cs * root) / den;
+  *lm = (vn * (1.0 - cs2) - cs * root) / den;
+}
+
+/* min/max of the side estimates and the clamp to |s| < 1; 9 = the NaN stop */
+static int hlle_speeds(double gam, const side_t *L, const side_t *R, int dir, double *s1, double *s2) {
+  double lmL, lpL, lmR, lpR;
+  hlle_lambda(gam, L, dir == 0 ? L->vx : L->vy, &lmL, &lpL);
+  hlle_lambda(gam, R, dir == 0 ? R->vx : R->vy, &lmR, &lpR);
+  double a = lmL < lmR ? lmL : lmR; /* min */
+  double b = lpL > lpR ? lpL : lpR; /* max */
+  if (a != a || b != b) return 9;
+  const double e2 = 1.0e-9;
+  if (fabs(a) > 1.0) a = clampf(a, -1.0 + e2, 1.0 - e2);
+  if (fabs(b) > 1.0) b = clampf(b, -1.0 + e2, 1.0 - e2);
+  *s1 = a;
+  *s2 = b;
+  return 0;
+}
+
+/* HLL state and flux with wave speeds a < b (16_hlle_flow_mpx.f95:2826-2862) */
+static inline double hlle_state(double a, double b, double ul, double ur, double fl, double fr) {
+  return (((b * ur - a * ul) + fl) - fr) / (b - a);
+}
+static inline double hlle_flux(double a, double b, double fl, double fr, double ul, double ur) {
+  return ((b * fl - a * fr) + (a * b) * (ur - ul)) / (b - a);
+}
+
+/* fluid fluxes of one HLLE face from its wave speeds (a, b), contact speed v,
+ * HLL states and fluxes and transverse terms.  n = normal index (0 x,
+ * 1 y) selects which momentum is normal.  16_hlle_flow_mpx.f95:3336-3412
+ * (x star states), 3459-3540 (star fluxes), 4021-4175 (x selection); y at
+ * :3413-3458, 3573-3654, 4413-4500. */
+typedef struct {
+  double q_f, dd_h, dd_f, tau_h, tau_f, s_h[3], s_f[3];
+  double er[3], e[3], b[3], e2b2s;
+} hlle_face_t;
+
+static int hlle_fluid(const hlle_face_t *F, int n, double a, double b, double v, double out[6]) {
+  int t1 = n == 0 ? 1 : 2, t2 = n == 0 ? 2 : 0; /* transverse momenta in reference order */
+  double pstr = (F->s_f[n] + (sq(F->e[n]) + sq(F->b[n]))) - (F->tau_f - F->er[n]) * v;
+  double snR = (b * F->s_h[n] - F->tau_f * v) / (b - v);
+  double snL = (-(a * F->s_h[n]) + F->tau_f * v) / (v - a);
+  double s1R = ((((b * F->s_h[t1] - F->s_f[t1]) - F->er[t1] * v) - F->e[n] * F->e[t1]) - F->b[n] * F->b[t1]) / (b - v);
+  double s1L = ((((-(a * F->s_h[t1]) + F->s_f[t1]) + F->er[t1] * v) + F->e[n] * F->e[t1]) + F->b[n] * F->b[t1]) / (v - a);
+  double s2R = ((((b * F->s_h[t2] - F->s_f[t2]) - F->er[t2] * v) - F->e[n] * F->e[t2]) - F->b[n] * F->b[t2]) / (b - v);
+  double s2L = ((((-(a * F->s_h[t2]) + F->s_f[t2]) + F->er[t2] * v) + F->e[n] * F->e[t2]) + F->b[n] * F->b[t2]) / (v - a);
+  double tauR = ((snR - F->er[n]) / v - pstr) + F->e2b2s;
+  double tauL = ((snL - F->er[n]) / v - pstr) + F->e2b2s;
+  double ddR = (b * F->dd_h - F->dd_f) / (b - v);
+  double ddL = (-(a * F->dd_h) + F->dd_f) / (v - a);
+  double sR[3], sL[3];
+  sR[n] = snR; sL[n] = snL; sR[t1] = s1R; sL[t1] = s1L; sR[t2] = s2R; sL[t2] = s2L;
+  double fdR, fdL, ftR, ftL, fsR[3], fsL[3];
+  if (v == 0.0) {
+    fdR = fdL = F->dd_f; ftR = ftL = F->tau_f;
+    for (int k = 0; k < 3; k++) fsR[k] = fsL[k] = F->s_f[k];
+  } else {
+    double cr = b * (v - a), cl = a * (b - v), den = b - a;
+    fdR = F->dd_f + (cr * (ddR - ddL)) / den;
+    fdL = F->dd_f - (cl * (ddR - ddL)) / den;
+    ftR = F->tau_f + (cr * (tauR - tauL)) / den;
+    ftL = F->tau_f - (cl * (tauR - tauL)) / den;
+    for (int k = 0; k < 3; k++) {
+      fsR[k] = F->s_f[k] + (cr * (sR[k] - sL[k])) / den;
+      fsL[k] = F->s_f[k] - (cl * (sR[k] - sL[k])) / den;
+    }
+  }
+  /* out = {J, FD, Ftau, FS x, FS y, FS z}.  For a face with both waves on
+   * one side (a >= 0 or b < 0) the reference selects the scalars Jxint_L,
+   * FDxint_L, ... (:4021-4033, 4143-4160), which only its 1D branch assigns
+   * (:222-475); in 2D they keep their zero initial value. */
+  if (a >= 0.0 || (!(a <= 0.0 && 0.0 <= v) && !(v <= 0.0 && 0.0 <= b) && b < 0.0)) {
+    for (int k = 0; k < 6; k++) out[k] = 0.0;
+  } else if (a <= 0.0 && 0.0 <= v) {
+    out[0] = F->q_f; out[1] = fdL; out[2] = ftL; out[3] = fsL[0]; out[4] = fsL[1]; out[5] = fsL[2];
+  } else if (v <= 0.0 && 0.0 <= b) {
+    out[0] = F->q_f; out[1] = fdR; out[2] = ftR; out[3] = fsR[0]; out[4] = fsR[1]; out[5] = fsR[2];
+  } else {
+    return 8; /* "Revise Vxstar velocity" stop */
+  }
+  return 0;
+}
+
+/* HLLE x and y faces of one (i,j).  The first Newton point of the x contact
+ * solve in the flang -O2 reference build is the y face's (E x B)_y of the
+ * HLL states (still in the register); that of the y solve is the x solve's
+ * last iterate.  16_hlle_flow_mpx.f95:2263-4618. */
+static int hlle_xy(double gam, const side_t *xl, const side_t *xr, const side_t *yl, const side_t *yr,
+                   double fx[FX_N], double fy[FY_N]) {
+  /* electromagnetic HLL states, s = -1, +1 (:2520-2585) */
+  double xpsi_h = hll_state(xl->psi, xr->psi, xl->ex, xr->ex);
+  double xphi_h = hll_state(xl->phi, xr->phi, xl->bx, xr->bx);
+  double xex_h = hll_state(xl->ex, xr->ex, xl->psi, xr->psi);
+  double xey_h = hll_state(xl->ey, xr->ey, xl->bz, xr->bz);
+  double xez_h = hll_state(xl->ez, xr->ez, xl->bym, xr->bym);
+  double xbx_h = hll_state(xl->bx, xr->bx, xl->phi, xr->phi);
+  double xby_h = hll_state(xl->by, xr->by, xl->ezm, xr->ezm);
+  double xbz_h = hll_state(xl->bz, xr->bz, xl->ey, xr->ey);
+  double ypsi_h = hll_state(yl->psi, yr->psi, yl->ey, yr->ey);
+  double yphi_h = hll_state(yl->phi, yr->phi, yl->by, yr->by);
+  double yey_h = hll_state(yl->ey, yr->ey, yl->psi, yr->psi);
+  double yez_h = hll_state(yl->ez, yr->ez, yl->bx, yr->bx);
+  double yex_h = hll_state(yl->ex, yr->ex, yl->bzm, yr->bzm);
+  double yby_h = hll_state(yl->by, yr->by, yl->phi, yr->phi);
+  double ybz_h = hll_state(yl->bz, yr->bz, yl->exm, yr->exm);
+  double ybx_h = hll_state(yl->bx, yr->bx, yl->ez, yr->ez);
+  (void)xpsi_h; (void)xphi_h; (void)ypsi_h; (void)yphi_h;
+
+  /* electromagnetic HLL fluxes (:2587-2708) are the face fluxes */
+  fx[FX_EXPSI] = hll_flux(xl->ex, xr->ex, xl->psi, xr->psi);
+  fx[FX_BXPHI] = hll_flux(xl->bx, xr->bx, xl->phi, xr->phi);
+  fx[FX_PHIBX] = hll_flux(xl->phi, xr->phi, xl->bx, xr->bx);
+  fx[FX_EZBY] = -hll_flux(xl->ezm, xr->ezm, xl->by, xr->by);
+  fx[FX_EYBZ] = hll_flux(xl->ey, xr->ey, xl->bz, xr->bz);
+  fx[FX_PSIEX] = hll_flux(xl->psi, xr->psi, xl->ex, xr->ex);
+  fx[FX_BZEY] = hll_flux(xl->bz, xr->bz, xl->ey, xr->ey);
+  fx[FX_BYEZ] = -hll_flux(xl->bym, xr->bym, xl->ez, xr->ez);
+  fy[FY_EYPSI] = hll_flux(yl->ey, yr->ey, yl->psi, yr->psi);
+  fy[FY_BYPHI] = hll_flux(yl->by, yr->by, yl->phi, yr->phi);
+  fy[FY_PHIBY] = hll_flux(yl->phi, yr->phi, yl->by, yr->by);
+  fy[FY_EXBZ] = -hll_flux(yl->exm, yr->exm, yl->bz, yr->bz);
+  fy[FY_EZBX] = hll_flux(yl->ez, yr->ez, yl->bx, yr->bx);
+  fy[FY_PSIEY] = hll_flux(yl->psi, yr->psi, yl->ey, yr->ey);
+  fy[FY_BXEZ] = hll_flux(yl->bx, yr->bx, yl->ez, yr->ez);
+  fy[FY_BZEX] = -hll_flux(yl->bzm, yr->bzm, yl->ex, yr->ex);
+
+  /* x fluid part (:2715-2862) */
+  double a, b;
+  int st = hlle_speeds(gam, xl, xr, 0, &a, &b);
+  if (st) return st;
+  hlle_face_t X;
+  X.q_f = hlle_flux(a, b, xl->j1, xr->j1, xl->q, xr->q);
+  X.dd_h = hlle_state(a, b, xl->d, xr->d, xl->fd, xr->fd);
+  X.tau_h = hlle_state(a, b, xl->tau, xr->tau, xl->ftau, xr->ftau);
+  X.s_h[0] = hlle_state(a, b, xl->sx, xr->sx, xl->fs1, xr->fs1);
+  X.s_h[1] = hlle_state(a, b, xl->sy, xr->sy, xl->fs2, xr->fs2);
+  X.s_h[2] = hlle_state(a, b, xl->sz, xr->sz, xl->fs3, xr->fs3);
+  X.dd_f = hlle_flux(a, b, xl->fd, xr->fd, xl->d, xr->d);
+  X.tau_f = hlle_flux(a, b, xl->ftau, xr->ftau, xl->tau, xr->tau);
+  X.s_f[0] = hlle_flux(a, b, xl->fs1, xr->fs1, xl->sx, xr->sx);
+  X.s_f[1] = hlle_flux(a, b, xl->fs2, xr->fs2, xl->sy, xr->sy);
+  X.s_f[2] = hlle_flux(a, b, xl->fs3, xr->fs3, xl->sz, xr->sz);
+  X.e[0] = xex_h; X.e[1] = xey_h; X.e[2] = xez_h;
+  X.b[0] = xbx_h; X.b[1] = xby_h; X.b[2] = xbz_h;
+  X.er[0] = xbz_h * xey_h - xby_h * xez_h;
+  X.er[1] = xbx_h * xez_h - xbz_h * xex_h;
+  X.er[2] = xby_h * xex_h - xbx_h * xey_h;
+  X.e2b2s = 0.5 * (((sq(xex_h) + sq(xey_h)) + sq(xez_h)) + ((sq(xbx_h) + sq(xby_h)) + sq(xbz_h)));
+
+  /* y fluid part (:2893-3040) */
+  double c, d;
+  st = hlle_speeds(gam, yl, yr, 1, &c, &d);
+  if (st) return st;
+  hlle_face_t Y;
+  Y.q_f = hlle_flux(c, d, yl->j1, yr->j1, yl->q, yr->q);
+  Y.dd_h = hlle_state(c, d, yl->d, yr->d, yl->fd, yr->fd);
+  Y.tau_h = hlle_state(c, d, yl->tau, yr->tau, yl->ftau, yr->ftau);
+  Y.s_h[0] = hlle_state(c, d, yl->sx, yr->sx, yl->fs1, yr->fs1);
+  Y.s_h[1] = hlle_state(c, d, yl->sy, yr->sy, yl->fs2, yr->fs2);
+  Y.s_h[2] = hlle_state(c, d, yl->sz, yr->sz, yl->fs3, yr->fs3);
+  Y.dd_f = hlle_flux(c, d, yl->fd, yr->fd, yl->d, yr->d);
+  Y.tau_f = hlle_flux(c, d, yl->ftau, yr->ftau, yl->tau, yr->tau);
+  Y.s_f[0] = hlle_flux(c, d, yl->fs1, yr->fs1, yl->sx, yr->sx);
+  Y.s_f[1] = hlle_flux(c, d, yl->fs2, yr->fs2, yl->sy, yr->sy);
+  Y.s_f[2] = hlle_flux(c, d, yl->fs3, yr->fs3, yl->sz, yr->sz);
+  Y.e[0] = yex_h; Y.e[1] = yey_h; Y.e[2] = yez_h;
+  Y.b[0] = ybx_h; Y.b[1] = yby_h; Y.b[2] = ybz_h;
+  Y.er[0] = ybz_h * yey_h - yby_h * yez_h;
+  Y.er[1] = ybx_h * yez_h - ybz_h * yex_h;
+  Y.er[2] = yby_h * yex_h - ybx_h * yey_h;
+  Y.e2b2s = 0.5 * (((sq(yex_h) + sq(yey_h)) + sq(yez_h)) + ((sq(ybx_h) + sq(yby_h)) + sq(ybz_h)));
+
+  /* contact speeds (:3225-3334) */
+  double ax = X.tau_f - X.er[0];
+  double bx = ((((sq(xey_h) + sq(xez_h)) + (sq(xby_h) + sq(xbz_h))) - X.tau_h) - X.s_f[0]);
+  double cx = X.s_h[0] - X.er[0];
+  double vx = hllc_contact(ax, bx, cx, g_hllc_seed_mode ? Y.er[1] : NAN);
+  if (vx != vx) return 7;
+  double ay = Y.tau_f - Y.er[1];
+  double by = ((((sq(yex_h) + sq(yez_h)) + (sq(ybx_h) + sq(ybz_h))) - Y.tau_h) - Y.s_f[1]);
+  double cy = Y.s_h[1] - Y.er[1];
+  double vy = hllc_contact(ay, by, cy, g_hllc_seed_mode ? g_hllc_last_iterate : NAN);
+  if (vy != vy) return 7;
+
+  double o[6];
+  st = hlle_fluid(&X, 0, a, b, vx, o);
+  if (st) return st;
+  fx[FX_J] = o[0]; fx[FX_FD] = o[1]; fx[FX_FTAU] = o[2];
+  fx[FX_FSX] = o[3]; fx[FX_FSY] = o[4]; fx[FX_FSZ] = o[5];
+  st = hlle_fluid(&Y, 1, c, d, vy, o);
+  if (st) return st;
+  fy[FY_J] = o[0]; fy[FY_FD] = o[1]; fy[FY_FTAU] = o[2];
+  fy[FY_FSX] = o[3]; fy[FY_FSY] = o[4]; fy[FY_FSZ] = o[5];
+  return 0;
+}
+
+/* face loop: 16_hll_flow_mpx.f95:1358-1958, 16_hllc_flow_mpx.f95:2401-4612,
+ * 16_hlle_flow_mpx.f95:2258-4618 */
 static int face_loop(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
   double **P = s->prim;
@@ -954,12 +1172,13 @@ static int face_loop(oracle_state *s) {
       load_y(s, s->yR, k, &yr);
       /* superluminal fallback (16_hll_flow_mpx.f95:1392-1461): any of the
        * four face speeds >= 1 replaces all of them by cell values, with the
-       * reference's (i-1,j) and (i,j-1) right states. */
+       * reference's (i-1,j) and (i,j-1) right states.  HLLE only prints
+       * (16_hlle_flow_mpx.f95:2270-2300). */
       double v2xl = (sq(xl.vx) + sq(xl.vy)) + sq(xl.vz);
       double v2xr = (sq(xr.vx) + sq(xr.vy)) + sq(xr.vz);
       double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
       double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
-      if (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0) {
+      if (fs != OR_HLLE && (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0)) {
         size_t kc = IX(s, i, j), km = IX(s, i - 1, j), kj = IX(s, i, j - 1);
         xl.vx = P[OR_VX][kc]; xl.vy = P[OR_VY][kc]; xl.vz = P[OR_VZ][kc];
         xr.vx = P[OR_VX][km]; xr.vy = P[OR_VY][km]; xr.vz = P[OR_VZ][km];
@@ -973,10 +1192,11 @@ static int face_loop(oracle_state *s) {
         else if (xl.rho < 0.0) xl.rho = rc;
         else if (xr.rho < 0.0) xr.rho = rc;
       }
-      side_physics(s->c.gamma, 0, &xl);
-      side_physics(s->c.gamma, 0, &xr);
-      side_physics(s->c.gamma, 1, &yl);
-      side_physics(s->c.gamma, 1, &yr);
+      int he = fs == OR_HLLE;
+      side_physics(s->c.gamma, 0, he, &xl);
+      side_physics(s->c.gamma, 0, he, &xr);
+      side_physics(s->c.gamma, 1, he, &yl);
+      side_physics(s->c.gamma, 1, he, &yr);
       double fx[FX_N], fy[FY_N];
       if (fs == OR_HLL) {
         hll_x(&xl, &xr, fx);
@@ -987,7 +1207,8 @@ static int face_loop(oracle_state *s) {
         st = hllc_y(&yl, &yr, fy);
         if (st) return st;
       } else {
-        return 9; /* HLLE is not restated yet */
+        int st = hlle_xy(s->c.gamma, &xl, &xr, &yl, &yr, fx, fy);
+        if (st) return st;
       }
       for (int f = 0; f < FX_N; f++) s->fx[f][k] = fx[f];
       for (int f = 0; f < FY_N; f++) s->fy[f][k] = fy[f];

@@ -26,7 +26,8 @@ import cueva  # noqa: E402
 
 FIELDS = list(cueva.PRIM_NAMES) + list(cueva.CONS_NAMES)
 
-# name: (native imax, native jmax, flux_solver, rec_mp, order, sigma, nsteps, dumps, full dumps, expect_stop)
+# name: (native imax, native jmax, flux_solver, rec_mp, order, sigma, nsteps, dumps, full dumps, expect_stop
+#        [, initial-data keywords of cueva.kh_problem])
 CASES = {
     "kh_hll_mp5_32x64": (32, 64, 5, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None),
     "kh_hllc_mp5_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None),
@@ -35,6 +36,7 @@ CASES = {
     "kh_hll_mp9_ssp3_32x64": (32, 64, 5, 2, 4, 1.0e3, 10, [1, 7], [7], 8),
     "kh_cfg1_hll_mp5_128x256": (128, 256, 5, 0, 2, 1.0e3, 100, [1, 10, 100], [100], None),
     "kh_hllc_mp5_128x256": (128, 256, 7, 0, 2, 1.0e6, 100, [1, 10, 100], [], None),
+    "kh_hlle_mp7_bz_32x64": (32, 64, 6, 1, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
 }
 SEED = 20261202
 
@@ -45,9 +47,11 @@ def field_digest(a):
 
 def main(only=None):
     os.environ["OMP_NUM_THREADS"] = "1"
-    for name, (ni, nj, fs, rec, order, sigma, nsteps, dumps, full, stop) in CASES.items():
+    for name, case in CASES.items():
         if only and name not in only:
             continue
+        ni, nj, fs, rec, order, sigma, nsteps, dumps, full, stop = case[:10]
+        init_kw = case[10] if len(case) > 10 else {}
         imax, jmax = nj, ni  # rotated frame
         refdir = os.path.join(ROOT, "oracle", "_ref", name)
         exe = os.path.join(refdir, "cueva_ref")
@@ -56,7 +60,7 @@ def main(only=None):
                             str(order), str(fs), str(rec), "6", "%.1fd0" % sigma if sigma < 1e4 else "1.d6",
                             "0.4d0", "2.d0", "1.d0"], check=True)
         p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
-                                    rotated=True, seed=SEED)
+                                    rotated=True, seed=SEED, **init_kw)
         assert (p.imax, p.jmax) == (imax, jmax)
         with tempfile.TemporaryDirectory() as tmp:
             inp = os.path.join(tmp, "in.bin")
@@ -66,7 +70,7 @@ def main(only=None):
             stopped = "Fortran STOP" in res.stdout + res.stderr
             meta = {"name": name, "native_grid": [ni, nj], "grid": [imax, jmax], "flux_solver": fs,
                     "rec_mp": rec, "order": order, "sigma": sigma, "seed": SEED, "frame": "rotated",
-                    "nsteps": nsteps, "init_sha256": field_digest(prims),
+                    "nsteps": nsteps, "init_sha256": field_digest(prims), "init": init_kw,
                     "reference_stopped": stopped, "expected_stop_step": stop, "dumps": {}}
             arrays = {}
             for d in dumps:

@@ -30,12 +30,14 @@ from parity import FIELDS, interior, ndiff, rel_l1
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
+# name: (flux solver, rec_mp, IMEX order, sigma, initial-data keywords)
 CASES = {
-    "hll_mp5": (cueva.HLL, 0, 2, 1.0e3),
-    "hllc_mp5": (cueva.HLLC, 0, 2, 1.0e6),
-    "hll_mp7": (cueva.HLL, 1, 2, 1.0e3),
-    "hllc_mp9": (cueva.HLLC, 2, 2, 1.0e6),
-    "hll_mp9_ssp3": (cueva.HLL, 2, 4, 1.0e3),
+    "hll_mp5": (cueva.HLL, 0, 2, 1.0e3, {}),
+    "hllc_mp5": (cueva.HLLC, 0, 2, 1.0e6, {}),
+    "hll_mp7": (cueva.HLL, 1, 2, 1.0e3, {}),
+    "hllc_mp9": (cueva.HLLC, 2, 2, 1.0e6, {}),
+    "hll_mp9_ssp3": (cueva.HLL, 2, 4, 1.0e3, {}),
+    "hlle_mp7_bz": (cueva.HLLE, 1, 2, 1.0e6, {"bz0": 0.1}),
 }
 
 
@@ -71,9 +73,9 @@ def run_pair(p, prims, steps, every=1):
 @pytest.mark.parametrize("rotated", [True, False], ids=["rotated", "native"])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_gpu_equals_oracle_bitwise(case, rotated):
-    fs, rec, order, sigma = CASES[case]
+    fs, rec, order, sigma, kw = CASES[case]
     say("case", case, "rotated" if rotated else "native", "32x64, 5 steps")
-    p, prims = cueva.kh_problem(32, 64, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=rotated)
+    p, prims = cueva.kh_problem(32, 64, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=rotated, **kw)
     g, r = run_pair(p, prims, 5)
     nd = ndiff(g, r)
     say(case, "differing values:", nd)
@@ -90,12 +92,12 @@ def test_gpu_equals_oracle_bitwise_100_steps(fs, sigma):
     assert sum(nd.values()) == 0, (nd, rel_l1(g, r))
 
 
-@pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3"])
+@pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz"])
 def test_routine_entry_points_match_fused_step(case):
     """The per-subroutine entry points (the drop-in path) and cueva_step agree bit for bit."""
-    fs, rec, order, sigma = CASES[case]
+    fs, rec, order, sigma, kw = CASES[case]
     say("case", case)
-    p, prims = cueva.kh_problem(32, 64, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True)
+    p, prims = cueva.kh_problem(32, 64, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True, **kw)
     with cueva.Solver(p) as a, cueva.Solver(p) as b:
         a.upload(prims)
         b.upload(prims)
@@ -134,7 +136,7 @@ def golden(name):
 # dumps where the correctly rounded path equals the reference bit for bit
 BITWISE_DUMPS = {
     "kh_hll_mp5_32x64": (1, 10), "kh_hllc_mp5_32x64": (1, 10), "kh_hll_mp7_32x64": (1, 10),
-    "kh_hllc_mp9_32x64": (1, 10), "kh_hll_mp9_ssp3_32x64": (1, 7),
+    "kh_hllc_mp9_32x64": (1, 10), "kh_hll_mp9_ssp3_32x64": (1, 7), "kh_hlle_mp7_bz_32x64": (1, 10),
     "kh_cfg1_hll_mp5_128x256": (1,), "kh_hllc_mp5_128x256": (1, 10),
 }
 
@@ -146,7 +148,8 @@ def test_against_reference_fixture(name):
     ni, nj = meta["native_grid"]
     say("fixture", name)
     p, prims = cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"],
-                                rec_mp=meta["rec_mp"], order=meta["order"], rotated=True, seed=meta["seed"])
+                                rec_mp=meta["rec_mp"], order=meta["order"], rotated=True, seed=meta["seed"],
+                                **meta.get("init", {}))
     assert digest(prims) == meta["init_sha256"]
     with cueva.Solver(p) as s:
         s.upload(prims)

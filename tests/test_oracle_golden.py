@@ -34,7 +34,8 @@ def load_case(name):
 def problem(meta):
     ni, nj = meta["native_grid"]
     return cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"],
-                            rec_mp=meta["rec_mp"], order=meta["order"], rotated=True, seed=meta["seed"])
+                            rec_mp=meta["rec_mp"], order=meta["order"], rotated=True, seed=meta["seed"],
+                            **meta.get("init", {}))
 
 
 def digest(a):
