@@ -149,13 +149,24 @@ int cueva_step(cueva_ctx *ctx, int nsteps);
  * last call (waits for the device). */
 int cueva_step_counters(cueva_ctx *ctx, int64_t counters[3], int64_t *fallback);
 
-/* FP64 global integrals of 22_writer.f95:821-1000 over i=0..imax, j=0..jmax,
- * each times Delx*Dely: out = {psi, phi, Ex, Ey, Ez, Bx, By, Bz, vx, vy, vz,
- * E2, B2, v2, rho, D, q, p, W}.  Waits for the device. */
-int cueva_reduce_globals(cueva_ctx *ctx, double out[19]);
+/* The 37 global diagnostics the reference writes every hglb steps
+ * (22_writer.f95:818-991), in the order of its output units 80..217:
+ * out = {sum_psi, sum_phi, sum_ex, sum_ey, sum_ez, sum_exey, sum_exez,
+ * sum_eyez, sum_bx, sum_by, sum_bz, sum_bxby, sum_bxbz, sum_bybz, sum_vxvyvz,
+ * sum_bxbybz, sum_vx, sum_vy, sum_vz, sum_vxvy, sum_vxvz, sum_vyvz, sum_q,
+ * sum_p, sum_rho, ekin_c, ekin_r, etot, efluid_r, emag, eint_c, eint_r,
+ * lorentz, sum_D, sum_D_med, max_val_bx, max_val_phi} over i=0..imax,
+ * j=0..jmax with each cell term formed as the reference forms it.  FP64,
+ * deterministic; differs from the reference's sequential sum only by the
+ * summation order.  Waits for the device. */
+#define CUEVA_N_GLOBALS 37
+int cueva_reduce_globals(cueva_ctx *ctx, double out[CUEVA_N_GLOBALS]);
 
-/* Largest signal speed max|v| + ... used for an adaptive dt (new; the reference
- * uses the fixed dt of parameters.f95:312). Waits for the device. */
+/* Largest relativistic acoustic signal speed |lambda| over the interior and
+ * both directions.  No reference counterpart: the reference takes the fixed
+ * dt = CFL*min(Delx, Dely, 1) of parameters.f95:312 (light speed 1, which
+ * bounds every wave of the augmented system); this lets a caller check that
+ * bound or choose an adaptive dt.  Waits for the device. */
 int cueva_max_speed(cueva_ctx *ctx, double *vmax);
 
 /* Replace the conserved D tau Sx Sy Sz (5 planes, host layout) that the

@@ -10,6 +10,7 @@
 
 #include "cueva.h"
 #include "cueva_kernels.hpp"
+#include "cueva_reduce.hpp"
 
 using namespace cueva;
 
@@ -22,6 +23,7 @@ struct cueva_ctx {
   double* base;
   int64_t bytes;
   DevStatus* dst;
+  double* red;  // reduction partials (RED_BLOCKS * G_N) and results (G_N), after dst
   int rec_order;
   int err;  // sticky host-side error
   // kernel timing (cueva_set_profiling)
@@ -201,12 +203,15 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     delete c;
     return CUEVA_E_NOMEM;
   }
-  if (hipMalloc(&c->dst, sizeof(DevStatus)) != hipSuccess) {
+  const size_t status_bytes = (sizeof(DevStatus) + 255) / 256 * 256;
+  const size_t red_bytes = (size_t)(RED_BLOCKS + 1) * G_N * sizeof(double);
+  if (hipMalloc(&c->dst, status_bytes + red_bytes) != hipSuccess) {
     hipFree(c->base);
     hipStreamDestroy(c->stream);
     delete c;
     return CUEVA_E_NOMEM;
   }
+  c->red = (double*)((char*)c->dst + status_bytes);
   double* q = c->base;
   auto take = [&]() { double* r = q; q += g.plane; return r; };
   for (int k = 0; k < P_N; k++) c->P.p[k] = take();
@@ -569,12 +574,22 @@ extern "C" int cueva_time_steps(cueva_ctx* c, int nsteps, double* ms) {
   return rc;
 }
 
-extern "C" int cueva_reduce_globals(cueva_ctx* c, double out[19]) {
-  (void)c; (void)out;
-  return CUEVA_E_ARG;  // built in the reductions unit
+extern "C" int cueva_reduce_globals(cueva_ctx* c, double out[37]) {
+  if (!c || !out) return CUEVA_E_ARG;
+  double* res = c->red + (size_t)RED_BLOCKS * G_N;
+  LAUNCH(K_OTHER, k_globals_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), c->g, c->P, c->S, c->red);
+  LAUNCH(K_OTHER, k_globals_final, dim3(1), dim3(RED_THREADS), (const double*)c->red, res);
+  HIPCHK(hipMemcpyAsync(out, res, G_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
 }
 
 extern "C" int cueva_max_speed(cueva_ctx* c, double* vmax) {
-  (void)c; (void)vmax;
-  return CUEVA_E_ARG;
+  if (!c || !vmax) return CUEVA_E_ARG;
+  double* res = c->red + (size_t)RED_BLOCKS * G_N;
+  LAUNCH(K_OTHER, k_speed_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), c->g, c->P, c->S, c->red);
+  LAUNCH(K_OTHER, k_speed_final, dim3(1), dim3(RED_THREADS), (const double*)c->red, res);
+  HIPCHK(hipMemcpyAsync(vmax, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
 }

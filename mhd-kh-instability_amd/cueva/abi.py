@@ -14,6 +14,12 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libcueva.so")
 
 HLL, HLLE, HLLC = 5, 6, 7
+# output order of cueva_reduce_globals (the reference's units 80..217)
+GLOBAL_NAMES = ("sum_psi", "sum_phi", "sum_ex", "sum_ey", "sum_ez", "sum_exey", "sum_exez", "sum_eyez",
+                "sum_bx", "sum_by", "sum_bz", "sum_bxby", "sum_bxbz", "sum_bybz", "sum_vxvyvz", "sum_bxbybz",
+                "sum_vx", "sum_vy", "sum_vz", "sum_vxvy", "sum_vxvz", "sum_vyvz", "sum_q", "sum_p", "sum_rho",
+                "ekin_c", "ekin_r", "etot", "efluid_r", "emag", "eint_c", "eint_r", "lorentz", "sum_D",
+                "sum_D_med", "max_val_bx", "max_val_phi")
 BC_COPY, BC_PERIODIC = 0, 1
 FP64, MIXED = 0, 1
 C2P_CARDANO, C2P_LEARNED = 0, 1
@@ -221,6 +227,18 @@ class Solver:
         ms = ctypes.c_double()
         check(self.lib.cueva_time_steps(self.ctx, int(n), ctypes.byref(ms)), "time_steps")
         return ms.value
+
+    # -- diagnostics
+    def globals(self):
+        """The 37 global diagnostics of 22_writer.f95:818-991 (GLOBAL_NAMES order)."""
+        out = np.zeros(len(GLOBAL_NAMES))
+        check(self.lib.cueva_reduce_globals(self.ctx, _dp(out)), "reduce_globals")
+        return dict(zip(GLOBAL_NAMES, out.tolist()))
+
+    def max_speed(self):
+        v = ctypes.c_double()
+        check(self.lib.cueva_max_speed(self.ctx, ctypes.byref(v)), "max_speed")
+        return v.value
 
     # -- fused step
     def step(self, n=1):

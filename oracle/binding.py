@@ -49,6 +49,7 @@ def load():
         lib.oracle_c2p_point.argtypes = [ctypes.c_double, _P, _P, ctypes.POINTER(ctypes.c_int)]
         lib.oracle_set_hllc_seed_mode.argtypes = [ctypes.c_int]
         lib.oracle_set_libm_mode.argtypes = [ctypes.c_int]
+        lib.oracle_globals.argtypes = [ctypes.c_void_p, _P]
         _lib = lib
     return _lib
 
@@ -107,3 +108,9 @@ class Oracle:
         c = (ctypes.c_int64 * 3)()
         self.lib.oracle_get_counters(self.s, c)
         return list(c)
+
+    def globals(self):
+        """37 diagnostics of 22_writer.f95:818-991, sequential as the reference sums them."""
+        out = np.zeros(37)
+        self.lib.oracle_globals(self.s, _dp(out))
+        return out

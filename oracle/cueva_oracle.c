@@ -1462,3 +1462,70 @@ int oracle_step(oracle_state *s, int nsteps) {
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* global diagnostics, 22_writer.f95:818-991: sequential sums with i    */
+/* outer and j inner, each statement evaluated left to right.           */
+/* ------------------------------------------------------------------ */
+void oracle_globals(const oracle_state *s, double out[37]) {
+  const double a = s->c.dx, b = s->c.dy, gm = s->c.gamma, B0 = 1.0e-1; /* parameters.f95:131 */
+  double *const *P = s->prim;
+  double sm[35] = {0};
+  double mbx = -INFINITY, mphi = -INFINITY;
+  for (int i = 0; i <= s->c.imax; i++)
+    for (int j = 0; j <= s->c.jmax; j++) {
+      size_t k = IX(s, i, j);
+      double psi = P[OR_PSI][k], phi = P[OR_PHI][k], ex = P[OR_EX][k], ey = P[OR_EY][k], ez = P[OR_EZ][k];
+      double bx = P[OR_BX][k], by = P[OR_BY][k], bz = P[OR_BZ][k], vx = P[OR_VX][k], vy = P[OR_VY][k];
+      double vz = P[OR_VZ][k], q = P[OR_Q][k], p = P[OR_P][k], rho = P[OR_RHO][k];
+      double d = s->D[k], tau = s->tau[k];
+      sm[0] = sm[0] + psi * a * b;
+      sm[1] = sm[1] + phi * a * b;
+      sm[2] = sm[2] + sq(ex) * a * b;
+      sm[3] = sm[3] + sq(ey) * a * b;
+      sm[4] = sm[4] + sq(ez) * a * b;
+      sm[5] = sm[5] + ex * ey * a * b;
+      sm[6] = sm[6] + ex * ez * a * b;
+      sm[7] = sm[7] + ey * ez * a * b;
+      sm[8] = sm[8] + sq(bx) * a * b;
+      sm[9] = sm[9] + sq(by) * a * b;
+      sm[10] = sm[10] + sq(bz) * a * b;
+      double m1 = bx / B0;
+      mbx = m1 > mbx ? m1 : mbx;
+      mphi = phi > mphi ? phi : mphi;
+      sm[11] = sm[11] + bx * by * a * b;
+      sm[12] = sm[12] + bx * bz * a * b;
+      sm[13] = sm[13] + by * bz * a * b;
+      sm[15] = sm[15] + (sq(bx) + sq(by) + sq(bz)) * a * b; /* sum_bxbybz */
+      sm[14] = sm[14] + (sq(vx) + sq(vy) + sq(vz)) * a * b; /* sum_vxvyvz */
+      sm[16] = sm[16] + sq(vx) * a * b;
+      sm[17] = sm[17] + sq(vy) * a * b;
+      sm[18] = sm[18] + sq(vz) * a * b;
+      sm[19] = sm[19] + vx * vy * a * b;
+      sm[20] = sm[20] + vx * vz * a * b;
+      sm[21] = sm[21] + vy * vz * a * b;
+      sm[22] = sm[22] + q * a * b;
+      sm[23] = sm[23] + p * a * b;
+      sm[24] = sm[24] + rho * a * b;
+      sm[33] = sm[33] + d * a * b; /* sum_D */
+      double eps = p / ((gm - 1.0) * rho);
+      double h = rho * (1.0 + eps) + p;
+      double w = 1.0 / sqrt(1.0 - (sq(vx) + sq(vy) + sq(vz)));
+      double e2 = sq(ex) + sq(ey) + sq(ez);
+      double b2 = sq(by) + sq(bz);
+      sm[25] = sm[25] + 0.5 * rho * (sq(vx) + sq(vy) + sq(vz)) * a * b; /* ekin_c */
+      sm[26] = sm[26] + d * (w - 1.0) * a * b;                       /* ekin_r */
+      sm[30] = sm[30] + rho * eps * a * b;                           /* eint_c */
+      sm[31] = sm[31] + sq(w) * (rho * eps + p) - p * a * b;         /* eint_r */
+      sm[27] = sm[27] + tau * a * b;                                 /* etot */
+      sm[28] = sm[28] + h * sq(w) - p * a * b;                       /* efluid_r */
+      sm[29] = sm[29] + 0.5 * (e2 + b2) * a * b;                     /* emag */
+      sm[32] = sm[32] + w * a * b;                                   /* lorentz */
+    }
+  int imed1 = (int)floor(0.25 * s->c.imax), imed2 = (int)floor(0.75 * s->c.imax);
+  for (int i = imed1; i <= imed2; i++)
+    for (int j = 0; j <= s->c.jmax; j++) sm[34] = sm[34] + s->D[IX(s, i, j)] * a * b;
+  for (int k = 0; k < 35; k++) out[k] = sm[k];
+  out[35] = mbx;
+  out[36] = mphi;
+}

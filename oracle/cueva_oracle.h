@@ -70,6 +70,10 @@ int oracle_step(oracle_state *s, int nsteps);
  * which = 0..16 stage RHS (14 flux divergences then 3 sources). */
 void oracle_get_rhs(const oracle_state *s, int l, int which, double *out);
 
+/* The 37 global diagnostics of 22_writer.f95:818-991 in the reference's
+ * order and summation order (i outer, j inner). */
+void oracle_globals(const oracle_state *s, double out[37]);
+
 /* Pointwise/linewise pieces, exported for unit tests. */
 void oracle_mp_line(int order, const double *u, int n, double *up, double *um, int positivity);
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0);
