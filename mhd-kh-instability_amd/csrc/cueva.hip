@@ -12,6 +12,12 @@
 #include "cueva_kernels.hpp"
 #include "cueva_reduce.hpp"
 
+namespace cueva {
+// FP32 face kernels (cueva_mixed.hip)
+hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
+                             const Scheme& S, DevStatus* st);
+}
+
 using namespace cueva;
 
 struct cueva_ctx {
@@ -157,7 +163,7 @@ static int check_params(const cueva_params* p) {
     return CUEVA_E_ARG;
   if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
-  if (p->precision != CUEVA_FP64) return CUEVA_E_ARG;
+  if (p->precision != CUEVA_FP64 && p->precision != CUEVA_MIXED) return CUEVA_E_ARG;
   if (p->c2p_mode != CUEVA_C2P_CARDANO) return CUEVA_E_ARG;
   if (!(p->longx > 0.0) || !(p->longy > 0.0) || !(p->cfl > 0.0) || !(p->sigma >= 0.0)) return CUEVA_E_ARG;
   return CUEVA_OK;
@@ -168,7 +174,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   *out = nullptr;
   int rc = check_params(p);
   if (rc) return rc;
-  cueva_ctx* c = new (std::nothrow) cueva_ctx;
+  cueva_ctx* c = new (std::nothrow) cueva_ctx();
   if (!c) return CUEVA_E_NOMEM;
   memset(c, 0, sizeof(*c));
   c->prm = *p;
@@ -219,10 +225,30 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   for (int k = 0; k < S_N; k++) c->P.s[k] = take();
   for (int k = 0; k < FX_N; k++) c->P.fx[k] = take();
   for (int k = 0; k < FY_N; k++) c->P.fy[k] = take();
+  // the FP32 face-flux planes of the mixed form reuse the FP64 ones
+  for (int k = 0; k < FX_N; k++) c->P.fxf[k] = (float*)c->P.fx[k];
+  for (int k = 0; k < FY_N; k++) c->P.fyf[k] = (float*)c->P.fy[k];
   for (int l = 0; l < MAXL; l++)
     for (int k = 0; k < R_N; k++) c->P.rhs[l][k] = take();
   for (int k = 0; k < A_N; k++) c->P.ints[k] = take();
   for (int k = 0; k < A_N; k++) c->P.fins[k] = take();
+  // every plane pointer the kernels receive must lie inside the allocation
+  {
+    const char* lo = (const char*)c->base;
+    const char* hi = lo + c->bytes;
+    void* const* ptrs = (void* const*)&c->P;
+    const int nptr = (int)(sizeof(Planes) / sizeof(void*));
+    for (int k = 0; k < nptr; k++) {
+      const char* a = (const char*)ptrs[k];
+      if (a < lo || a + g.plane * (int64_t)sizeof(float) > hi) {
+        hipFree(c->dst);
+        hipFree(c->base);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return CUEVA_E_HIP;
+      }
+    }
+  }
   if (hipMemsetAsync(c->base, 0, (size_t)c->bytes, c->stream) != hipSuccess ||
       hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -412,6 +438,13 @@ static int launch_faces_ord(cueva_ctx* c) {
 }
 
 static int launch_faces(cueva_ctx* c) {
+  if (c->prm.precision == CUEVA_MIXED) {
+    dim3 grid((unsigned)((c->g.imax + 2 + BLK.x - 1) / BLK.x), (unsigned)((c->g.jmax + 2 + BLK.y - 1) / BLK.y));
+    prof_begin(c, K_FACES);
+    hipError_t e = launch_faces_fp32(c->rec_order, grid, BLK, c->stream, c->g, c->P, c->S, c->dst);
+    prof_end(c);
+    return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
+  }
   switch (c->rec_order) {
     case 5: return launch_faces_ord<5>(c);
     case 7: return launch_faces_ord<7>(c);
@@ -431,7 +464,10 @@ extern "C" int cueva_flux(cueva_ctx* c, int l) {
 
 extern "C" int cueva_fluxandsource(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  LAUNCH(K_FLUXSRC, k_fluxandsource, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
+  if (c->prm.precision == CUEVA_MIXED)
+    LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
+  else
+    LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
   return launch_ok();
 }
 
@@ -486,10 +522,17 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
       if (launch_ok()) return CUEVA_E_HIP;
       int rc = launch_faces(c);
       if (rc) return rc;
+      const bool mixed = c->prm.precision == CUEVA_MIXED;
       if (l < c->S.lmax - 1) {
-        LAUNCH(K_FLUXSRC, k_fluxandsource, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
+        if (mixed)
+          LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
+        else
+          LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
       } else {
-        LAUNCH(K_FINAL, k_final_fused, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
+        if (mixed)
+          LAUNCH(K_FINAL, k_final_fused<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
+        else
+          LAUNCH(K_FINAL, k_final_fused<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
       }
       if (launch_ok()) return CUEVA_E_HIP;
     }

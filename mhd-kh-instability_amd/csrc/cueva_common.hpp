@@ -1,0 +1,84 @@
+// Device-side data of the CUEVA step: plane sets, scheme constants and the
+// status word shared by every kernel.
+#pragma once
+
+#include "cueva_device.hpp"
+
+namespace cueva {
+
+// ---- field planes --------------------------------------------------------
+// primitive/EM planes, restart order (22_writer.f95:1116-1172)
+enum { P_PSI, P_PHI, P_BX, P_BY, P_BZ, P_VX, P_VY, P_VZ, P_EX, P_EY, P_EZ, P_Q, P_RHO, P_P, P_N };
+// startconvar outputs (06_conserved_var.f95)
+enum { C_D, C_TAU, C_SX, C_SY, C_SZ, C_W, C_N };
+// stage planes (the l-indexed arrays of fourvectors.f95, one stage at a time)
+enum { S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_EXA, S_EYA, S_EZA, S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ,
+       S_EX, S_EY, S_EZ, S_JX, S_JY, S_JZ, S_N };
+// accumulated fields of sumint/sumfin (09_sumint.f95, 18_sumfin.f95)
+enum { A_PSI, A_PHI, A_BX, A_BY, A_BZ, A_EX, A_EY, A_EZ, A_Q, A_D, A_TAU, A_SX, A_SY, A_SZ, A_N };
+// stage right-hand side: 14 flux divergences (A_* order) + 3 stiff sources
+enum { R_SEX = 14, R_SEY = 15, R_SEZ = 16, R_N = 17 };
+
+constexpr int MAXL = 4;
+
+struct Planes {
+  double* p[P_N];
+  double* c[C_N];
+  double* s[S_N];
+  double* fx[FX_N];
+  double* fy[FY_N];
+  float* fxf[FX_N];  // the same memory as fx/fy, as FP32 planes (mixed form)
+  float* fyf[FY_N];
+  double* rhs[MAXL][R_N];
+  double* ints[A_N];
+  double* fins[A_N];
+};
+
+struct Scheme {
+  double dx, dy, dt, sigma, kpsi, kphi, gamma;
+  double inv_dx, inv_dy;
+  int div_exact;  // dx, dy powers of two: x/d == x*(1/d)
+  int lmax, flux_solver, order;
+  double wb[MAXL], wbt[MAXL], Ab[MAXL][MAXL], Abt[MAXL][MAXL];
+  int hllc_seed_ref;
+  int crmath;
+};
+
+// status words in device memory
+struct DevStatus {
+  int err;          // first error code (0 none)
+  int err_i, err_j; // where
+  int pad;
+  unsigned long long cnt[3];  // contador_r, contador_p, contador_v
+  unsigned long long fallback;
+};
+
+__device__ inline void report(DevStatus* st, int code, int i, int j) {
+  if (atomicCAS(&st->err, 0, code) == 0) {
+    st->err_i = i;
+    st->err_j = j;
+  }
+}
+
+// U^n of accumulated field a (in A_* order) at offset k
+__device__ inline double un_value(const Planes& P, int a, int64_t k) {
+  switch (a) {
+    case A_PSI: return P.p[P_PSI][k];
+    case A_PHI: return P.p[P_PHI][k];
+    case A_BX: return P.p[P_BX][k];
+    case A_BY: return P.p[P_BY][k];
+    case A_BZ: return P.p[P_BZ][k];
+    case A_EX: return P.p[P_EX][k];
+    case A_EY: return P.p[P_EY][k];
+    case A_EZ: return P.p[P_EZ][k];
+    case A_Q: return P.p[P_Q][k];
+    case A_D: return P.c[C_D][k];
+    case A_TAU: return P.c[C_TAU][k];
+    case A_SX: return P.c[C_SX][k];
+    case A_SY: return P.c[C_SY][k];
+    default: return P.c[C_SZ][k];
+  }
+}
+
+
+}  // namespace cueva

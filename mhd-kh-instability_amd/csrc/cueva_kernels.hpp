@@ -1,4 +1,4 @@
-// Kernels of the CUEVA step for gfx950 (FP64 path).
+// Kernels of the CUEVA step for gfx950.
 //
 // Two ways through one step share these kernels:
 //  - one launch per reference subroutine (the drop-in entry points of
@@ -9,82 +9,10 @@
 //    recovery.  Both apply the same operations in the same order.
 #pragma once
 
-#include "cueva_device.hpp"
+#include "cueva_common.hpp"
+#include "cueva_faces.hpp"
 
 namespace cueva {
-
-// ---- field planes --------------------------------------------------------
-// primitive/EM planes, restart order (22_writer.f95:1116-1172)
-enum { P_PSI, P_PHI, P_BX, P_BY, P_BZ, P_VX, P_VY, P_VZ, P_EX, P_EY, P_EZ, P_Q, P_RHO, P_P, P_N };
-// startconvar outputs (06_conserved_var.f95)
-enum { C_D, C_TAU, C_SX, C_SY, C_SZ, C_W, C_N };
-// stage planes (the l-indexed arrays of fourvectors.f95, one stage at a time)
-enum { S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_EXA, S_EYA, S_EZA, S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ,
-       S_EX, S_EY, S_EZ, S_JX, S_JY, S_JZ, S_N };
-// accumulated fields of sumint/sumfin (09_sumint.f95, 18_sumfin.f95)
-enum { A_PSI, A_PHI, A_BX, A_BY, A_BZ, A_EX, A_EY, A_EZ, A_Q, A_D, A_TAU, A_SX, A_SY, A_SZ, A_N };
-// stage right-hand side: 14 flux divergences (A_* order) + 3 stiff sources
-enum { R_SEX = 14, R_SEY = 15, R_SEZ = 16, R_N = 17 };
-
-constexpr int MAXL = 4;
-
-struct Planes {
-  double* p[P_N];
-  double* c[C_N];
-  double* s[S_N];
-  double* fx[FX_N];
-  double* fy[FY_N];
-  double* rhs[MAXL][R_N];
-  double* ints[A_N];
-  double* fins[A_N];
-};
-
-struct Scheme {
-  double dx, dy, dt, sigma, kpsi, kphi, gamma;
-  double inv_dx, inv_dy;
-  int div_exact;  // dx, dy powers of two: x/d == x*(1/d)
-  int lmax, flux_solver, order;
-  double wb[MAXL], wbt[MAXL], Ab[MAXL][MAXL], Abt[MAXL][MAXL];
-  int hllc_seed_ref;
-  int crmath;
-};
-
-// status words in device memory
-struct DevStatus {
-  int err;          // first error code (0 none)
-  int err_i, err_j; // where
-  int pad;
-  unsigned long long cnt[3];  // contador_r, contador_p, contador_v
-  unsigned long long fallback;
-};
-
-__device__ inline void report(DevStatus* st, int code, int i, int j) {
-  if (atomicCAS(&st->err, 0, code) == 0) {
-    st->err_i = i;
-    st->err_j = j;
-  }
-}
-
-// U^n of accumulated field a (in A_* order) at offset k
-__device__ inline double un_value(const Planes& P, int a, int64_t k) {
-  switch (a) {
-    case A_PSI: return P.p[P_PSI][k];
-    case A_PHI: return P.p[P_PHI][k];
-    case A_BX: return P.p[P_BX][k];
-    case A_BY: return P.p[P_BY][k];
-    case A_BZ: return P.p[P_BZ][k];
-    case A_EX: return P.p[P_EX][k];
-    case A_EY: return P.p[P_EY][k];
-    case A_EZ: return P.p[P_EZ][k];
-    case A_Q: return P.p[P_Q][k];
-    case A_D: return P.c[C_D][k];
-    case A_TAU: return P.c[C_TAU][k];
-    case A_SX: return P.c[C_SX][k];
-    case A_SY: return P.c[C_SY][k];
-    default: return P.c[C_SZ][k];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // ghost fill: every ghost point copies the interior point of the index map
 // ---------------------------------------------------------------------------
@@ -249,150 +177,31 @@ __global__ void k_c2p(Geom g, Planes P, Scheme S, DevStatus* st) {
 }
 
 // ---------------------------------------------------------------------------
-// faces: reconstruction + Riemann solve for the face pair (i,j), i.e. the x
-// face between (i,j),(i+1,j) and the y face between (i,j),(i,j+1), for
-// i in [-1,imax], j in [-1,jmax] (16_*_flow_mpx.f95 face loops)
-// ---------------------------------------------------------------------------
-template <int ORD>
-__device__ inline void face_rec(const double* __restrict__ a, int64_t c, int64_t st, double& L, double& R) {
-  constexpr int K = MPc<ORD>::K;
-  double v[2 * K + 2];
-#pragma unroll
-  for (int t = 0; t < 2 * K + 2; t++) v[t] = a[c + (int64_t)(t - K) * st];
-  mp_face<ORD>(v, L, R);
-}
-template <int ORD>
-__device__ inline void face_rec_pos(const double* __restrict__ a, int64_t c, int64_t st, double& L, double& R) {
-  constexpr int K = MPc<ORD>::K;
-  double v[2 * K + 2];
-#pragma unroll
-  for (int t = 0; t < 2 * K + 2; t++) v[t] = a[c + (int64_t)(t - K) * st];
-  mp_face_pos<ORD>(v, L, R);
-}
-
-// reconstruct every non-kinematic quantity of one face (slots 1-20, 26/27/63)
-template <int ORD, int DIR>
-__device__ inline void face_fields(const Planes& P, int64_t c, int64_t st, Side<double>& L, Side<double>& R) {
-  face_rec<ORD>(P.s[S_PSI], c, st, L.psi, R.psi);
-  face_rec<ORD>(P.s[S_PHI], c, st, L.phi, R.phi);
-  face_rec<ORD>(P.s[S_EX], c, st, L.ex, R.ex);
-  face_rec<ORD>(P.s[S_EY], c, st, L.ey, R.ey);
-  face_rec<ORD>(P.s[S_EZ], c, st, L.ez, R.ez);
-  face_rec<ORD>(P.s[S_BX], c, st, L.bx, R.bx);
-  face_rec<ORD>(P.s[S_BY], c, st, L.by, R.by);
-  face_rec<ORD>(P.s[S_BZ], c, st, L.bz, R.bz);
-  face_rec<ORD>(P.s[S_Q], c, st, L.q, R.q);
-  face_rec<ORD>(P.s[S_D], c, st, L.d, R.d);
-  face_rec<ORD>(P.s[S_TAU], c, st, L.tau, R.tau);
-  face_rec<ORD>(P.s[S_SX], c, st, L.sx, R.sx);
-  face_rec<ORD>(P.s[S_SY], c, st, L.sy, R.sy);
-  face_rec<ORD>(P.s[S_SZ], c, st, L.sz, R.sz);
-  face_rec<ORD>(P.s[DIR == 0 ? S_JX : S_JY], c, st, L.jn, R.jn);
-}
-
-template <int ORD, int SOLVER>
-__global__ void __launch_bounds__(256) k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
-  if (i > g.imax || j > g.jmax) return;
-  const int64_t c = g.at(i, j);
-  const int64_t sx = 1, sy = g.pitch;
-  Side<double> xl, xr, yl, yr;
-  // kinematics of both faces first: the fallbacks couple them
-  face_rec<ORD>(P.p[P_VX], c, sx, xl.vx, xr.vx);
-  face_rec<ORD>(P.p[P_VY], c, sx, xl.vy, xr.vy);
-  face_rec<ORD>(P.p[P_VZ], c, sx, xl.vz, xr.vz);
-  face_rec<ORD>(P.p[P_VX], c, sy, yl.vx, yr.vx);
-  face_rec<ORD>(P.p[P_VY], c, sy, yl.vy, yr.vy);
-  face_rec<ORD>(P.p[P_VZ], c, sy, yl.vz, yr.vz);
-  face_rec_pos<ORD>(P.p[P_P], c, sx, xl.p, xr.p);
-  face_rec_pos<ORD>(P.p[P_RHO], c, sx, xl.rho, xr.rho);
-  face_rec_pos<ORD>(P.p[P_P], c, sy, yl.p, yr.p);
-  face_rec_pos<ORD>(P.p[P_RHO], c, sy, yl.rho, yr.rho);
-  // superluminal fallback, 16_hll_flow_mpx.f95:1392-1461 (same in hllc;
-  // hlle only prints, 16_hlle_flow_mpx.f95:2270-2300)
-  if (SOLVER != 6) {
-    double v2xl = (sq(xl.vx) + sq(xl.vy)) + sq(xl.vz);
-    double v2xr = (sq(xr.vx) + sq(xr.vy)) + sq(xr.vz);
-    double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
-    double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
-    if (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0) {
-      const int64_t cm = g.at(i - 1, j), cj = g.at(i, j - 1);
-      xl.vx = P.p[P_VX][c]; xl.vy = P.p[P_VY][c]; xl.vz = P.p[P_VZ][c];
-      xr.vx = P.p[P_VX][cm]; xr.vy = P.p[P_VY][cm]; xr.vz = P.p[P_VZ][cm];
-      yl.vx = P.p[P_VX][c]; yl.vy = P.p[P_VY][c]; yl.vz = P.p[P_VZ][c];
-      yr.vx = P.p[P_VX][cj]; yr.vy = P.p[P_VY][cj]; yr.vz = P.p[P_VZ][cj];
-    }
-  }
-  if (SOLVER == 7) {  // density fix-up, 16_hllc_flow_mpx.f95:2663-2693
-    double rc = 0.1 * P.p[P_RHO][c];
-    if (yl.rho < 0.0) yl.rho = rc;
-    else if (yr.rho < 0.0) yr.rho = rc;
-    else if (xl.rho < 0.0) xl.rho = rc;
-    else if (xr.rho < 0.0) xr.rho = rc;
-  }
-  constexpr bool HW = SOLVER == 6;
-  // HLLE: first Newton point of the x contact solve in the reference build
-  // is the y face's (E x B)_y; that of the y solve is the x solve's last iterate
-  double seed_x = 0.0, seed_y = 0.0;
-  if (SOLVER == 6 && S.hllc_seed_ref) {
-    face_rec<ORD>(P.s[S_EX], c, sy, yl.ex, yr.ex);
-    face_rec<ORD>(P.s[S_EZ], c, sy, yl.ez, yr.ez);
-    face_rec<ORD>(P.s[S_BX], c, sy, yl.bx, yr.bx);
-    face_rec<ORD>(P.s[S_BZ], c, sy, yl.bz, yr.bz);
-    seed_x = hlle_y_ery(yl, yr);
-  }
-  // x face
-  {
-    face_fields<ORD, 0>(P, c, sx, xl, xr);
-    side_physics<0, HW>(S.gamma, xl);
-    side_physics<0, HW>(S.gamma, xr);
-    double f[FX_N];
-    if (SOLVER == 7) {
-      int code = hllc_x(xl, xr, f, S.hllc_seed_ref != 0);
-      if (code) report(st, code, i, j);
-    } else if (SOLVER == 6) {
-      int code = hlle_face<0>(S.gamma, xl, xr, f, S.hllc_seed_ref != 0, seed_x, seed_y);
-      if (code) report(st, code, i, j);
-    } else {
-      hll_x(xl, xr, f);
-    }
-#pragma unroll
-    for (int t = 0; t < FX_N; t++) P.fx[t][c] = f[t];
-  }
-  // y face
-  {
-    face_fields<ORD, 1>(P, c, sy, yl, yr);
-    side_physics<1, HW>(S.gamma, yl);
-    side_physics<1, HW>(S.gamma, yr);
-    double f[FY_N];
-    if (SOLVER == 7) {
-      int code = hllc_y(yl, yr, f, S.hllc_seed_ref != 0);
-      if (code) report(st, code, i, j);
-    } else if (SOLVER == 6) {
-      double unused;
-      int code = hlle_face<1>(S.gamma, yl, yr, f, S.hllc_seed_ref != 0, seed_y, unused);
-      if (code) report(st, code, i, j);
-    } else {
-      hll_y(yl, yr, f);
-    }
-#pragma unroll
-    for (int t = 0; t < FY_N; t++) P.fy[t][c] = f[t];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // flux differences and stiff sources, 17_fluxandsource.f95:442-542
 // ---------------------------------------------------------------------------
 struct Rhs { double r[R_N]; };
 
-__device__ inline double ddx(const Scheme& S, double a) { return S.div_exact ? a * S.inv_dx : a / S.dx; }
-__device__ inline double ddy(const Scheme& S, double a) { return S.div_exact ? a * S.inv_dy : a / S.dy; }
+// face-flux planes of the flux arithmetic FT (double, or float in the mixed form)
+template <typename FT> __device__ inline const FT* flux_x(const Planes& P, int a);
+template <typename FT> __device__ inline const FT* flux_y(const Planes& P, int a);
+template <> __device__ inline const double* flux_x<double>(const Planes& P, int a) { return P.fx[a]; }
+template <> __device__ inline const double* flux_y<double>(const Planes& P, int a) { return P.fy[a]; }
+template <> __device__ inline const float* flux_x<float>(const Planes& P, int a) { return P.fxf[a]; }
+template <> __device__ inline const float* flux_y<float>(const Planes& P, int a) { return P.fyf[a]; }
 
+template <typename FT> __device__ inline FT ddx(const Scheme& S, FT a) {
+  return S.div_exact ? a * (FT)S.inv_dx : a / (FT)S.dx;
+}
+template <typename FT> __device__ inline FT ddy(const Scheme& S, FT a) {
+  return S.div_exact ? a * (FT)S.inv_dy : a / (FT)S.dy;
+}
+
+// flux differences in FT, combined with the FP64 sources in FP64
+template <typename FT>
 __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k) {
   const int64_t kw = g.at(i - 1, j), ks = g.at(i, j - 1);
-#define DXF(a) ddx(S, P.fx[a][k] - P.fx[a][kw])
-#define DYF(a) ddy(S, P.fy[a][k] - P.fy[a][ks])
+#define DXF(a) ((double)ddx<FT>(S, flux_x<FT>(P, a)[k] - flux_x<FT>(P, a)[kw]))
+#define DYF(a) ((double)ddy<FT>(S, flux_y<FT>(P, a)[k] - flux_y<FT>(P, a)[ks]))
   double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
   double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
   double q = P.s[S_Q][k];
@@ -418,9 +227,10 @@ __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& 
   return o;
 }
 
+template <typename FT>
 __global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l) {
   INTERIOR_IJ
-  Rhs o = flux_source(g, P, S, i, j, k);
+  Rhs o = flux_source<FT>(g, P, S, i, j, k);
 #pragma unroll
   for (int t = 0; t < R_N; t++) P.rhs[l][t][k] = o.r[t];
 }
@@ -478,10 +288,11 @@ __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
 }
 
 // Last stage + sumfin + varfin + electricfield + varprimitivecardano.
+template <typename FT>
 __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
   INTERIOR_IJ
   const int L = S.lmax - 1;
-  Rhs last = flux_source(g, P, S, i, j, k);
+  Rhs last = flux_source<FT>(g, P, S, i, j, k);
   double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
   double ex0 = P.p[P_EX][k], ey0 = P.p[P_EY][k], ez0 = P.p[P_EZ][k];
   double bx0 = P.p[P_BX][k], by0 = P.p[P_BY][k], bz0 = P.p[P_BZ][k];

@@ -1,0 +1,33 @@
+// FP32 face kernels of the mixed form: reconstruction, Riemann solve and
+// flux writes in FP32, built in their own translation unit with contraction
+// to fused multiply-add allowed (the FP64 path is built without it, as the
+// reference build evaluates products and sums separately).
+#include <hip/hip_runtime.h>
+
+#include "cueva_faces.hpp"
+
+namespace cueva {
+
+template <int ORD>
+static hipError_t launch_ord(dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P, const Scheme& S,
+                             DevStatus* st) {
+  if (S.flux_solver == 7)
+    hipLaunchKernelGGL((k_faces<ORD, 7, float>), grid, blk, 0, s, g, P, S, st);
+  else if (S.flux_solver == 6)
+    hipLaunchKernelGGL((k_faces<ORD, 6, float>), grid, blk, 0, s, g, P, S, st);
+  else
+    hipLaunchKernelGGL((k_faces<ORD, 5, float>), grid, blk, 0, s, g, P, S, st);
+  return hipGetLastError();
+}
+
+// one launch of the FP32 face kernel over the face pairs [-1,imax] x [-1,jmax]
+hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
+                             const Scheme& S, DevStatus* st) {
+  switch (rec_order) {
+    case 5: return launch_ord<5>(grid, blk, s, g, P, S, st);
+    case 7: return launch_ord<7>(grid, blk, s, g, P, S, st);
+    default: return launch_ord<9>(grid, blk, s, g, P, S, st);
+  }
+}
+
+}  // namespace cueva

@@ -32,14 +32,26 @@ def _run(cmd, cwd=None):
     return res
 
 
+# translation units and their contraction mode: the FP64 path multiplies and
+# adds separately as the reference build does; the FP32 face kernels of the
+# mixed form may fuse
+UNITS = [("cueva.hip", "off"), ("cueva_mixed.hip", "fast"), ("kh_init.cpp", "off")]
+
+
 def build_hip(verbose=False):
     """Compile libcueva.so (kernels, C ABI, host KH generator)."""
     os.makedirs(LIB, exist_ok=True)
+    bdir = os.path.join(LIB, "_obj")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for src, contract in UNITS:
+        obj = os.path.join(bdir, src.rsplit(".", 1)[0] + ".o")
+        flags = [f for f in HIP_FLAGS if not f.startswith("-ffp-contract")] + ["-ffp-contract=" + contract]
+        _run([HIPCC] + flags + ["-c", "-I", INCLUDE, "-I", CSRC, "-o", obj, os.path.join(CSRC, src)])
+        objs.append(obj)
     out = os.path.join(LIB, "libcueva.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, "cueva.hip"), os.path.join(CSRC, "kh_init.cpp")]
-    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-I", INCLUDE, "-I", CSRC, "-o", tmp] + srcs
-    _run(cmd)
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + objs)
     os.replace(tmp, out)
     if verbose:
         print("built", out)
