@@ -188,6 +188,41 @@ int cueva_kernel_times(cueva_ctx *ctx, double ms[6], int64_t launches[6]);
 /* Milliseconds of nsteps fused steps timed with HIP events on the ctx stream. */
 int cueva_time_steps(cueva_ctx *ctx, int nsteps, double *ms);
 
+/* ---- learned start point of the primitive recovery ----------------------
+ * No reference counterpart.  A 3 -> 32 -> 32 -> 2 MLP (tanh, BF16 weights on
+ * the matrix cores, FP32 accumulation) maps the normalised features
+ * (log(D/C2), |S - E x B|/C2, log C2), C2 = tau - (E^2+B^2)/2, of a conserved
+ * state to (log W, log(p/C2)).  In c2p_mode CUEVA_C2P_LEARNED its W starts at
+ * most two FP64 Newton steps on the quartic of 12_varprimitive.f95:245-296;
+ * a result with relative quartic residual > 1e-10 or W < 1 falls back to the
+ * Cardano path (counted in the fallback counter), and the reference's
+ * pressure loop, fix-ups and velocities follow either way.
+ * Packed parameters, CUEVA_MLP_NPACK floats: W1[32][3] b1[32] W2[32][32]
+ * b2[32] W3[2][32] b3[2] (the CUEVA_MLP_NPARAM trainable ones), then
+ * in_mean[3] in_std[3] out_mean[2] out_std[2]. */
+#define CUEVA_MLP_NPARAM 1250
+#define CUEVA_MLP_NPACK 1260
+typedef struct cueva_mlp cueva_mlp;
+
+/* Attach trained parameters to a solver (required before a step in
+ * CUEVA_C2P_LEARNED mode). */
+int cueva_set_c2p_params(cueva_ctx *ctx, const float *packed);
+
+/* Training on the device: FP32 master weights, Adam, mean squared error on
+ * normalised targets.  Forward on the matrix cores, backward through LDS. */
+int cueva_mlp_create(cueva_mlp **m);
+int cueva_mlp_destroy(cueva_mlp *m);
+int cueva_mlp_set_params(cueva_mlp *m, const float *packed);  /* also resets Adam */
+int cueva_mlp_get_params(cueva_mlp *m, float *packed);
+/* n samples: x[n][3] normalised features, y[n][2] normalised targets */
+int cueva_mlp_set_data(cueva_mlp *m, const float *x, const float *y, int64_t n);
+/* gradient (CUEVA_MLP_NPARAM floats) and mean loss over samples [off, off+cnt) */
+int cueva_mlp_grad(cueva_mlp *m, int64_t off, int64_t cnt, float *grad, double *loss);
+/* steps Adam updates over consecutive batches; loss = mean batch loss */
+int cueva_mlp_train(cueva_mlp *m, int64_t batch, int steps, float lr, double *loss);
+/* normalised outputs y[n][2] of normalised inputs x[n][3] */
+int cueva_mlp_forward(cueva_mlp *m, const float *x, int64_t n, float *y);
+
 /* Seeded Kelvin-Helmholtz initial data on the host (no reference counterpart:
  * the reference has no such setup).  Periodic x in [-Lx/2, Lx/2], outflow y in
  * [-Ly/2, Ly/2]; rho = p = 1, vx = v0 tanh(y/0.01) + U(-1e-4,1e-4) noise,

@@ -16,6 +16,9 @@ namespace cueva {
 // FP32 face kernels (cueva_mixed.hip)
 hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
                              const Scheme& S, DevStatus* st);
+// learned primitive recovery (cueva_mlp.hip)
+hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
+                              DevStatus* st);
 }
 
 using namespace cueva;
@@ -30,6 +33,8 @@ struct cueva_ctx {
   int64_t bytes;
   DevStatus* dst;
   double* red;  // reduction partials (RED_BLOCKS * G_N) and results (G_N), after dst
+  float* mlpw;  // packed learned-recovery parameters (CUEVA_MLP_NPACK), after red
+  int has_mlp;
   int rec_order;
   int err;  // sticky host-side error
   // kernel timing (cueva_set_profiling)
@@ -164,7 +169,7 @@ static int check_params(const cueva_params* p) {
   if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
   if (p->precision != CUEVA_FP64 && p->precision != CUEVA_MIXED) return CUEVA_E_ARG;
-  if (p->c2p_mode != CUEVA_C2P_CARDANO) return CUEVA_E_ARG;
+  if (p->c2p_mode != CUEVA_C2P_CARDANO && p->c2p_mode != CUEVA_C2P_LEARNED) return CUEVA_E_ARG;
   if (!(p->longx > 0.0) || !(p->longy > 0.0) || !(p->cfl > 0.0) || !(p->sigma >= 0.0)) return CUEVA_E_ARG;
   return CUEVA_OK;
 }
@@ -211,13 +216,15 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   }
   const size_t status_bytes = (sizeof(DevStatus) + 255) / 256 * 256;
   const size_t red_bytes = (size_t)(RED_BLOCKS + 1) * G_N * sizeof(double);
-  if (hipMalloc(&c->dst, status_bytes + red_bytes) != hipSuccess) {
+  const size_t mlp_bytes = (size_t)CUEVA_MLP_NPACK * sizeof(float);
+  if (hipMalloc(&c->dst, status_bytes + red_bytes + mlp_bytes) != hipSuccess) {
     hipFree(c->base);
     hipStreamDestroy(c->stream);
     delete c;
     return CUEVA_E_NOMEM;
   }
   c->red = (double*)((char*)c->dst + status_bytes);
+  c->mlpw = (float*)((char*)c->red + red_bytes);
   double* q = c->base;
   auto take = [&]() { double* r = q; q += g.plane; return r; };
   for (int k = 0; k < P_N; k++) c->P.p[k] = take();
@@ -495,10 +502,30 @@ static int read_status(cueva_ctx* c, DevStatus* h) {
   return CUEVA_OK;
 }
 
+static int launch_c2p(cueva_ctx* c) {
+  if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
+    if (!c->has_mlp) return CUEVA_E_ARG;
+    prof_begin(c, K_OTHER);
+    hipError_t e = launch_c2p_learned(c->stream, c->g, c->P, c->S, c->mlpw, c->dst);
+    prof_end(c);
+    return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
+  }
+  LAUNCH(K_OTHER, k_c2p, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
+  return launch_ok() ? CUEVA_E_HIP : CUEVA_OK;
+}
+
+extern "C" int cueva_set_c2p_params(cueva_ctx* c, const float* packed) {
+  if (!c || !packed) return CUEVA_E_ARG;
+  HIPCHK(hipMemcpyAsync(c->mlpw, packed, (size_t)CUEVA_MLP_NPACK * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->has_mlp = 1;
+  return CUEVA_OK;
+}
+
 extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t* fallback) {
   if (!c || l_swap != 1) return CUEVA_E_ARG;
-  LAUNCH(K_OTHER, k_c2p, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
-  if (launch_ok()) return CUEVA_E_HIP;
+  int lrc = launch_c2p(c);
+  if (lrc) return lrc;
   DevStatus h;
   int rc = read_status(c, &h);
   if (rc) return rc;
@@ -528,6 +555,15 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
           LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
         else
           LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
+      } else if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
+        if (!c->has_mlp) return CUEVA_E_ARG;
+        if (mixed)
+          LAUNCH(K_FINAL, (k_final_fused<float, false>), interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
+        else
+          LAUNCH(K_FINAL, (k_final_fused<double, false>), interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
+        if (launch_ok()) return CUEVA_E_HIP;
+        int crc = launch_c2p(c);
+        if (crc) return crc;
       } else {
         if (mixed)
           LAUNCH(K_FINAL, k_final_fused<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);

@@ -746,7 +746,7 @@ struct C2PIn {
 struct C2POut { double vx, vy, vz, rho, p; int fix_rho, fix_p, fix_v; };
 
 // Quartic coefficients in W (12_varprimitive.f95:245-296)
-struct Quartic { double e2b2, g1, v2old, b3, b2, b1, b0, d2, d1, d0; };
+struct Quartic { double e2b2, g1, v2old, b3, b2, b1, b0, d2, d1, d0, c1, c2; };
 
 __device__ inline Quartic c2p_quartic(double gamma, const C2PIn& in) {
   Quartic qd;
@@ -756,6 +756,8 @@ __device__ inline Quartic c2p_quartic(double gamma, const C2PIn& in) {
   double C1 = (sq((in.sx - in.ey * in.bz) + in.ez * in.by) + sq((in.sy - in.ez * in.bx) + in.ex * in.bz)) +
               sq((in.sz - in.ex * in.by) + in.ey * in.bx);
   double C2 = in.tau - qd.e2b2;
+  qd.c1 = C1;
+  qd.c2 = C2;
   double g1 = qd.g1, D = in.d;
   double a4 = C1 - sq(C2);
   double a3 = ((2.0 * g1) * C2) * D;

@@ -287,8 +287,10 @@ __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
   P.s[S_JX][k] = jx; P.s[S_JY][k] = jy; P.s[S_JZ][k] = jz;
 }
 
-// Last stage + sumfin + varfin + electricfield + varprimitivecardano.
-template <typename FT>
+// Last stage + sumfin + varfin + electricfield + varprimitivecardano
+// (C2P false: without the primitive recovery, which then runs as its own
+// kernel, the learned one).
+template <typename FT, bool C2P = true>
 __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
   INTERIOR_IJ
   const int L = S.lmax - 1;
@@ -320,7 +322,7 @@ __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
   P.p[P_Q][k] = nu[A_Q];
   P.c[C_D][k] = nu[A_D]; P.c[C_TAU][k] = nu[A_TAU];
   P.c[C_SX][k] = nu[A_SX]; P.c[C_SY][k] = nu[A_SY]; P.c[C_SZ][k] = nu[A_SZ];
-  c2p_store(g, P, S, st, i, j, k);
+  if (C2P) c2p_store(g, P, S, st, i, j, k);
 }
 
 }  // namespace cueva

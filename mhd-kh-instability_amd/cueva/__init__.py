@@ -5,6 +5,7 @@ Host-side Python mirror of the reference's per-stage subroutine interface
 """
 from .abi import (  # noqa: F401
     BC_COPY, BC_PERIODIC, C2P_CARDANO, C2P_LEARNED, CONS_NAMES, FP64, GLOBAL_NAMES, HLL, HLLC, HLLE, MIXED,
-    PRIM_NAMES, CuevaError, Params, Solver, default_params, derived, kh_init, load, plane_shape,
+    MLP_NPACK, MLP_NPARAM, PRIM_NAMES, CuevaError, Mlp, Params, Solver, default_params, derived, kh_init, load,
+    plane_shape,
 )
 from .problems import kh_params, kh_problem  # noqa: F401

@@ -96,6 +96,15 @@ SIGNATURES = {
     "cueva_set_profiling": (_i, [_c, _i]),
     "cueva_kernel_times": (_i, [_c, _P(_d), _P(ctypes.c_int64)]),
     "cueva_time_steps": (_i, [_c, _i, _P(_d)]),
+    "cueva_set_c2p_params": (_i, [_c, _P(ctypes.c_float)]),
+    "cueva_mlp_create": (_i, [_P(_c)]),
+    "cueva_mlp_destroy": (_i, [_c]),
+    "cueva_mlp_set_params": (_i, [_c, _P(ctypes.c_float)]),
+    "cueva_mlp_get_params": (_i, [_c, _P(ctypes.c_float)]),
+    "cueva_mlp_set_data": (_i, [_c, _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int64]),
+    "cueva_mlp_grad": (_i, [_c, ctypes.c_int64, ctypes.c_int64, _P(ctypes.c_float), _P(_d)]),
+    "cueva_mlp_train": (_i, [_c, ctypes.c_int64, _i, ctypes.c_float, _P(_d)]),
+    "cueva_mlp_forward": (_i, [_c, _P(ctypes.c_float), ctypes.c_int64, _P(ctypes.c_float)]),
 }
 
 KERNEL_CLASSES = ("stage", "faces", "fluxsource", "final", "ghosts", "other")
@@ -154,6 +163,74 @@ def kh_init(imax, jmax, lx, ly, rotated, v0=0.5, bx0=0.01, bz0=0.0, seed=2026120
     out = np.zeros((14, jmax + 13, imax + 13))
     check(load().cueva_kh_init(imax, jmax, lx, ly, int(rotated), v0, bx0, bz0, seed, _dp(out)), "kh_init")
     return out
+
+
+MLP_NPARAM, MLP_NPACK = 1250, 1260
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Mlp:
+    """Device copy of the learned-recovery network with its trainer (cueva_mlp)."""
+
+    def __init__(self, packed=None):
+        self.lib = load()
+        self.m = ctypes.c_void_p()
+        check(self.lib.cueva_mlp_create(ctypes.byref(self.m)), "mlp_create")
+        if packed is not None:
+            self.set_params(packed)
+
+    def close(self):
+        if self.m:
+            self.lib.cueva_mlp_destroy(self.m)
+            self.m = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, packed):
+        packed = np.ascontiguousarray(packed, dtype=np.float32)
+        assert packed.shape == (MLP_NPACK,)
+        check(self.lib.cueva_mlp_set_params(self.m, _fp(packed)), "mlp_set_params")
+
+    def params(self):
+        out = np.zeros(MLP_NPACK, dtype=np.float32)
+        check(self.lib.cueva_mlp_get_params(self.m, _fp(out)), "mlp_get_params")
+        return out
+
+    def set_data(self, x, y):
+        self._x = np.ascontiguousarray(x, dtype=np.float32)
+        self._y = np.ascontiguousarray(y, dtype=np.float32)
+        assert self._x.shape[1:] == (3,) and self._y.shape == (self._x.shape[0], 2)
+        check(self.lib.cueva_mlp_set_data(self.m, _fp(self._x), _fp(self._y), self._x.shape[0]), "mlp_set_data")
+
+    def grad(self, off, cnt):
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_double()
+        check(self.lib.cueva_mlp_grad(self.m, int(off), int(cnt), _fp(g), ctypes.byref(loss)), "mlp_grad")
+        return g, loss.value
+
+    def train(self, batch, steps, lr):
+        loss = ctypes.c_double()
+        check(self.lib.cueva_mlp_train(self.m, int(batch), int(steps), float(lr), ctypes.byref(loss)), "mlp_train")
+        return loss.value
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], 2), dtype=np.float32)
+        check(self.lib.cueva_mlp_forward(self.m, _fp(x), x.shape[0], _fp(y)), "mlp_forward")
+        return y
 
 
 class Solver:
@@ -227,6 +304,12 @@ class Solver:
         ms = ctypes.c_double()
         check(self.lib.cueva_time_steps(self.ctx, int(n), ctypes.byref(ms)), "time_steps")
         return ms.value
+
+    # -- learned primitive recovery
+    def set_c2p_params(self, packed):
+        packed = np.ascontiguousarray(packed, dtype=np.float32)
+        assert packed.shape == (MLP_NPACK,), packed.shape
+        check(self.lib.cueva_set_c2p_params(self.ctx, _fp(packed)), "set_c2p_params")
 
     # -- diagnostics
     def globals(self):

@@ -35,7 +35,7 @@ def _run(cmd, cwd=None):
 # translation units and their contraction mode: the FP64 path multiplies and
 # adds separately as the reference build does; the FP32 face kernels of the
 # mixed form may fuse
-UNITS = [("cueva.hip", "off"), ("cueva_mixed.hip", "fast"), ("kh_init.cpp", "off")]
+UNITS = [("cueva.hip", "off"), ("cueva_mixed.hip", "fast"), ("cueva_mlp.hip", "off"), ("kh_init.cpp", "off")]
 
 
 def build_hip(verbose=False):

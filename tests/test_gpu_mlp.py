@@ -1,0 +1,111 @@
+"""Learned start point of the primitive recovery on the GPU.
+
+* The device forward (BF16 matrix cores, FP32 accumulation) and the device
+  gradient against the CPU restatement oracle/mlp_oracle.py, which rounds to
+  BF16 at the same points and is itself pinned by finite differences.
+* Training on the device reduces the loss.
+* With the shipped parameters a step in CUEVA_C2P_LEARNED mode gives the
+  Cardano step's fields to rounding, with a fallback rate below 1 %; the
+  same pipeline with an untrained network falls back on most cells (what the
+  trained network gains).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cueva
+from cueva import learned_c2p as lc
+from oracle import mlp_oracle as mo
+from parity import FIELDS, interior, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def random_packed(seed=0):
+    rng = np.random.default_rng(seed)
+    w = lc.init_weights(seed) + rng.normal(scale=0.05, size=cueva.MLP_NPARAM).astype(np.float32)
+    return lc.pack(w, np.zeros(3), np.ones(3), np.zeros(2), np.ones(2))
+
+
+def test_forward_matches_oracle():
+    rng = np.random.default_rng(1)
+    p = random_packed(1)
+    x = rng.normal(size=(4096 + 37, 3)).astype(np.float32)  # not a multiple of the wavefront
+    with cueva.Mlp(p) as m:
+        y = m.forward(x)
+    want, _ = mo.forward(p, x)
+    err = np.abs(y - want)
+    say("forward max err %.2e mean %.2e" % (err.max(), err.mean()))
+    assert err.max() <= 5e-3 and err.mean() <= 2e-4
+
+
+def test_gradient_matches_oracle():
+    rng = np.random.default_rng(2)
+    p = random_packed(2)
+    n = 2048 + 64
+    x = rng.normal(size=(n, 3)).astype(np.float32)
+    t = rng.normal(size=(n, 2)).astype(np.float32)
+    with cueva.Mlp(p) as m:
+        m.set_data(x, t)
+        g, loss = m.grad(64, 2048)
+    want_loss, want = mo.loss_and_grad(p, x[64:], t[64:])
+    say("loss %.6e vs %.6e" % (loss, want_loss))
+    assert abs(loss - want_loss) <= 1e-4 * want_loss
+    blocks = {"W1": (mo.OFF_W1, mo.OFF_B1), "b1": (mo.OFF_B1, mo.OFF_W2), "W2": (mo.OFF_W2, mo.OFF_B2),
+              "b2": (mo.OFF_B2, mo.OFF_W3), "W3": (mo.OFF_W3, mo.OFF_B3), "b3": (mo.OFF_B3, mo.NPARAM)}
+    for name, (a, b) in blocks.items():
+        rel = np.linalg.norm(g[a:b] - want[a:b]) / np.linalg.norm(want[a:b])
+        say("grad %s rel err %.2e" % (name, rel))
+        assert rel <= 2e-3, name
+
+
+def test_training_reduces_loss():
+    x, y = lc.synthetic(65536, seed=5)
+    xm, xs, ym, ys = x.mean(0), x.std(0), y.mean(0), y.std(0)
+    xn = ((x - xm) / xs).astype(np.float32)
+    yn = ((y - ym) / ys).astype(np.float32)
+    with cueva.Mlp(lc.pack(lc.init_weights(0), xm, xs, ym, ys)) as m:
+        m.set_data(xn, yn)
+        _, l0 = m.grad(0, 65536)
+        m.train(1024, 400, 3e-3)
+        _, l1 = m.grad(0, 65536)
+    say("loss %.3e -> %.3e" % (l0, l1))
+    assert l1 < 0.1 * l0
+
+
+def run_kh(c2p_mode, packed=None, steps=10, precision=0):
+    p, prims = cueva.kh_problem(128, 256, sigma=1.0e6, flux_solver=cueva.HLLC, rotated=True, c2p_mode=c2p_mode,
+                                precision=precision)
+    with cueva.Solver(p) as s:
+        if packed is not None:
+            s.set_c2p_params(packed)
+        s.upload(prims)
+        fb = 0
+        for _ in range(steps):
+            s.step(1)
+            _, f = s.step_counters()
+            fb += f
+        return interior(*s.download(), p), fb, p
+
+
+@pytest.mark.skipif(not os.path.exists(lc.PARAMS_FILE), reason="no trained parameters shipped")
+def test_learned_step_matches_cardano_with_few_fallbacks():
+    packed = lc.load_params()
+    ref, _, p = run_kh(cueva.C2P_CARDANO)
+    got, fb, _ = run_kh(cueva.C2P_LEARNED, packed)
+    cells = (p.imax + 1) * (p.jmax + 1) * 10
+    err = rel_l1(got, ref)
+    say("learned vs Cardano: max rel L1 %.2e; fallbacks %d of %d" % (max(err.values()), fb, cells))
+    assert max(err.values()) <= 1e-10
+    assert fb <= 0.01 * cells
+    # the same pipeline with an untrained network: most starts miss
+    _, fb0, _ = run_kh(cueva.C2P_LEARNED, random_packed(3), steps=1)
+    say("untrained network: fallbacks %d of %d" % (fb0, cells // 10))
+    assert fb0 > 10 * max(fb / 10, 1)
