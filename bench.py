@@ -1,11 +1,14 @@
 """Benchmark of one CUEVA 2D resistive-RMHD step on one MI355X.
 
-Default workload: the 2048x4096 Kelvin-Helmholtz shear layer (periodic x,
-outflow y), sigma = 1e6, MP5 + HLLC, IMEX SSP2(2,2,2), CFL 0.4, plain FP64.
-A step is one full time step of the whole grid; inputs are resident in HBM
-before timing starts.  Prints one JSON line.
+Workload: the 2048x4096 Kelvin-Helmholtz shear layer (periodic x, outflow
+y), sigma = 1e6, MP5 + HLLC, IMEX SSP2(2,2,2), CFL 0.4.  A step is one full
+time step of the whole grid; inputs are resident in HBM before timing
+starts.  Two forms are timed: plain FP64 (bitwise with the reference build)
+and the mixed form with the learned start of the primitive recovery; `value`
+is the latter, the plain one and the accuracy after 100 steps (relative L1
+of every field against plain FP64) are reported beside it.  One JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--form F] [--no-cpu]
   python bench.py --copy-calibration   # plain copy of known size, for rocprof counters
 """
 import argparse
@@ -26,9 +29,6 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 METRIC = "zone-updates/s, 2048x4096 KH: plain FP64 vs mixed+learned-c2p at equal L1"
 
-# compulsory bytes of the face kernel per face pair: 16 stage planes + 5
-# primitive planes (vx vy vz p rho) read once, 28 face-flux planes written
-FACES_BYTES_PER_PAIR = (16 + 5 + 28) * 8
 # compulsory bytes of one SSP2(2,2,2) step per zone-update (stage k reads U^n,
 # P^n and earlier stage right-hand sides, writes its own; the last stage
 # writes U^{n+1}, P^{n+1}): 91 doubles
@@ -100,13 +100,88 @@ def cpu_baseline(workload_cfg, budget_s=20.0):
             "sample": "oracle restatement, %dx%d, 2 steps, 1 thread" % (sample_ni, sample_nj)}
 
 
+# the forms the metric compares; crmath 1 = the reference's correctly rounded
+# acos/cos/cbrt (bitwise form), 0 = device math library
+FORMS = {
+    "plain_fp64": dict(precision=0, crmath=1, c2p_mode=0),
+    "mixed": dict(precision=1, crmath=0, c2p_mode=0),
+    "mixed_learned": dict(precision=1, crmath=0, c2p_mode=1),
+}
+# compulsory bytes per face pair of k_faces: 16 stage + 5 primitive FP64
+# planes read; 28 face-flux planes written (FP64, or FP32 in the mixed forms)
+FACES_BYTES = {"plain_fp64": (16 + 5) * 8 + 28 * 8, "mixed": (16 + 5) * 8 + 28 * 4,
+               "mixed_learned": (16 + 5) * 8 + 28 * 4}
+ACCURACY_STEPS = 100
+
+
+def load_packed():
+    from cueva import learned_c2p
+    if os.path.exists(learned_c2p.PARAMS_FILE):
+        return learned_c2p.load_params()
+    return None
+
+
+def time_form(cfg, form, steps, warmup, packed):
+    import cueva
+    from cueva import problems
+    p, prims = problems.config(cfg, **FORMS[form])
+    s = cueva.Solver(p)
+    if packed is not None:
+        s.set_c2p_params(packed)
+    s.upload(prims)
+    del prims
+    log("%s: warmup %d steps" % (form, warmup))
+    s.step(warmup)
+    s.step_counters()
+    s.sync()
+    s.set_profiling(True)
+    log("%s: timed %d steps" % (form, steps))
+    ms = s.time_steps(steps)
+    s.sync()
+    kt = s.kernel_times()
+    s.set_profiling(False)
+    cnt, fallback = s.step_counters()
+    s.close()
+    zones = p.imax * p.jmax
+    return {"ms_per_step": ms / steps, "value": zones * steps / (ms / 1e3),
+            "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()},
+            "kernel_launches": {k: v[1] for k, v in kt.items()},
+            "fixups": cnt, "c2p_fallbacks": fallback, "p": p}
+
+
+def accuracy(cfg, form, packed, steps=ACCURACY_STEPS):
+    """Relative L1 of every field of `form` against plain FP64 after `steps` steps."""
+    import numpy as np
+    import cueva
+    from cueva import problems
+    out = {}
+    for f in ("plain_fp64", form):
+        p, prims = problems.config(cfg, **FORMS[f])
+        with cueva.Solver(p) as s:
+            if packed is not None:
+                s.set_c2p_params(packed)
+            s.upload(prims)
+            s.step(steps)
+            s.step_counters()
+            pr, co = s.download()
+        sl = (slice(None), slice(6, 6 + p.jmax + 1), slice(6, 6 + p.imax + 1))
+        out[f] = np.concatenate([pr, co])[sl]
+        del pr, co
+    names = list(cueva.PRIM_NAMES) + list(cueva.CONS_NAMES)
+    a, b = out[form], out["plain_fp64"]
+    return {n: float(np.abs(a[k] - b[k]).sum() / max(np.abs(b[k]).sum(), 1e-300)) for k, n in enumerate(names)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--form", default=None, choices=sorted(FORMS),
+                    help="time one form only (default: plain FP64 and mixed + learned c2p)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -118,7 +193,6 @@ def main():
         return
 
     import torch
-    import cueva
     from cueva import problems
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -126,60 +200,56 @@ def main():
     cfg = args.config
     ni, nj, sigma, fs, rec, order, v0, bx, bz = problems.CONFIGS[cfg]
     log("workload %s: %dx%d KH, sigma=%g, solver=%d, mp=%d, order=%d" % (cfg, ni, nj, sigma, fs, 5 + 2 * rec, order))
-    p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, v0=v0, bx0=bx,
-                                bz0=bz)
-    s = cueva.Solver(p)
-    s.upload(prims)
-    del prims
-    zones = ni * nj
-    log("warmup %d steps" % args.warmup)
-    s.step(args.warmup)
-    s.step_counters()
-    torch.cuda.synchronize()
-    s.sync()
-    s.set_profiling(True)
-    log("timed %d steps" % args.steps)
-    ms = s.time_steps(args.steps)
-    torch.cuda.synchronize()
-    s.sync()
-    kt = s.kernel_times()
-    s.set_profiling(False)
-    cnt, _ = s.step_counters()
-    t_step = ms / args.steps
-    value = zones * args.steps / (ms / 1e3)
-    dom = max(kt, key=lambda k: kt[k][0])
-    dom_ms, dom_n = kt[dom]
-    dom_avg_s = dom_ms / dom_n / 1e3
-    pairs = (p.imax + 2) * (p.jmax + 2)
-    if dom == "faces":
-        alg_bytes = pairs * FACES_BYTES_PER_PAIR
+    packed = load_packed()
+    if args.form:
+        forms = [args.form]
     else:
-        alg_bytes = None
+        forms = ["plain_fp64", "mixed_learned"] if packed is not None else ["plain_fp64", "mixed"]
+    if "mixed_learned" in forms and packed is None:
+        sys.exit("no trained parameters (cueva/c2p_mlp.json) for the learned form")
+    res = {f: time_form(cfg, f, args.steps, args.warmup, packed) for f in forms}
+    head = forms[-1]
+    r = res[head]
+    p = r.pop("p")
+    for f in forms[:-1]:
+        res[f].pop("p")
+    kt = r["kernel_ms_per_step"]
+    dom = max(kt, key=lambda k: kt[k])
+    dom_avg_ms = kt[dom] / (r["kernel_launches"][dom] / args.steps)
+    pairs = (p.imax + 2) * (p.jmax + 2)
+    alg_bytes = pairs * FACES_BYTES[head] if dom == "faces" else None
+    achieved = alg_bytes / (dom_avg_ms / 1e3) / 1e9 if alg_bytes else None
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, cfg))
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (dom, cfg, head))
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("bytes_per_launch")
-    achieved = alg_bytes / dom_avg_s / 1e9 if alg_bytes else None
+    form_name = {"plain_fp64": "plain FP64", "mixed": "mixed FP32/FP64, Cardano c2p",
+                 "mixed_learned": "mixed FP32/FP64 + learned c2p (BF16 MLP on MFMA)"}[head]
     out = {
-        "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": t_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded KH initial data)",
-        "config": {"workload": "%s: KH %dx%d, sigma=%g, MP%d + %s, IMEX %s, CFL 0.4, plain FP64" % (
+        "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if head == "plain_fp64" else "f32+f64 (bf16 MLP)",
+        "data": "synthetic (seeded KH initial data)",
+        "config": {"workload": "%s: KH %dx%d, sigma=%g, MP%d + %s, IMEX %s, CFL 0.4" % (
             cfg, ni, nj, sigma, 5 + 2 * rec, {5: "HLL", 6: "HLLE", 7: "HLLC"}[fs],
-            "SSP2(2,2,2)" if order == 2 else "SSP3(4,3,3)", ), "form": "plain FP64"},
+            "SSP2(2,2,2)" if order == 2 else "SSP3(4,3,3)"), "form": form_name},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms / dom_n},
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_avg_ms},
         "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_ZONE,
-                          "achieved_GBs": value * STEP_BYTES_PER_ZONE / 1e9,
-                          "frac": value * STEP_BYTES_PER_ZONE / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
-        "fixups": cnt,
+                          "achieved_GBs": r["value"] * STEP_BYTES_PER_ZONE / 1e9,
+                          "frac": r["value"] * STEP_BYTES_PER_ZONE / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms_per_step": kt, "fixups": r["fixups"], "c2p_fallbacks": r["c2p_fallbacks"],
+        "forms": {f: {k: v for k, v in res[f].items() if k != "kernel_launches"} for f in forms},
     }
+    if "plain_fp64" in res and head != "plain_fp64":
+        out["speedup_vs_plain_fp64"] = r["value"] / res["plain_fp64"]["value"]
+        if not args.no_accuracy:
+            log("accuracy: %d steps of %s against plain FP64" % (ACCURACY_STEPS, head))
+            out["rel_l1_vs_plain_fp64_after_%d_steps" % ACCURACY_STEPS] = accuracy(cfg, head, packed)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(cfg)
-    s.close()
     print(json.dumps(out))
 
 
