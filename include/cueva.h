@@ -233,6 +233,13 @@ int cueva_mlp_forward(cueva_mlp *m, const float *x, int64_t n, float *y);
 int cueva_kh_init(int imax, int jmax, double lx, double ly, int rotated, double v0,
                   double bx0, double bz0, uint64_t seed, double *prims14);
 
+/* Circularly polarised Alfven wave along x, uniform in y, for a periodic
+ * box (03_init_var_primitive.f95:129-160 extended to 2D): rho0, p0, guide
+ * field b0, amplitude eta; *va receives the exact Alfven speed (one period
+ * is Lx / va).  Ghosts are filled periodically. */
+int cueva_cpaw_init(int imax, int jmax, double lx, double gamma, double rho0, double p0, double b0, double eta,
+                    double *prims14, double *va);
+
 #ifdef __cplusplus
 }
 #endif

@@ -434,7 +434,7 @@ extern "C" int cueva_flowinit(cueva_ctx* c, int l) {
 
 template <int ORD>
 static int launch_faces_ord(cueva_ctx* c) {
-  dim3 grid((unsigned)((c->g.imax + 2 + BLK.x - 1) / BLK.x), (unsigned)((c->g.jmax + 2 + BLK.y - 1) / BLK.y));
+  const dim3 grid = faces_grid(c->g, BLK);
   if (c->prm.flux_solver == CUEVA_HLLC)
     LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, c->dst);
   else if (c->prm.flux_solver == CUEVA_HLLE)
@@ -446,7 +446,7 @@ static int launch_faces_ord(cueva_ctx* c) {
 
 static int launch_faces(cueva_ctx* c) {
   if (c->prm.precision == CUEVA_MIXED) {
-    dim3 grid((unsigned)((c->g.imax + 2 + BLK.x - 1) / BLK.x), (unsigned)((c->g.jmax + 2 + BLK.y - 1) / BLK.y));
+    const dim3 grid = faces_grid(c->g, BLK);
     prof_begin(c, K_FACES);
     hipError_t e = launch_faces_fp32(c->rec_order, grid, BLK, c->stream, c->g, c->P, c->S, c->dst);
     prof_end(c);

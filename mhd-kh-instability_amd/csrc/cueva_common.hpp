@@ -53,6 +53,22 @@ struct DevStatus {
   unsigned long long fallback;
 };
 
+// XCD-aware tile order for a 1D launch of 8 * ceil(n/8) workgroups over n =
+// tiles_x * tiles_y tiles: the hardware hands workgroup w to XCD w % 8, and
+// w takes tile (w % 8) * ceil(n/8) + w / 8, so each XCD sweeps one
+// contiguous band of tile rows and the stencil rows neighbouring tiles share
+// stay in that XCD's L2.  Returns false for the padding workgroups.
+__device__ inline bool xcd_tile(int tiles_x, int tiles_y, int& bx, int& by) {
+  const int n = tiles_x * tiles_y;
+  const int per = (n + 7) / 8;
+  const int w = (int)blockIdx.x;
+  const int t = (w % 8) * per + w / 8;
+  if (t >= n) return false;
+  bx = t % tiles_x;
+  by = t / tiles_x;
+  return true;
+}
+
 __device__ inline void report(DevStatus* st, int code, int i, int j) {
   if (atomicCAS(&st->err, 0, code) == 0) {
     st->err_i = i;

@@ -101,14 +101,17 @@ template <int ORD, typename T> __device__ inline T vor_minus(const T* v, int c) 
                T(1375) * v[c - 1]) - T(305) * v[c - 2]) + T(55) * v[c - 3]) - T(5) * v[c - 4]);
 }
 
-// U^L_{i+1/2} at cell c (mp5.f95:104-136)
+// U^L_{i+1/2} at cell c (mp5.f95:104-136).  The limiter is evaluated only
+// when some lane of the wavefront needs it; lanes that keep the unlimited
+// value select it, so every lane gets the reference's value.
 template <int ORD, typename T> __device__ inline T mp_up(const T* v, int c) {
   const T alpha = T(2), epsm = T(1.0e-10), b2 = T(4.0 / 3.0);
   T u0 = v[c];
   T du0 = v[c] - v[c - 1], du1 = v[c + 1] - v[c];
   T vorp = vor_plus<ORD>(v, c);
   T vmpp = u0 + dmm(du1, alpha * du0);
-  if ((vorp - u0) * (vorp - vmpp) <= epsm) return vorp;
+  const bool keep = (vorp - u0) * (vorp - vmpp) <= epsm;
+  if (__all(keep)) return vorp;  // no lane of the wavefront needs the limiter
   T d2m = (v[c - 2] - T(2) * v[c - 1]) + v[c];
   T d20 = (v[c - 1] - T(2) * v[c]) + v[c + 1];
   T d2p = (v[c] - T(2) * v[c + 1]) + v[c + 2];
@@ -120,7 +123,7 @@ template <int ORD, typename T> __device__ inline T mp_up(const T* v, int c) {
   T vlc = (u0 + T(0.5) * du0) + b2 * dm4jmh;
   T vmin = mx(mn(mn(u0, v[c + 1]), vmd), mn(mn(u0, vul), vlc));
   T vmax = mn(mx(mx(u0, v[c + 1]), vmd), mx(mx(u0, vul), vlc));
-  return vorp + dmm(vmin - vorp, vmax - vorp);
+  return keep ? vorp : vorp + dmm(vmin - vorp, vmax - vorp);
 }
 
 // U^R_{i-1/2} at cell c (mp5.f95:142-167)
@@ -130,7 +133,8 @@ template <int ORD, typename T> __device__ inline T mp_um(const T* v, int c) {
   T du0 = v[c] - v[c - 1], du1 = v[c + 1] - v[c];
   T vorm = vor_minus<ORD>(v, c);
   T vmpm = u0 + dmm(-du0, -(alpha * du1));
-  if ((vorm - u0) * (vorm - vmpm) <= epsm) return vorm;
+  const bool keep = (vorm - u0) * (vorm - vmpm) <= epsm;
+  if (__all(keep)) return vorm;
   T d2m = (v[c - 2] - T(2) * v[c - 1]) + v[c];
   T d20 = (v[c - 1] - T(2) * v[c]) + v[c + 1];
   T d2p = (v[c] - T(2) * v[c + 1]) + v[c + 2];
@@ -142,7 +146,7 @@ template <int ORD, typename T> __device__ inline T mp_um(const T* v, int c) {
   T vlc = (u0 - T(0.5) * du1) + b2 * dm4jph;
   T vmin = mx(mn(mn(u0, v[c - 1]), vmd), mn(mn(u0, vul), vlc));
   T vmax = mn(mx(mx(u0, v[c - 1]), vmd), mx(mx(u0, vul), vlc));
-  return vorm + dmm(vmin - vorm, vmax - vorm);
+  return keep ? vorm : vorm + dmm(vmin - vorm, vmax - vorm);
 }
 
 // Face value pair at the face between cells i and i+1: L = up(i), R = um(i+1).

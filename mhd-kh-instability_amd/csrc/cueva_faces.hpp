@@ -7,7 +7,26 @@
 
 #include "cueva_common.hpp"
 
+#ifndef CUEVA_XCD_FACES
+#define CUEVA_XCD_FACES 1
+#endif
+#ifndef CUEVA_FACES_WAVES_PER_EU
+#define FACES_WAVES
+#else
+#define FACES_WAVES __attribute__((amdgpu_waves_per_eu(CUEVA_FACES_WAVES_PER_EU, CUEVA_FACES_WAVES_PER_EU)))
+#endif
+
 namespace cueva {
+
+// launch grid of k_faces over the face pairs [-1,imax] x [-1,jmax]
+inline dim3 faces_grid(const Geom& g, dim3 blk) {
+  const unsigned tx = (unsigned)((g.imax + 2 + blk.x - 1) / blk.x), ty = (unsigned)((g.jmax + 2 + blk.y - 1) / blk.y);
+#if CUEVA_XCD_FACES
+  return dim3(8 * ((tx * ty + 7) / 8));
+#else
+  return dim3(tx, ty);
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // faces: reconstruction + Riemann solve for the face pair (i,j), i.e. the x
@@ -52,9 +71,17 @@ __device__ inline void face_fields(const Planes& P, int64_t c, int64_t st, Side<
 }
 
 template <int ORD, int SOLVER, typename T = double>
-__global__ void __launch_bounds__(256) k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
+__global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
+#if CUEVA_XCD_FACES
+  int bx, by;
+  if (!xcd_tile((g.imax + 2 + (int)blockDim.x - 1) / (int)blockDim.x,
+                (g.jmax + 2 + (int)blockDim.y - 1) / (int)blockDim.y, bx, by))
+    return;
+#else
+  const int bx = blockIdx.x, by = blockIdx.y;
+#endif
+  const int i = bx * blockDim.x + threadIdx.x - 1;
+  const int j = by * blockDim.y + threadIdx.y - 1;
   if (i > g.imax || j > g.jmax) return;
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;

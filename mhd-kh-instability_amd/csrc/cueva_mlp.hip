@@ -7,7 +7,7 @@
 //    FP32 master weights (the forward uses their BF16 rounding).
 //  - k_mlp_adam: one Adam update of the master weights; clears the gradient.
 //  - k_mlp_forward: normalised outputs for normalised inputs (tests).
-//  - k_c2p_learned: per cell, the network's W start, at most two FP64 Newton
+//  - k_c2p_learned: per cell, the network's start for W, at most two FP64 Newton
 //    steps on the quartic of 12_varprimitive.f95:245-296, acceptance when the
 //    relative quartic residual is <= 1e-10, W >= 1 and the state it implies is
 //    physical and agrees with the network's pressure, otherwise the Cardano
@@ -209,7 +209,14 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
   forward_wave(o, lane, f0, f1, f2, y0, y1);
   if (!valid) return;
   if (qd.v2old > 1.0) { report(st, CUEVA_E_C2P_OLDV, i, j); return; }
-  double W = exp((double)(y0 * wp[OFF_OUT_STD + 0] + wp[OFF_OUT_MEAN + 0]));
+  // Start: W from the network's pressure through |S - E x B| = (C2 + p) v,
+  // W = 1/sqrt(1 - v^2); its error is v^2 W^2 p/(C2+p) times that of p,
+  // an order of magnitude below the error of the network's own W, which is
+  // the start only where that v would not be subluminal.
+  const double lp_net = (double)(y1 * wp[OFF_OUT_STD + 1] + wp[OFF_OUT_MEAN + 1]);
+  const double pn = exp(lp_net) * C2;
+  const double ve = sf / (C2 + pn);
+  double W = ve < 1.0 ? 1.0 / sqrt(1.0 - ve * ve) : exp((double)(y0 * wp[OFF_OUT_STD + 0] + wp[OFF_OUT_MEAN + 0]));
   W = c2p_newton_w(qd, W);
   W = c2p_newton_w(qd, W);
   bool ok = W >= 1.0 && c2p_residual(qd, W) <= 1.0e-10;
@@ -220,7 +227,6 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
     const double rho = in.d / W;
     const double h = (qd.c2 - qd.g1 * rho) / (sq(W) - qd.g1);
     const double pw = qd.g1 * (h - rho);
-    const double lp_net = (double)(y1 * wp[OFF_OUT_STD + 1] + wp[OFF_OUT_MEAN + 1]);
     ok = pw > 0.0 && qd.c1 < sq(qd.c2 + pw) && fabs(log(pw / qd.c2) - lp_net) <= 0.6931471805599453;
   }
   C2POut out;

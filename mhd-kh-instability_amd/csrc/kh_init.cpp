@@ -79,3 +79,35 @@ extern "C" int cueva_kh_init(int imax, int jmax, double lx, double ly, int rotat
   }
   return CUEVA_OK;
 }
+
+// Circularly polarised Alfven wave along x, uniform in y, periodic box
+// (03_init_var_primitive.f95:129-160, DIM = 1, TEST = 2, extended to 2D):
+// B = B0 (1, eta cos kx, eta sin kx), v = -eta VA (0, cos kx, sin kx),
+// E = -v x B = VA eta B0 (0, sin kx, -cos kx), k = 2 pi / Lx, with the
+// reference's exact Alfven speed VA; rho, p uniform.
+extern "C" int cueva_cpaw_init(int imax, int jmax, double lx, double gamma, double rho0, double p0, double b0,
+                               double eta, double* prims14, double* va) {
+  if (imax < 8 || jmax < 8 || !prims14 || !(lx > 0.0) || !(gamma > 1.0)) return CUEVA_E_ARG;
+  const int nxp = imax + 13, nyp = jmax + 13;
+  const size_t plane = (size_t)nxp * (size_t)nyp;
+  const double pi = 3.14159265358979323846264338327950288;
+  const double eps = p0 / ((gamma - 1.0) * rho0);
+  const double h = rho0 * (1.0 + eps) + p0;
+  const double a = h + b0 * b0 * (1.0 + eta * eta);
+  const double r = 2.0 * eta * b0 * b0 / a;
+  const double VA = sqrt(2.0 * b0 * b0 / (a * (1.0 + sqrt(1.0 - r * r))));
+  const double dx = lx / (double)imax, kx = 2.0 * pi / lx;
+  for (int j = -6; j <= jmax + 6; j++)
+    for (int i = -6; i <= imax + 6; i++) {
+      // periodic in x: ghost i takes the value of i mod imax (cell imax == cell 0)
+      int ic = ((i % imax) + imax) % imax;
+      const double x = -0.5 * lx + ic * dx;
+      const double c = cos(kx * x), s = sin(kx * x);
+      const double val[14] = {0.0, 0.0, b0, eta * b0 * c, eta * b0 * s, 0.0, -eta * VA * c, -eta * VA * s,
+                              0.0, VA * eta * b0 * s, -VA * eta * b0 * c, 0.0, rho0, p0};
+      const size_t k = (size_t)(j + 6) * nxp + (size_t)(i + 6);
+      for (int f = 0; f < 14; f++) prims14[f * plane + k] = val[f];
+    }
+  if (va) *va = VA;
+  return CUEVA_OK;
+}

@@ -97,6 +97,7 @@ SIGNATURES = {
     "cueva_kernel_times": (_i, [_c, _P(_d), _P(ctypes.c_int64)]),
     "cueva_time_steps": (_i, [_c, _i, _P(_d)]),
     "cueva_set_c2p_params": (_i, [_c, _P(ctypes.c_float)]),
+    "cueva_cpaw_init": (_i, [_i, _i, _d, _d, _d, _d, _d, _d, _P(_d), _P(_d)]),
     "cueva_mlp_create": (_i, [_P(_c)]),
     "cueva_mlp_destroy": (_i, [_c]),
     "cueva_mlp_set_params": (_i, [_c, _P(ctypes.c_float)]),
@@ -170,6 +171,15 @@ MLP_NPARAM, MLP_NPACK = 1250, 1260
 
 def _fp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def cpaw_init(imax, jmax, lx=1.0, gamma=4.0 / 3.0, rho0=1.0, p0=0.1, b0=1.0, eta=0.1):
+    """Circularly polarised Alfven wave primitives and the Alfven speed (cueva_cpaw_init)."""
+    out = np.zeros((14, jmax + 13, imax + 13))
+    va = ctypes.c_double()
+    check(load().cueva_cpaw_init(imax, jmax, lx, gamma, rho0, p0, b0, eta, _dp(out), ctypes.byref(va)),
+          "cpaw_init")
+    return out, va.value
 
 
 class Mlp:

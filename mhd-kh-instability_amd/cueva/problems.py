@@ -40,3 +40,12 @@ def kh_problem(imax, jmax, sigma=1.0e3, flux_solver=abi.HLL, rec_mp=0, order=2, 
 def config(name, rotated=False, **kw):
     ni, nj, sigma, fs, rec, order, v0, bx, bz = CONFIGS[name]
     return kh_problem(ni, nj, sigma, fs, rec, order, rotated, v0=v0, bx0=bx, bz0=bz, **kw)
+
+
+def cpaw_problem(n, sigma=1.0e6, flux_solver=abi.HLLC, rec_mp=0, order=2, **kw):
+    """Circularly polarised Alfven wave on an n x n periodic unit box (rho = 1,
+    p = 0.1, B0 = 1, amplitude 0.1): (params, primitives, Alfven speed)."""
+    p = abi.default_params(imax=n, jmax=n, longx=1.0, longy=1.0, bc_x=abi.BC_PERIODIC, bc_y=abi.BC_PERIODIC,
+                           sigma=sigma, flux_solver=flux_solver, rec_mp=rec_mp, order=order, cfl=0.4, **kw)
+    prims, va = abi.cpaw_init(n, n, 1.0, p.gamma)
+    return p, prims, va

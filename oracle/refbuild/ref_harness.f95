@@ -9,7 +9,7 @@
 ! by a binary read of the same 14 fields, so that seeded initial data reach
 ! the reference bit for bit.
 !
-! usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug]
+! usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug] [-globals]
 !   INPUT      stream file: 14 arrays (-6:imax+6,-6:jmax+6) of real(8), x fastest,
 !              in the restart order psi phi Bx By Bz Vx Vy Vz Ex Ey Ez q rho p
 !   OUTPREFIX  each dump step h writes OUTPREFIX_hNNNNN.bin:
@@ -18,6 +18,10 @@
 !   -debug     also write OUTPREFIX_stage.bin after step 1 with, for every stage l:
 !              W, Exast..Ezast, the 14 stage fields, Jx Jy Jz, the 28 face fluxes,
 !              and the 17 stage right-hand sides (see write_stage below)
+!   -globals   after the last step, run the reference's own lector with h = 1,
+!              which writes its global diagnostics (22_writer.f95:818-991) to
+!              the units 80..217 (unconnected: files fort.80 ... in the
+!              working directory)
 
 program cueva_ref_harness
 
@@ -31,7 +35,7 @@ program cueva_ref_harness
   character(len=1024) :: arg, infile, prefix, fname
   integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out
   integer, allocatable :: dumps(:)
-  logical             :: debug
+  logical             :: debug, globals
 
   nargs = command_argument_count()
   if (nargs < 3) then
@@ -45,10 +49,13 @@ program cueva_ref_harness
   allocate(dumps(max(1,nargs)))
   ndump = 0
   debug = .false.
+  globals = .false.
   do iarg = 4, nargs
      call get_command_argument(iarg, arg)
      if (trim(arg) == "-debug") then
         debug = .true.
+     else if (trim(arg) == "-globals") then
+        globals = .true.
      else
         ndump = ndump + 1
         read(arg,*) dumps(ndump)
@@ -128,6 +135,11 @@ program cueva_ref_harness
         end if
      end do
   end do
+
+  if (globals) then
+     h = 1
+     call lector
+  end if
 
   close(301)
   close(302)

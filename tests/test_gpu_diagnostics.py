@@ -88,3 +88,30 @@ def test_max_speed(v0):
     want = max(acoustic(p.gamma, rho, pp, v2, vx).max(), acoustic(p.gamma, rho, pp, v2, vy).max())
     assert got == pytest.approx(want, rel=1e-15, abs=0)
     assert 0.0 < got < 1.0  # bounded by the light speed that sets the reference's dt
+
+
+@pytest.mark.parametrize("name", ["kh_hllc_mp5_32x64", "kh_hlle_mp7_bz_32x64", "kh_cfg1_hll_mp5_128x256"])
+def test_globals_match_reference_lector(name):
+    """Against the reference's own lector output after the fixture's last step."""
+    import json
+    import os
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), "golden", name + ".json")))
+    if "globals_final" not in meta:
+        pytest.skip("fixture without the reference's global diagnostics")
+    ni, nj = meta["native_grid"]
+    say("globals vs reference", name)
+    p, prims = cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"], rec_mp=meta["rec_mp"],
+                                order=meta["order"], rotated=True, seed=meta["seed"], **meta.get("init", {}))
+    with cueva.Solver(p) as s:
+        s.upload(prims)
+        s.step(meta["nsteps"])
+        s.step_counters()
+        got = s.globals()
+    spread = max(meta["dumps"][str(meta["nsteps"])]["libm_spread"].values())
+    for n, want in meta["globals_final"].items():
+        if n.startswith("max_") and spread == 0.0:
+            assert got[n] == want, (n, got[n], want)
+        else:
+            # summation order, and beyond the bitwise fixtures the reference's own libm spread
+            tol = max(1e-12, 10 * spread)
+            assert abs(got[n] - want) <= tol * max(1.0, abs(want)), (n, got[n], want)

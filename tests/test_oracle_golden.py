@@ -4,9 +4,11 @@ The fixtures in tests/golden/ come from the reference compiled from its own
 sources (oracle/refbuild/build_ref.sh) and run single-threaded on seeded
 Kelvin-Helmholtz data (tests/golden/make_golden.py).  Every field of every
 dump must have the same SHA-256 as the reference's, and the fix-up counters
-must match.  This pins every oracle routine on the path: MP5/MP7/MP9,
-HLL/HLLC, the IMEX SSP2(2,2,2)/SSP3(4,3,3) stage sums, the E solve, the
-boundary maps and the Cardano recovery.
+must match, and the 37 global diagnostics must equal what the reference's
+own lector wrote after the last step.  This pins every oracle routine on
+the path: MP5/MP7/MP9, HLL/HLLE/HLLC, the IMEX SSP2(2,2,2)/SSP3(4,3,3) stage
+sums, the E solve, the boundary maps, the Cardano recovery and the global
+reductions.
 """
 import glob
 import hashlib
@@ -72,6 +74,12 @@ def run_case(name):
         else:
             diff = {}
         assert not bad, (name, d, bad, diff)
+    if "globals_final" in meta:
+        # the reference's own lector output after the last step (22_writer.f95:818-991)
+        assert done == meta["nsteps"]
+        got = dict(zip(cueva.GLOBAL_NAMES, o.globals().tolist()))
+        bad = {n: (got[n], v) for n, v in meta["globals_final"].items() if got[n] != v}
+        assert not bad, (name, bad)
     if meta["expected_stop_step"]:
         with pytest.raises(RuntimeError, match="status 22"):
             o.step(meta["expected_stop_step"] - done)
