@@ -558,12 +558,11 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
       } else if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
         if (!c->has_mlp) return CUEVA_E_ARG;
         if (mixed)
-          LAUNCH(K_FINAL, (k_final_fused<float, false>), interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
+          LAUNCH(K_FINAL, k_final_learned<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
+                 (const float*)c->mlpw, c->dst);
         else
-          LAUNCH(K_FINAL, (k_final_fused<double, false>), interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
-        if (launch_ok()) return CUEVA_E_HIP;
-        int crc = launch_c2p(c);
-        if (crc) return crc;
+          LAUNCH(K_FINAL, k_final_learned<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
+                 (const float*)c->mlpw, c->dst);
       } else {
         if (mixed)
           LAUNCH(K_FINAL, k_final_fused<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);

@@ -11,6 +11,7 @@
 
 #include "cueva_common.hpp"
 #include "cueva_faces.hpp"
+#include "cueva_mlp.hpp"
 
 namespace cueva {
 // ---------------------------------------------------------------------------
@@ -287,12 +288,11 @@ __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
   P.s[S_JX][k] = jx; P.s[S_JY][k] = jy; P.s[S_JZ][k] = jz;
 }
 
-// Last stage + sumfin + varfin + electricfield + varprimitivecardano
-// (C2P false: without the primitive recovery, which then runs as its own
-// kernel, the learned one).
-template <typename FT, bool C2P = true>
-__global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
-  INTERIOR_IJ
+// Last stage right-hand side, sumfin, varfin and electricfield at one
+// interior point: U^{n+1} in A_* order, from registers.
+template <typename FT>
+__device__ inline void final_update(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k,
+                                    double* nu) {
   const int L = S.lmax - 1;
   Rhs last = flux_source<FT>(g, P, S, i, j, k);
   double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
@@ -301,7 +301,6 @@ __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
   Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, P.p[P_RHO][k], P.p[P_P][k]);
   double un[A_N] = {P.p[P_PSI][k], P.p[P_PHI][k], bx0, by0, bz0, ex0, ey0, ez0, P.p[P_Q][k],
                     cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
-  double nu[A_N];
 #pragma unroll
   for (int a = 0; a < A_N; a++) {
     double acc = 0.0;
@@ -316,13 +315,79 @@ __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
     }
     nu[a] = un[a] + S.dt * acc;
   }
+}
+
+__device__ inline void store_update(Planes& P, int64_t k, const double* nu) {
   P.p[P_PSI][k] = nu[A_PSI]; P.p[P_PHI][k] = nu[A_PHI];
   P.p[P_BX][k] = nu[A_BX]; P.p[P_BY][k] = nu[A_BY]; P.p[P_BZ][k] = nu[A_BZ];
   P.p[P_EX][k] = nu[A_EX]; P.p[P_EY][k] = nu[A_EY]; P.p[P_EZ][k] = nu[A_EZ];
   P.p[P_Q][k] = nu[A_Q];
   P.c[C_D][k] = nu[A_D]; P.c[C_TAU][k] = nu[A_TAU];
   P.c[C_SX][k] = nu[A_SX]; P.c[C_SY][k] = nu[A_SY]; P.c[C_SZ][k] = nu[A_SZ];
+}
+
+// Last stage + sumfin + varfin + electricfield + varprimitivecardano
+// (C2P false: without the primitive recovery, which then runs on its own).
+template <typename FT, bool C2P = true>
+__global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
+  INTERIOR_IJ
+  double nu[A_N];
+  final_update<FT>(g, P, S, i, j, k, nu);
+  store_update(P, k, nu);
   if (C2P) c2p_store(g, P, S, st, i, j, k);
+}
+
+// The same with the learned primitive recovery.  Every lane of a wavefront
+// (one row of 64 points, block 64 x 4) stays active for the network: points
+// past the grid edge compute on the edge point and store nothing.  The new
+// state is stored before the network runs and read back after it, so that
+// it does not occupy registers across the network.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_final_learned(Geom g, Planes P, Scheme S, const float* __restrict__ wp,
+                                                       DevStatus* st) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j0 = blockIdx.y * blockDim.y + threadIdx.y;
+  const bool valid = i0 <= g.imax && j0 <= g.jmax;
+  const int i = i0 <= g.imax ? i0 : g.imax, j = j0 <= g.jmax ? j0 : g.jmax;
+  const int64_t k = g.at(i, j);
+  float f0, f1, f2;
+  {
+    double nu[A_N];
+    final_update<FT>(g, P, S, i, j, k, nu);
+    C2PIn in;
+    in.d = nu[A_D]; in.tau = nu[A_TAU]; in.sx = nu[A_SX]; in.sy = nu[A_SY]; in.sz = nu[A_SZ];
+    in.ex = nu[A_EX]; in.ey = nu[A_EY]; in.ez = nu[A_EZ];
+    in.bx = nu[A_BX]; in.by = nu[A_BY]; in.bz = nu[A_BZ];
+    in.vx0 = in.vy0 = in.vz0 = 0.0;  // not used by the features
+    in.rho_old = in.p_old = 0.0;
+    c2p_features(S.gamma, in, wp, f0, f1, f2);
+    if (valid) store_update(P, k, nu);
+  }
+  const int lane = threadIdx.x & 63;
+  float y0 = 0.0f, y1 = 0.0f;
+  {
+    const mlp::Operands o = mlp::load_operands(wp, lane);
+    mlp::forward_wave(o, lane, f0, f1, f2, y0, y1);
+  }
+  if (!valid) return;
+  C2PIn in;
+  in.d = P.c[C_D][k]; in.tau = P.c[C_TAU][k];
+  in.sx = P.c[C_SX][k]; in.sy = P.c[C_SY][k]; in.sz = P.c[C_SZ][k];
+  in.ex = P.p[P_EX][k]; in.ey = P.p[P_EY][k]; in.ez = P.p[P_EZ][k];
+  in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
+  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
+  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
+  C2POut out;
+  bool fallback = false;
+  const int code = c2p_learned_finish(S.gamma, in, wp, y0, y1, S.crmath != 0, out, fallback);
+  if (fallback) atomicAdd(&st->fallback, 1ULL);
+  if (code) { report(st, code, i, j); return; }
+  if (out.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
+  if (out.fix_p) atomicAdd(&st->cnt[1], 1ULL);
+  if (out.fix_v) atomicAdd(&st->cnt[2], 1ULL);
+  P.p[P_VX][k] = out.vx; P.p[P_VY][k] = out.vy; P.p[P_VZ][k] = out.vz;
+  P.p[P_RHO][k] = out.rho; P.p[P_P][k] = out.p;
+  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(out.vx) + sq(out.vy)) + sq(out.vz)));
 }
 
 }  // namespace cueva

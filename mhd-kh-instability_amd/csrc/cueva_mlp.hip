@@ -196,49 +196,12 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
   in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
   in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
   in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
-  Quartic qd = c2p_quartic(S.gamma, in);
-  // features (FP64 -> FP32), normalised
-  const double C2 = qd.c2, sf = sqrt(qd.c1);
-  float f0 = (float)log(in.d / C2), f1 = (float)(sf / C2), f2 = (float)log(C2);
-  f0 = (f0 - wp[OFF_IN_MEAN + 0]) / wp[OFF_IN_STD + 0];
-  f1 = (f1 - wp[OFF_IN_MEAN + 1]) / wp[OFF_IN_STD + 1];
-  f2 = (f2 - wp[OFF_IN_MEAN + 2]) / wp[OFF_IN_STD + 2];
-  if (!(f0 == f0) || !(f2 == f2)) f0 = f1 = f2 = 0.0f;  // unphysical input: the guess is rejected below
   Operands o = load_operands(wp, lane);
-  float y0 = 0.0f, y1 = 0.0f;
-  forward_wave(o, lane, f0, f1, f2, y0, y1);
-  if (!valid) return;
-  if (qd.v2old > 1.0) { report(st, CUEVA_E_C2P_OLDV, i, j); return; }
-  // Start: W from the network's pressure through |S - E x B| = (C2 + p) v,
-  // W = 1/sqrt(1 - v^2); its error is v^2 W^2 p/(C2+p) times that of p,
-  // an order of magnitude below the error of the network's own W, which is
-  // the start only where that v would not be subluminal.
-  const double lp_net = (double)(y1 * wp[OFF_OUT_STD + 1] + wp[OFF_OUT_MEAN + 1]);
-  const double pn = exp(lp_net) * C2;
-  const double ve = sf / (C2 + pn);
-  double W = ve < 1.0 ? 1.0 / sqrt(1.0 - ve * ve) : exp((double)(y0 * wp[OFF_OUT_STD + 0] + wp[OFF_OUT_MEAN + 0]));
-  W = c2p_newton_w(qd, W);
-  W = c2p_newton_w(qd, W);
-  bool ok = W >= 1.0 && c2p_residual(qd, W) <= 1.0e-10;
-  if (ok) {
-    // physical and the root the network meant: p > 0, |v| < 1, and p within a
-    // factor 2 of the network's own pressure estimate (h and p as
-    // 12_varprimitive.f95:550-553 form them)
-    const double rho = in.d / W;
-    const double h = (qd.c2 - qd.g1 * rho) / (sq(W) - qd.g1);
-    const double pw = qd.g1 * (h - rho);
-    ok = pw > 0.0 && qd.c1 < sq(qd.c2 + pw) && fabs(log(pw / qd.c2) - lp_net) <= 0.6931471805599453;
-  }
   C2POut out;
-  int code;
-  if (ok) {
-    code = c2p_finish(S.gamma, in, qd, W, false, out);
-  } else {
-    atomicAdd(&st->fallback, 1ULL);
-    double Wc;
-    code = c2p_cardano(qd, Wc, S.crmath != 0);
-    if (!code) code = c2p_finish(S.gamma, in, qd, Wc, true, out);
-  }
+  bool fallback = false;
+  const int code = c2p_learned_cell(S.gamma, in, wp, o, lane, S.crmath != 0, out, fallback);
+  if (!valid) return;
+  if (fallback) atomicAdd(&st->fallback, 1ULL);
   if (code) { report(st, code, i, j); return; }
   if (out.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
   if (out.fix_p) atomicAdd(&st->cnt[1], 1ULL);

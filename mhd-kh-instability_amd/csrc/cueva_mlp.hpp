@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cueva_device.hpp"
+
 namespace cueva {
 namespace mlp {
 
@@ -164,4 +166,69 @@ __device__ inline void forward_wave(const Operands& o, int lane, float x0, float
 }
 
 }  // namespace mlp
+
+// Normalised network features of a conserved state: (log(D/C2),
+// |S - E x B|/C2, log C2), C2 = tau - (E^2+B^2)/2, formed as the quartic of
+// 12_varprimitive.f95:245-296 forms C1 and C2.
+__device__ inline void c2p_features(double gamma, const C2PIn& in, const float* __restrict__ wp, float& f0,
+                                    float& f1, float& f2) {
+  using namespace mlp;
+  const Quartic qd = c2p_quartic(gamma, in);
+  const double C2 = qd.c2, sf = sqrt(qd.c1);
+  f0 = ((float)log(in.d / C2) - wp[OFF_IN_MEAN + 0]) / wp[OFF_IN_STD + 0];
+  f1 = ((float)(sf / C2) - wp[OFF_IN_MEAN + 1]) / wp[OFF_IN_STD + 1];
+  f2 = ((float)log(C2) - wp[OFF_IN_MEAN + 2]) / wp[OFF_IN_STD + 2];
+  if (!(f0 == f0) || !(f1 == f1) || !(f2 == f2)) f0 = f1 = f2 = 0.0f;  // unphysical: the start is rejected later
+}
+
+// Primitive recovery of one cell from the network outputs (y0, y1): start W
+// -> two FP64 Newton steps on the quartic of 12_varprimitive.f95:245-296 ->
+// accepted when the relative quartic residual is <= 1e-10, W >= 1, p > 0,
+// |v| < 1 and p is within a factor 2 of the network's own p; otherwise the
+// reference's Cardano root (fallback = true).  Then the reference's pressure
+// loop, fix-ups and velocities (12_varprimitive.f95:510-696).  Returns 0 or a
+// status.
+__device__ inline int c2p_learned_finish(double gamma, const C2PIn& in, const float* __restrict__ wp, float y0,
+                                         float y1, bool crm, C2POut& out, bool& fallback) {
+  using namespace mlp;
+  const Quartic qd = c2p_quartic(gamma, in);
+  fallback = false;
+  if (qd.v2old > 1.0) return 20;  // as the Cardano path (12_varprimitive.f95:229-238)
+  const double C2 = qd.c2, sf = sqrt(qd.c1);
+  // Start: W from the network's pressure through |S - E x B| = (C2 + p) v,
+  // W = 1/sqrt(1 - v^2); its error is v^2 W^2 p/(C2+p) times that of p,
+  // an order of magnitude below the error of the network's own W, which is
+  // the start only where that v would not be subluminal.
+  const double lp_net = (double)(y1 * wp[OFF_OUT_STD + 1] + wp[OFF_OUT_MEAN + 1]);
+  const double pn = exp(lp_net) * C2;
+  const double ve = sf / (C2 + pn);
+  double W = ve < 1.0 ? 1.0 / sqrt(1.0 - ve * ve) : exp((double)(y0 * wp[OFF_OUT_STD + 0] + wp[OFF_OUT_MEAN + 0]));
+  W = c2p_newton_w(qd, W);
+  W = c2p_newton_w(qd, W);
+  bool ok = W >= 1.0 && c2p_residual(qd, W) <= 1.0e-10;
+  if (ok) {
+    // physical and the root the network meant (h and p as 12_varprimitive.f95:550-553 form them)
+    const double rho = in.d / W;
+    const double h = (qd.c2 - qd.g1 * rho) / (sq(W) - qd.g1);
+    const double pw = qd.g1 * (h - rho);
+    ok = pw > 0.0 && qd.c1 < sq(qd.c2 + pw) && fabs(log(pw / qd.c2) - lp_net) <= 0.6931471805599453;
+  }
+  if (ok) return c2p_finish(gamma, in, qd, W, false, out);
+  fallback = true;
+  double Wc;
+  int code = c2p_cardano(qd, Wc, crm);
+  if (!code) code = c2p_finish(gamma, in, qd, Wc, true, out);
+  return code;
+}
+
+// Both together (every lane of the wavefront must call: the network runs
+// wavefront-wide).
+__device__ inline int c2p_learned_cell(double gamma, const C2PIn& in, const float* __restrict__ wp,
+                                       const mlp::Operands& o, int lane, bool crm, C2POut& out, bool& fallback) {
+  float f0, f1, f2, y0 = 0.0f, y1 = 0.0f;
+  c2p_features(gamma, in, wp, f0, f1, f2);
+  mlp::forward_wave(o, lane, f0, f1, f2, y0, y1);
+  return c2p_learned_finish(gamma, in, wp, y0, y1, crm, out, fallback);
+}
+
 }  // namespace cueva
