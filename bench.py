@@ -107,10 +107,17 @@ FORMS = {
     "mixed": dict(precision=1, crmath=0, c2p_mode=0),
     "mixed_learned": dict(precision=1, crmath=0, c2p_mode=1),
 }
-# compulsory bytes per face pair of k_faces: 16 stage + 5 primitive FP64
-# planes read; 28 face-flux planes written (FP64, or FP32 in the mixed forms)
-FACES_BYTES = {"plain_fp64": (16 + 5) * 8 + 28 * 8, "mixed": (16 + 5) * 8 + 28 * 4,
-               "mixed_learned": (16 + 5) * 8 + 28 * 4}
+# compulsory FP64 bytes per cell of the marching face kernel (every form: the
+# stage planes are FP64, the FP32 work stays on chip).  Both launches read
+# the 16 stage + 5 primitive planes the stencils touch; the stage launch
+# writes its 17-plane right-hand side; the last launch also reads the earlier
+# stages' right-hand sides (17 each) and the 11 other U^n planes of the
+# update, and writes U^{n+1} (14 planes).
+MARCH_STAGE_BYTES = (16 + 5 + 17) * 8
+
+
+def march_last_bytes(lmax):
+    return (16 + 5 + 17 * (lmax - 1) + 11 + 14) * 8
 ACCURACY_STEPS = 100
 
 
@@ -216,8 +223,10 @@ def main():
     kt = r["kernel_ms_per_step"]
     dom = max(kt, key=lambda k: kt[k])
     dom_avg_ms = kt[dom] / (r["kernel_launches"][dom] / args.steps)
-    pairs = (p.imax + 2) * (p.jmax + 2)
-    alg_bytes = pairs * FACES_BYTES[head] if dom == "faces" else None
+    cells = (p.imax + 1) * (p.jmax + 1)
+    lmax = 2 if order == 2 else 4
+    # the face launches of one step: lmax - 1 stage launches and the last one
+    alg_bytes = cells * (MARCH_STAGE_BYTES * (lmax - 1) + march_last_bytes(lmax)) / lmax if dom == "faces" else None
     achieved = alg_bytes / (dom_avg_ms / 1e3) / 1e9 if alg_bytes else None
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (dom, cfg, head))

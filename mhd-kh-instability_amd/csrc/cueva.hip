@@ -16,6 +16,8 @@ namespace cueva {
 // FP32 face kernels (cueva_mixed.hip)
 hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
                              const Scheme& S, DevStatus* st);
+// FP32 marching face kernels (cueva_mixed.hip)
+const void* march_kernel_fp32(int rec_order, int solver, bool final_stage);
 // learned primitive recovery (cueva_mlp.hip)
 hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
                               DevStatus* st);
@@ -36,6 +38,10 @@ struct cueva_ctx {
   float* mlpw;  // packed learned-recovery parameters (CUEVA_MLP_NPACK), after red
   int has_mlp;
   int rec_order;
+  int march;         // fused step uses the marching face kernel (default; CUEVA_MARCH=0: face planes)
+  int march_rounds;  // resident-block rounds a marching launch is cut into (CUEVA_MARCH_ROUNDS)
+  int march_rows;    // rows per segment when set (CUEVA_MARCH_SEG; rounded up to 4m - 1)
+  int march_seg[2];  // rows per segment of the stage / last-stage kernel (0: not sized yet)
   int err;  // sticky host-side error
   // kernel timing (cueva_set_profiling)
   int prof;
@@ -205,6 +211,14 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   S.crmath = p->crmath;
   imex_tables(p->order, S);
   c->rec_order = 5 + 2 * p->rec_mp;
+  {
+    const char* e = getenv("CUEVA_MARCH");
+    c->march = e ? atoi(e) != 0 : 1;
+    e = getenv("CUEVA_MARCH_ROUNDS");
+    c->march_rounds = e && atoi(e) > 0 ? atoi(e) : 2;
+    e = getenv("CUEVA_MARCH_SEG");
+    c->march_rows = e && atoi(e) > 0 ? atoi(e) : 0;
+  }
 
   const int nplanes = P_N + C_N + S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N;
   c->bytes = (int64_t)nplanes * g.plane * (int64_t)sizeof(double);
@@ -502,15 +516,15 @@ static int read_status(cueva_ctx* c, DevStatus* h) {
   return CUEVA_OK;
 }
 
-static int launch_c2p(cueva_ctx* c) {
+static int launch_c2p(cueva_ctx* c, int cls = K_OTHER) {
   if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
     if (!c->has_mlp) return CUEVA_E_ARG;
-    prof_begin(c, K_OTHER);
+    prof_begin(c, cls);
     hipError_t e = launch_c2p_learned(c->stream, c->g, c->P, c->S, c->mlpw, c->dst);
     prof_end(c);
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
-  LAUNCH(K_OTHER, k_c2p, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
+  LAUNCH(cls, k_c2p, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
   return launch_ok() ? CUEVA_E_HIP : CUEVA_OK;
 }
 
@@ -537,26 +551,101 @@ extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t*
 // ---------------------------------------------------------------------------
 // fused step
 // ---------------------------------------------------------------------------
+static const dim3 MARCH_BLK(64, 4);
+
+// U^{n+1}, left by the last marching launch in rhs[L][0..13], becomes the
+// state by exchanging plane pointers: psi phi B E q into the primitive
+// planes, D tau S into the conserved ones; the planes they held become the
+// scratch of rhs[L].  The primitive recovery then runs as after varfin.
+static void adopt_update(cueva_ctx* c, int L) {
+  static const int to_p[A_Q + 1] = {P_PSI, P_PHI, P_BX, P_BY, P_BZ, P_EX, P_EY, P_EZ, P_Q};
+  static const int to_c[A_SZ - A_D + 1] = {C_D, C_TAU, C_SX, C_SY, C_SZ};
+  for (int a = A_PSI; a <= A_Q; a++) {
+    double* t = c->P.p[to_p[a]];
+    c->P.p[to_p[a]] = c->P.rhs[L][a];
+    c->P.rhs[L][a] = t;
+  }
+  for (int a = A_D; a <= A_SZ; a++) {
+    double* t = c->P.c[to_c[a - A_D]];
+    c->P.c[to_c[a - A_D]] = c->P.rhs[L][a];
+    c->P.rhs[L][a] = t;
+  }
+}
+
+static const void* march_kernel_of(cueva_ctx* c, bool fin) {
+  if (c->prm.precision == CUEVA_MIXED) return march_kernel_fp32(c->rec_order, c->prm.flux_solver, fin);
+  return march_kernel<double>(c->rec_order, c->prm.flux_solver, fin);
+}
+
+// Rows per segment of a marching launch: enough segments that the blocks
+// fill the resident slots of every CU march_rounds times, seg = 4m - 1 (the
+// segment's face rows, seg + 1, fill whole tile steps).
+static int march_seg(cueva_ctx* c, bool fin) {
+  int& seg = c->march_seg[fin ? 1 : 0];
+  if (seg) return seg;
+  if (c->march_rows > 0) {
+    seg = (int)(4 * ((c->march_rows + 1 + 3) / 4) - 1);
+    return seg;
+  }
+  int per_cu = 0, cus = 0, dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel_of(c, fin), 256, 0) != hipSuccess)
+    return 0;
+  const int64_t slots = (int64_t)(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+  const int strips = march_strips(c->g), rows = c->g.jmax + 1;
+  int64_t want = slots * c->march_rounds / strips;
+  if (want < 1) want = 1;
+  const int64_t per = (rows + want - 1) / want;
+  const int64_t m = (per + 1 + 3) / 4;
+  seg = (int)(4 * m - 1);
+  return seg;
+}
+
+static int launch_march(cueva_ctx* c, int l, bool fin) {
+  const void* fn = march_kernel_of(c, fin);
+  int seg = march_seg(c, fin);
+  if (!fn || seg <= 0) return CUEVA_E_HIP;
+  const dim3 grid((unsigned)march_strips(c->g), (unsigned)((c->g.jmax + 1 + seg - 1) / seg));
+  void* args[] = {(void*)&c->g, (void*)&c->P, (void*)&c->S, (void*)&l, (void*)&seg, (void*)&c->dst};
+  prof_begin(c, K_FACES);
+  hipError_t e = hipLaunchKernel(fn, grid, MARCH_BLK, args, 0, c->stream);
+  prof_end(c);
+  return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
+}
+
 extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return CUEVA_E_ARG;
   const Geom& g = c->g;
   const int halo = (c->rec_order - 1) / 2 + 1;  // stencil radius K plus the face offset
+  const bool learned = c->prm.c2p_mode == CUEVA_C2P_LEARNED;
+  if (learned && !c->has_mlp) return CUEVA_E_ARG;
   for (int h = 0; h < nsteps; h++) {
     for (int l = 0; l < c->S.lmax; l++) {
+      const bool last = l == c->S.lmax - 1;
       dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
               (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
       LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo);
       if (launch_ok()) return CUEVA_E_HIP;
+      if (c->march) {
+        // reconstruction + Riemann + flux difference (+ final update) in one pass
+        int rc = launch_march(c, l, last);
+        if (rc) return rc;
+        if (last) {
+          adopt_update(c, l);
+          if ((rc = launch_c2p(c, K_FINAL))) return rc;
+        }
+        continue;
+      }
       int rc = launch_faces(c);
       if (rc) return rc;
       const bool mixed = c->prm.precision == CUEVA_MIXED;
-      if (l < c->S.lmax - 1) {
+      if (!last) {
         if (mixed)
           LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
         else
           LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
-      } else if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
-        if (!c->has_mlp) return CUEVA_E_ARG;
+      } else if (learned) {
         if (mixed)
           LAUNCH(K_FINAL, k_final_learned<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
                  (const float*)c->mlpw, c->dst);

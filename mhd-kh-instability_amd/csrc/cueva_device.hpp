@@ -693,6 +693,7 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
 struct Cons { double w, d, tau, sx, sy, sz; };
 __device__ inline Cons prim_to_cons(double gam, double vx, double vy, double vz, double ex, double ey,
                                     double ez, double bx, double by, double bz, double rho, double p) {
+#pragma clang fp contract(off)  // FP64: products and sums rounded separately in every unit
   Cons c;
   double v2 = (sq(vx) + sq(vy)) + sq(vz);
   c.w = 1.0 / sqrt(1.0 - v2);
@@ -734,6 +735,7 @@ __device__ inline void e_solve(double abt, double sig, double dt, double w, doub
 // sigma W (E + v x B - (E.v) v) (14_flowinit.f95:281-301, 17_fluxandsource.f95:527-537)
 __device__ inline void ohm_source(double sig, double w, double vx, double vy, double vz, double ex, double ey,
                                   double ez, double bx, double by, double bz, double& sx, double& sy, double& sz) {
+#pragma clang fp contract(off)  // FP64: products and sums rounded separately in every unit
   double edv = (ex * vx + ey * vy) + ez * vz;
   double rx = bz * vy - by * vz, ry = bx * vz - bz * vx, rz = by * vx - bx * vy;
   sx = (sig * w) * ((ex + rx) - edv * vx);

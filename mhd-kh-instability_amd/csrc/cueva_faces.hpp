@@ -6,9 +6,17 @@
 #pragma once
 
 #include "cueva_common.hpp"
+#include "cueva_update.hpp"
 
 #ifndef CUEVA_XCD_FACES
 #define CUEVA_XCD_FACES 1
+#endif
+// minimum waves per SIMD the marching kernel is compiled for (0: no bound)
+#ifndef CUEVA_MARCH_F32_WAVES
+#define CUEVA_MARCH_F32_WAVES 0
+#endif
+#ifndef CUEVA_MARCH_F64_WAVES
+#define CUEVA_MARCH_F64_WAVES 0
 #endif
 #ifndef CUEVA_FACES_WAVES_PER_EU
 #define FACES_WAVES
@@ -70,19 +78,15 @@ __device__ inline void face_fields(const Planes& P, int64_t c, int64_t st, Side<
   face_rec<ORD>(P.s[DIR == 0 ? S_JX : S_JY], c, st, L.jn, R.jn);
 }
 
-template <int ORD, int SOLVER, typename T = double>
-__global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
-#if CUEVA_XCD_FACES
-  int bx, by;
-  if (!xcd_tile((g.imax + 2 + (int)blockDim.x - 1) / (int)blockDim.x,
-                (g.jmax + 2 + (int)blockDim.y - 1) / (int)blockDim.y, bx, by))
-    return;
-#else
-  const int bx = blockIdx.x, by = blockIdx.y;
-#endif
-  const int i = bx * blockDim.x + threadIdx.x - 1;
-  const int j = by * blockDim.y + threadIdx.y - 1;
-  if (i > g.imax || j > g.jmax) return;
+// both faces of the face pair (i,j): put_x(Fx(i+1/2,j)) as soon as the x
+// face is solved, put_y(Fy(i,j+1/2)) as soon as the y face is; each flux
+// array is dead after its put, which keeps the other solve's registers free
+// of it.  YFIRST solves the y face first (the two solves are independent
+// except under HLLE, whose y solve starts from the x solve's last iterate).
+template <int ORD, int SOLVER, typename T, bool YFIRST = false, typename PutX, typename PutY>
+__device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const Scheme& S, DevStatus* st, int i,
+                                          int j, PutX put_x, PutY put_y) {
+  static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;
   Side<T> xl, xr, yl, yr;
@@ -130,8 +134,7 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
     face_rec<ORD>(P.s[S_BZ], c, sy, yl.bz, yr.bz);
     seed_x = hlle_y_ery(yl, yr);
   }
-  // x face
-  {
+  auto xface = [&]() {
     face_fields<ORD, 0>(P, c, sx, xl, xr);
     side_physics<0, HW>((T)S.gamma, xl);
     side_physics<0, HW>((T)S.gamma, xr);
@@ -145,16 +148,9 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
     } else {
       hll_x(xl, xr, f);
     }
-    if constexpr (sizeof(T) == 8) {
-#pragma unroll
-      for (int t = 0; t < FX_N; t++) P.fx[t][c] = f[t];
-    } else {
-#pragma unroll
-      for (int t = 0; t < FX_N; t++) P.fxf[t][c] = f[t];
-    }
-  }
-  // y face
-  {
+    put_x(f);
+  };
+  auto yface = [&]() {
     face_fields<ORD, 1>(P, c, sy, yl, yr);
     side_physics<1, HW>((T)S.gamma, yl);
     side_physics<1, HW>((T)S.gamma, yr);
@@ -169,14 +165,171 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
     } else {
       hll_y(yl, yr, f);
     }
-    if constexpr (sizeof(T) == 8) {
-#pragma unroll
-      for (int t = 0; t < FY_N; t++) P.fy[t][c] = f[t];
-    } else {
-#pragma unroll
-      for (int t = 0; t < FY_N; t++) P.fyf[t][c] = f[t];
-    }
+    put_y(f);
+  };
+  if constexpr (YFIRST) {
+    yface();
+    xface();
+  } else {
+    xface();
+    yface();
   }
+}
+
+template <int ORD, int SOLVER, typename T = double>
+__global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
+#if CUEVA_XCD_FACES
+  int bx, by;
+  if (!xcd_tile((g.imax + 2 + (int)blockDim.x - 1) / (int)blockDim.x,
+                (g.jmax + 2 + (int)blockDim.y - 1) / (int)blockDim.y, bx, by))
+    return;
+#else
+  const int bx = blockIdx.x, by = blockIdx.y;
+#endif
+  const int i = bx * blockDim.x + threadIdx.x - 1;
+  const int j = by * blockDim.y + threadIdx.y - 1;
+  if (i > g.imax || j > g.jmax) return;
+  const int64_t c = g.at(i, j);
+  face_pair<ORD, SOLVER, T>(
+      g, P, S, st, i, j,
+      [&](const T* f) {
+#pragma unroll
+        for (int t = 0; t < FX_N; t++) {
+          if constexpr (sizeof(T) == 8) P.fx[t][c] = f[t];
+          else P.fxf[t][c] = f[t];
+        }
+      },
+      [&](const T* f) {
+#pragma unroll
+        for (int t = 0; t < FY_N; t++) {
+          if constexpr (sizeof(T) == 8) P.fy[t][c] = f[t];
+          else P.fyf[t][c] = f[t];
+        }
+      });
+}
+
+// ---------------------------------------------------------------------------
+// Marching face kernel: reconstruction + Riemann solve + flux difference in
+// one pass.  A block of 4 wavefronts owns a strip of MARCH_COLS cell columns
+// and a segment of `seg` cell rows (seg = 4m - 1).  Lane L of every wavefront
+// computes the face pair of column i0 - 1 + L; per tile step the 4 wavefronts
+// take 4 consecutive face rows.  Fx(i-1/2) is the left lane's x flux (a
+// cross-lane shuffle of the registers the x solve leaves; HLLE, which must
+// solve x first, goes through LDS); Fy(j-1/2) is the LDS row of the
+// wavefront below, or for the lowest wavefront the previous tile step's top
+// row.  Lane 0 only supplies Fx(i0-1/2), the segment's first face row only
+// Fy(j0-1/2): each face is solved once per strip and segment, plus one
+// column and one row per block.
+//
+// One barrier per tile step: the y rows alternate between two LDS buffers,
+// and wavefront 0 copies the top row it needs next into its own row after
+// the barrier; a buffer is rewritten two tile steps later, after every
+// reader has passed the barrier in between.
+//   FINAL false: the stage right-hand side -> rhs[l] (17_fluxandsource.f95);
+//   FINAL true:  sumfin + varfin + electricfield of the last stage -> U^{n+1}
+//                into rhs[l][0..13], for the primitive-recovery pass.
+// ---------------------------------------------------------------------------
+constexpr int MARCH_COLS = 63;
+
+inline int march_strips(const Geom& g) { return (g.imax + 1 + MARCH_COLS - 1) / MARCH_COLS; }
+
+template <int ORD, int SOLVER, typename T, bool FINAL>
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? (CUEVA_MARCH_F32_WAVES > 0 ? CUEVA_MARCH_F32_WAVES : 1)
+                                                     : (CUEVA_MARCH_F64_WAVES > 0 ? CUEVA_MARCH_F64_WAVES : 1))
+k_faces_march(Geom g, Planes P, Scheme S, int l, int seg, DevStatus* st) {
+  constexpr bool XLDS = SOLVER == 6;            // x solved first, its fluxes through LDS
+  __shared__ T fyb[2][4][FY_N][64];             // y-face rows of this and the previous tile step
+  __shared__ T fyp[FY_N][64];                   // wavefront 0's row below: previous step's top row
+  __shared__ T fxb[XLDS ? 4 : 1][FX_N][64];     // x-face rows (HLLE)
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int i = (int)blockIdx.x * MARCH_COLS - 1 + lane;
+  const int jc0 = (int)blockIdx.y * seg;
+  const int jend = min(jc0 + seg - 1, g.jmax);
+  int b = 0;
+  for (int jt = jc0 - 1; jt <= jend; jt += 4, b ^= 1) {  // uniform over the block: the barrier below
+    const int j = jt + w;
+    const bool face_ok = i <= g.imax && j <= jend;
+    T fx[FX_N];
+    auto put_y = [&](const T* f) {
+#pragma unroll
+      for (int a = 0; a < FY_N; a++) fyb[b][w][a][lane] = f[a];
+    };
+    auto put_x = [&](const T* f) {
+#pragma unroll
+      for (int a = 0; a < FX_N; a++) {
+        if constexpr (XLDS) fxb[w][a][lane] = f[a];
+        else fx[a] = f[a];
+      }
+    };
+    if (face_ok) {
+      face_pair<ORD, SOLVER, T, !XLDS>(g, P, S, st, i, j, put_x, put_y);
+    } else {
+      T z[FX_N > FY_N ? FX_N : FY_N];
+#pragma unroll
+      for (int a = 0; a < (FX_N > FY_N ? FX_N : FY_N); a++) z[a] = T(0);
+      put_x(z);
+      put_y(z);
+    }
+    double dxf[FX_N];
+    if constexpr (!XLDS) {
+#pragma unroll
+      for (int a = 0; a < FX_N; a++) {
+        const T left = __shfl_up(fx[a], 1, 64);
+        dxf[a] = (double)ddx<T>(S, fx[a] - left);
+      }
+    }
+    __syncthreads();
+    const bool cell_ok = face_ok && lane >= 1 && j >= jc0;
+    if (cell_ok) {
+      if constexpr (XLDS) {
+#pragma unroll
+        for (int a = 0; a < FX_N; a++) dxf[a] = (double)ddx<T>(S, fxb[w][a][lane] - fxb[w][a][lane - 1]);
+      }
+      double dyf[FY_N];
+#pragma unroll
+      for (int a = 0; a < FY_N; a++) {
+        const T below = w == 0 ? fyp[a][lane] : fyb[b][w - 1][a][lane];
+        dyf[a] = (double)ddy<T>(S, fyb[b][w][a][lane] - below);
+      }
+      const int64_t k = g.at(i, j);
+      const Rhs r = rhs_of(P, S, k, dxf, dyf);
+      if constexpr (!FINAL) {
+#pragma unroll
+        for (int t = 0; t < R_N; t++) P.rhs[l][t][k] = r.r[t];
+      } else {
+        double nu[A_N];
+        final_update_from(P, S, k, r, nu);
+#pragma unroll
+        for (int a = 0; a < A_N; a++) P.rhs[l][a][k] = nu[a];
+      }
+    }
+    if (w == 0) {
+#pragma unroll
+      for (int a = 0; a < FY_N; a++) fyp[a][lane] = fyb[b][3][a][lane];
+    }
+    if constexpr (XLDS) __builtin_amdgcn_wave_barrier();  // this step's fxb reads before the next step's writes
+  }
+}
+
+// kernel of (rec_order, solver, stage kind), for a launch by pointer
+template <typename T>
+inline const void* march_kernel(int rec_order, int solver, bool final_stage) {
+#define CUEVA_MK(O, SV)                                                        \
+  (final_stage ? reinterpret_cast<const void*>(&k_faces_march<O, SV, T, true>) \
+               : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false>))
+  switch (rec_order * 10 + solver) {
+    case 55: return CUEVA_MK(5, 5);
+    case 56: return CUEVA_MK(5, 6);
+    case 57: return CUEVA_MK(5, 7);
+    case 75: return CUEVA_MK(7, 5);
+    case 76: return CUEVA_MK(7, 6);
+    case 77: return CUEVA_MK(7, 7);
+    case 95: return CUEVA_MK(9, 5);
+    case 96: return CUEVA_MK(9, 6);
+    case 97: return CUEVA_MK(9, 7);
+    default: return nullptr;
+  }
+#undef CUEVA_MK
 }
 
 }  // namespace cueva

@@ -11,6 +11,7 @@
 
 #include "cueva_common.hpp"
 #include "cueva_faces.hpp"
+#include "cueva_update.hpp"
 #include "cueva_mlp.hpp"
 
 namespace cueva {
@@ -151,16 +152,10 @@ __global__ void k_varfin(Geom g, Planes P, Scheme S, int which) {
   }
 }
 
-// primitive recovery over the interior; reads D tau S and E B of the new step
-__device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevStatus* st, int i, int j,
-                                 int64_t k) {
-  C2PIn in;
-  in.d = P.c[C_D][k]; in.tau = P.c[C_TAU][k];
-  in.sx = P.c[C_SX][k]; in.sy = P.c[C_SY][k]; in.sz = P.c[C_SZ][k];
-  in.ex = P.p[P_EX][k]; in.ey = P.p[P_EY][k]; in.ez = P.p[P_EZ][k];
-  in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
-  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
-  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
+// primitive recovery of one point (Cardano, 12_varprimitive.f95) and the
+// store of its primitives and W
+__device__ inline void c2p_apply(Planes& P, const Scheme& S, DevStatus* st, int i, int j, int64_t k,
+                                 const C2PIn& in) {
   C2POut o;
   int code = c2p_cardano_full(S.gamma, in, o, S.crmath != 0);
   if (code) { report(st, code, i, j); return; }
@@ -172,15 +167,28 @@ __device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevS
   P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
 }
 
+// primitive recovery over the interior; reads D tau S and E B of the new step
+__device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevStatus* st, int i, int j,
+                                 int64_t k) {
+  C2PIn in;
+  in.d = P.c[C_D][k]; in.tau = P.c[C_TAU][k];
+  in.sx = P.c[C_SX][k]; in.sy = P.c[C_SY][k]; in.sz = P.c[C_SZ][k];
+  in.ex = P.p[P_EX][k]; in.ey = P.p[P_EY][k]; in.ez = P.p[P_EZ][k];
+  in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
+  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
+  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
+  c2p_apply(P, S, st, i, j, k, in);
+}
+
 __global__ void k_c2p(Geom g, Planes P, Scheme S, DevStatus* st) {
   INTERIOR_IJ
   c2p_store(g, P, S, st, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
-// flux differences and stiff sources, 17_fluxandsource.f95:442-542
+// flux differences and stiff sources from the face-flux planes,
+// 17_fluxandsource.f95:442-542 (arithmetic in rhs_of)
 // ---------------------------------------------------------------------------
-struct Rhs { double r[R_N]; };
 
 // face-flux planes of the flux arithmetic FT (double, or float in the mixed form)
 template <typename FT> __device__ inline const FT* flux_x(const Planes& P, int a);
@@ -190,42 +198,15 @@ template <> __device__ inline const double* flux_y<double>(const Planes& P, int 
 template <> __device__ inline const float* flux_x<float>(const Planes& P, int a) { return P.fxf[a]; }
 template <> __device__ inline const float* flux_y<float>(const Planes& P, int a) { return P.fyf[a]; }
 
-template <typename FT> __device__ inline FT ddx(const Scheme& S, FT a) {
-  return S.div_exact ? a * (FT)S.inv_dx : a / (FT)S.dx;
-}
-template <typename FT> __device__ inline FT ddy(const Scheme& S, FT a) {
-  return S.div_exact ? a * (FT)S.inv_dy : a / (FT)S.dy;
-}
-
-// flux differences in FT, combined with the FP64 sources in FP64
 template <typename FT>
 __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k) {
   const int64_t kw = g.at(i - 1, j), ks = g.at(i, j - 1);
-#define DXF(a) ((double)ddx<FT>(S, flux_x<FT>(P, a)[k] - flux_x<FT>(P, a)[kw]))
-#define DYF(a) ((double)ddy<FT>(S, flux_y<FT>(P, a)[k] - flux_y<FT>(P, a)[ks]))
-  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
-  double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
-  double q = P.s[S_Q][k];
-  Rhs o;
-  o.r[0] = ((((-DXF(FX_EXPSI)) - DYF(FY_EYPSI)) - 0.0) + q) - S.kpsi * P.s[S_PSI][k];
-  o.r[1] = (((-DXF(FX_BXPHI)) - DYF(FY_BYPHI)) - 0.0) - S.kphi * P.s[S_PHI][k];
-  o.r[2] = ((-DYF(FY_EZBX)) + 0.0) - DXF(FX_PHIBX);
-  o.r[3] = ((-0.0) + DXF(FX_EZBY)) - DYF(FY_PHIBY);
-  o.r[4] = ((-DXF(FX_EYBZ)) + DYF(FY_EXBZ)) - 0.0;
-  o.r[5] = ((DYF(FY_BZEX) - 0.0) - DXF(FX_PSIEX)) - q * vx;
-  o.r[6] = ((0.0 - DXF(FX_BZEY)) - DYF(FY_PSIEY)) - q * vy;
-  o.r[7] = ((DXF(FX_BYEZ) - DYF(FY_BXEZ)) - 0.0) - q * vz;
-  o.r[8] = ((-DXF(FX_J)) - DYF(FY_J)) - 0.0;
-  o.r[9] = ((-DXF(FX_FD)) - DYF(FY_FD)) - 0.0;
-  o.r[10] = ((-DXF(FX_FTAU)) - DYF(FY_FTAU)) - 0.0;
-  o.r[11] = ((-DXF(FX_FSX)) - DYF(FY_FSX)) - 0.0;
-  o.r[12] = ((-DXF(FX_FSY)) - DYF(FY_FSY)) - 0.0;
-  o.r[13] = ((-DXF(FX_FSZ)) - DYF(FY_FSZ)) - 0.0;
-#undef DXF
-#undef DYF
-  ohm_source(S.sigma, w, vx, vy, vz, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k], P.s[S_BY][k],
-             P.s[S_BZ][k], o.r[R_SEX], o.r[R_SEY], o.r[R_SEZ]);
-  return o;
+  double dxf[FX_N], dyf[FY_N];
+#pragma unroll
+  for (int a = 0; a < FX_N; a++) dxf[a] = (double)ddx<FT>(S, flux_x<FT>(P, a)[k] - flux_x<FT>(P, a)[kw]);
+#pragma unroll
+  for (int a = 0; a < FY_N; a++) dyf[a] = (double)ddy<FT>(S, flux_y<FT>(P, a)[k] - flux_y<FT>(P, a)[ks]);
+  return rhs_of(P, S, k, dxf, dyf);
 }
 
 template <typename FT>
@@ -293,37 +274,7 @@ __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
 template <typename FT>
 __device__ inline void final_update(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k,
                                     double* nu) {
-  const int L = S.lmax - 1;
-  Rhs last = flux_source<FT>(g, P, S, i, j, k);
-  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
-  double ex0 = P.p[P_EX][k], ey0 = P.p[P_EY][k], ez0 = P.p[P_EZ][k];
-  double bx0 = P.p[P_BX][k], by0 = P.p[P_BY][k], bz0 = P.p[P_BZ][k];
-  Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, P.p[P_RHO][k], P.p[P_P][k]);
-  double un[A_N] = {P.p[P_PSI][k], P.p[P_PHI][k], bx0, by0, bz0, ex0, ey0, ez0, P.p[P_Q][k],
-                    cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
-#pragma unroll
-  for (int a = 0; a < A_N; a++) {
-    double acc = 0.0;
-    for (int l = 0; l <= L; l++) {
-      double f = l == L ? last.r[a] : P.rhs[l][a][k];
-      if (a >= A_EX && a <= A_EZ) {
-        double s = l == L ? last.r[R_SEX + (a - A_EX)] : P.rhs[l][R_SEX + (a - A_EX)][k];
-        acc = acc + (S.wb[l] * f - S.wbt[l] * s);
-      } else {
-        acc = acc + S.wb[l] * f;
-      }
-    }
-    nu[a] = un[a] + S.dt * acc;
-  }
-}
-
-__device__ inline void store_update(Planes& P, int64_t k, const double* nu) {
-  P.p[P_PSI][k] = nu[A_PSI]; P.p[P_PHI][k] = nu[A_PHI];
-  P.p[P_BX][k] = nu[A_BX]; P.p[P_BY][k] = nu[A_BY]; P.p[P_BZ][k] = nu[A_BZ];
-  P.p[P_EX][k] = nu[A_EX]; P.p[P_EY][k] = nu[A_EY]; P.p[P_EZ][k] = nu[A_EZ];
-  P.p[P_Q][k] = nu[A_Q];
-  P.c[C_D][k] = nu[A_D]; P.c[C_TAU][k] = nu[A_TAU];
-  P.c[C_SX][k] = nu[A_SX]; P.c[C_SY][k] = nu[A_SY]; P.c[C_SZ][k] = nu[A_SZ];
+  final_update_from(P, S, k, flux_source<FT>(g, P, S, i, j, k), nu);
 }
 
 // Last stage + sumfin + varfin + electricfield + varprimitivecardano
@@ -343,7 +294,7 @@ __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
 // state is stored before the network runs and read back after it, so that
 // it does not occupy registers across the network.
 template <typename FT>
-__global__ void __launch_bounds__(256) k_final_learned(Geom g, Planes P, Scheme S, const float* __restrict__ wp,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_final_learned(Geom g, Planes P, Scheme S, const float* __restrict__ wp,
                                                        DevStatus* st) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const int j0 = blockIdx.y * blockDim.y + threadIdx.y;

@@ -1,7 +1,9 @@
 // FP32 face kernels of the mixed form: reconstruction, Riemann solve and
-// flux writes in FP32, built in their own translation unit with contraction
-// to fused multiply-add allowed (the FP64 path is built without it, as the
-// reference build evaluates products and sums separately).
+// flux differences in FP32, built in their own translation unit where a
+// product may fuse into the sum of the same expression (the FP64 path is
+// built without contraction, as the reference build evaluates products and
+// sums separately; the FP64 sources and update compiled here turn it off in
+// their own bodies).
 #include <hip/hip_runtime.h>
 
 #include "cueva_faces.hpp"
@@ -28,6 +30,11 @@ hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, 
     case 7: return launch_ord<7>(grid, blk, s, g, P, S, st);
     default: return launch_ord<9>(grid, blk, s, g, P, S, st);
   }
+}
+
+// the FP32 marching face kernel of (rec_order, solver, stage kind)
+const void* march_kernel_fp32(int rec_order, int solver, bool final_stage) {
+  return march_kernel<float>(rec_order, solver, final_stage);
 }
 
 }  // namespace cueva

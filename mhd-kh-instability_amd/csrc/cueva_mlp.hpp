@@ -46,19 +46,27 @@ __host__ __device__ inline uint16_t bf16_bits(float f) {
 __host__ __device__ inline float bf16_round(float f) {
   return __builtin_bit_cast(float, (uint32_t)bf16_bits(f) << 16);
 }
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// four floats -> one BF16 operand, nearest even: two v_cvt_pk_bf16_f32
 __device__ inline bf16x4 pack4(float a, float b, float c, float d) {
+  const f32x2 lo = {a, b}, hi = {c, d};
+  const bf16x2 l = __builtin_convertvector(lo, bf16x2), h = __builtin_convertvector(hi, bf16x2);
+  const uint32_t lu = __builtin_bit_cast(uint32_t, l), hu = __builtin_bit_cast(uint32_t, h);
   bf16x4 r;
-  r[0] = (short)bf16_bits(a);
-  r[1] = (short)bf16_bits(b);
-  r[2] = (short)bf16_bits(c);
-  r[3] = (short)bf16_bits(d);
+  r[0] = (short)(lu & 0xffffu);
+  r[1] = (short)(lu >> 16);
+  r[2] = (short)(hu & 0xffffu);
+  r[3] = (short)(hu >> 16);
   return r;
 }
 
 __device__ inline float tanh_f(float x) {
-  // tanh(x) = 1 - 2/(exp(2x)+1); saturates cleanly for large |x|
-  float e = __expf(2.0f * x);
-  return 1.0f - 2.0f / (e + 1.0f);
+  // tanh(x) = 1 - 2/(exp(2x)+1) with the hardware exp2 and reciprocal (about
+  // 1 ulp each, far below the BF16 rounding the activation goes through);
+  // saturates cleanly for large |x|
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2/ln 2
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
 __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -175,9 +183,10 @@ __device__ inline void c2p_features(double gamma, const C2PIn& in, const float* 
   using namespace mlp;
   const Quartic qd = c2p_quartic(gamma, in);
   const double C2 = qd.c2, sf = sqrt(qd.c1);
-  f0 = ((float)log(in.d / C2) - wp[OFF_IN_MEAN + 0]) / wp[OFF_IN_STD + 0];
+  // the features only need single precision (the network reads them in BF16)
+  f0 = (__logf((float)(in.d / C2)) - wp[OFF_IN_MEAN + 0]) / wp[OFF_IN_STD + 0];
   f1 = ((float)(sf / C2) - wp[OFF_IN_MEAN + 1]) / wp[OFF_IN_STD + 1];
-  f2 = ((float)log(C2) - wp[OFF_IN_MEAN + 2]) / wp[OFF_IN_STD + 2];
+  f2 = (__logf((float)C2) - wp[OFF_IN_MEAN + 2]) / wp[OFF_IN_STD + 2];
   if (!(f0 == f0) || !(f1 == f1) || !(f2 == f2)) f0 = f1 = f2 = 0.0f;  // unphysical: the start is rejected later
 }
 
@@ -199,10 +208,10 @@ __device__ inline int c2p_learned_finish(double gamma, const C2PIn& in, const fl
   // W = 1/sqrt(1 - v^2); its error is v^2 W^2 p/(C2+p) times that of p,
   // an order of magnitude below the error of the network's own W, which is
   // the start only where that v would not be subluminal.
-  const double lp_net = (double)(y1 * wp[OFF_OUT_STD + 1] + wp[OFF_OUT_MEAN + 1]);
-  const double pn = exp(lp_net) * C2;
+  const float lp_net = y1 * wp[OFF_OUT_STD + 1] + wp[OFF_OUT_MEAN + 1];
+  const double pn = (double)__expf(lp_net) * C2;
   const double ve = sf / (C2 + pn);
-  double W = ve < 1.0 ? 1.0 / sqrt(1.0 - ve * ve) : exp((double)(y0 * wp[OFF_OUT_STD + 0] + wp[OFF_OUT_MEAN + 0]));
+  double W = ve < 1.0 ? 1.0 / sqrt(1.0 - ve * ve) : (double)__expf(y0 * wp[OFF_OUT_STD + 0] + wp[OFF_OUT_MEAN + 0]);
   W = c2p_newton_w(qd, W);
   W = c2p_newton_w(qd, W);
   bool ok = W >= 1.0 && c2p_residual(qd, W) <= 1.0e-10;
@@ -211,7 +220,7 @@ __device__ inline int c2p_learned_finish(double gamma, const C2PIn& in, const fl
     const double rho = in.d / W;
     const double h = (qd.c2 - qd.g1 * rho) / (sq(W) - qd.g1);
     const double pw = qd.g1 * (h - rho);
-    ok = pw > 0.0 && qd.c1 < sq(qd.c2 + pw) && fabs(log(pw / qd.c2) - lp_net) <= 0.6931471805599453;
+    ok = pw > 0.0 && qd.c1 < sq(qd.c2 + pw) && fabsf(__logf((float)(pw / qd.c2)) - lp_net) <= 0.6931472f;
   }
   if (ok) return c2p_finish(gamma, in, qd, W, false, out);
   fallback = true;

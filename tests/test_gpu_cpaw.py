@@ -8,7 +8,13 @@ one period the numerical fields are compared with the exact ones at the
 same time (phase shifted by VA t).  With rho = 1, p = 0.1, B0 = 1 the
 reference's Cardano recovery itself stops (det < 0) after 290 of the 496
 steps of the 128^2 period (the CPU restatement shows the same), so the
-convergence is measured on 16^2, 32^2 and 64^2.
+convergence is measured on 16^2, 32^2 and 64^2, with MP5 + HLL.
+
+The mixed form (FP32 faces) gives HLL's errors to four digits at every
+size.  With HLLC its FP32 contact solve loses the z components of this
+wave (v_z, E_y wrong by O(1) at 32^2, Cardano det < 0 at 64^2), while FP64
+HLLC converges like HLL: the FP32 HLLC path is not used for this wave, and
+HLLC is checked here in FP64 only.
 """
 import sys
 
@@ -35,8 +41,8 @@ def exact(n, va, t, eta=0.1, b0=1.0):
             "Ey": va * eta * b0 * s, "Ez": -va * eta * b0 * c}
 
 
-def run(n, precision=0, periods=1.0):
-    p, prims, va = problems.cpaw_problem(n, precision=precision, crmath=1 - precision)
+def run(n, precision=0, periods=1.0, flux_solver=cueva.HLL):
+    p, prims, va = problems.cpaw_problem(n, flux_solver=flux_solver, precision=precision, crmath=1 - precision)
     dt = cueva.derived(p)["dt"]
     steps = int(round(periods / va / dt))
     with cueva.Solver(p) as s:
@@ -61,7 +67,16 @@ def test_cpaw_converges_and_mixed_matches_fp64():
         order = np.log2(errs[32][f] / errs[64][f])
         say("order %s 32->64: %.2f" % (f, order))
         assert order > 0.8, (f, order)
-    mixed, _ = run(64, precision=1)
-    say("mixed n=64", " ".join("%s=%.2e" % kv for kv in mixed.items()))
+    for n in (16, 32, 64):
+        mixed, _ = run(n, precision=1)
+        say("mixed n=%d" % n, " ".join("%s=%.2e" % kv for kv in mixed.items()))
+        for f in FIELDS:
+            assert abs(mixed[f] - errs[n][f]) <= 0.05 * errs[n][f], (n, f, mixed[f], errs[n][f])
+
+
+def test_cpaw_converges_with_hllc_fp64():
+    errs = {n: run(n, flux_solver=cueva.HLLC)[0] for n in (32, 64)}
     for f in FIELDS:
-        assert abs(mixed[f] - errs[64][f]) <= 0.05 * errs[64][f], (f, mixed[f], errs[64][f])
+        order = np.log2(errs[32][f] / errs[64][f])
+        say("HLLC order %s 32->64: %.2f" % (f, order))
+        assert order > 0.8, (f, order)

@@ -1,0 +1,121 @@
+"""The marching face kernel against the face-plane path.
+
+cueva_step's default face pass (k_faces_march) reconstructs, solves the
+Riemann problems and forms the flux differences of a strip of 63 columns in
+one launch, keeping the face fluxes in LDS; on the last stage it also applies
+sumfin + varfin + electricfield, and the primitive recovery follows in its
+own pass.  CUEVA_MARCH=0 selects the face-plane path (k_faces writes every
+face flux to HBM, k_fluxandsource / k_final_fused difference them), which the
+per-subroutine entry points also use.
+
+* FP64: both paths evaluate the same expressions in the same order, so the
+  fields are bit-identical, for every solver, reconstruction order and IMEX
+  scheme, on grids of one and of two (ragged) strips, with segments of 3
+  rows, 23 rows (ragged last segment), the whole grid, and the length sized
+  for the device.
+* Mixed form: the FP32 faces are the same code in the same unit, the FP32
+  flux differences the same operations, and the FP64 sources and update are
+  evaluated without contraction in both units: bit-identical as well.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cueva
+from parity import interior, ndiff, rel_l1
+
+pytestmark = pytest.mark.gpu
+
+# name: (flux solver, rec_mp, IMEX order, sigma, init keywords)
+CASES = {
+    "hll_mp5": (cueva.HLL, 0, 2, 1.0e3, {}),
+    "hllc_mp5": (cueva.HLLC, 0, 2, 1.0e6, {}),
+    "hllc_mp7": (cueva.HLLC, 1, 2, 1.0e6, {}),
+    "hlle_mp7_bz": (cueva.HLLE, 1, 2, 1.0e6, {"bz0": 0.1}),
+    "hll_mp9_ssp3": (cueva.HLL, 2, 4, 1.0e3, {}),
+    "hllc_mp9": (cueva.HLLC, 2, 2, 1.0e6, {}),
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(p, prims, steps, march, seg=None):
+    old = {k: os.environ.get(k) for k in ("CUEVA_MARCH", "CUEVA_MARCH_SEG")}
+    os.environ["CUEVA_MARCH"] = "1" if march else "0"
+    if seg is not None:
+        os.environ["CUEVA_MARCH_SEG"] = str(seg)
+    else:
+        os.environ.pop("CUEVA_MARCH_SEG", None)
+    try:
+        with cueva.Solver(p) as s:
+            s.upload(prims)
+            counters = []
+            for _ in range(steps):
+                s.step(1)
+                counters.append(s.step_counters())
+            return interior(*s.download(), p), counters
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("shape", [(70, 37), (130, 68)])
+def test_march_equals_face_planes_fp64(name, shape):
+    fs, rec, order, sigma, kw = CASES[name]
+    ni, nj = shape
+    p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True, **kw)
+    steps = 3 if order == 4 else 5  # SSP3(4,3,3) stops within 8 steps on this problem
+    say("case", name, "grid %dx%d" % (p.imax, p.jmax), "steps", steps)
+    ref, cref = run(p, prims, steps, march=False)
+    for seg in (None, 3, 23, 4095):  # sized for the device, shortest, ragged, one segment
+        got, cgot = run(p, prims, steps, march=True, seg=seg)
+        nd = ndiff(got, ref)
+        say("  segment rows", seg, "differing points", sum(nd.values()))
+        assert cgot == cref
+        assert np.array_equal(got, ref), nd
+
+
+@pytest.mark.parametrize("name", ["hllc_mp5", "hlle_mp7_bz", "hll_mp9_ssp3"])
+def test_march_matches_face_planes_mixed(name):
+    fs, rec, order, sigma, kw = CASES[name]
+    p, prims = cueva.kh_problem(130, 68, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True,
+                                precision=cueva.MIXED, crmath=0, **kw)
+    say("case", name, "mixed", "grid %dx%d" % (p.imax, p.jmax))
+    steps = 3 if order == 4 else 5
+    ref, _ = run(p, prims, steps, march=False)
+    got, _ = run(p, prims, steps, march=True)
+    nd = ndiff(got, ref)
+    say("  differing points", sum(nd.values()), "max rel L1 %.2e" % max(rel_l1(got, ref).values()))
+    assert np.array_equal(got, ref), nd
+
+
+def test_march_learned_matches_face_planes():
+    from cueva import learned_c2p as lc
+    if not os.path.exists(lc.PARAMS_FILE):
+        pytest.skip("no trained parameters shipped")
+    packed = lc.load_params()
+    p, prims = cueva.kh_problem(130, 68, sigma=1.0e6, flux_solver=cueva.HLLC, rotated=True, c2p_mode=cueva.C2P_LEARNED)
+    out = {}
+    for march in (False, True):
+        os.environ["CUEVA_MARCH"] = "1" if march else "0"
+        try:
+            with cueva.Solver(p) as s:
+                s.set_c2p_params(packed)
+                s.upload(prims)
+                s.step(5)
+                c, fb = s.step_counters()
+                out[march] = (interior(*s.download(), p), c, fb)
+        finally:
+            os.environ.pop("CUEVA_MARCH", None)
+    say("fallbacks", out[False][2], out[True][2])
+    assert out[True][1] == out[False][1] and out[True][2] == out[False][2]
+    assert np.array_equal(out[True][0], out[False][0])
