@@ -179,6 +179,39 @@ def accuracy(cfg, form, packed, steps=ACCURACY_STEPS):
     return {n: float(np.abs(a[k] - b[k]).sum() / max(np.abs(b[k]).sum(), 1e-300)) for k, n in enumerate(names)}
 
 
+GROWTH_STEPS = 280  # cfg2 stops at step 290 in every form (the reference's HLLC contact solve)
+
+
+def growth(cfg, form, packed, nsteps=GROWTH_STEPS, every=10):
+    """KH vy-amplitude growth rate of `form` and of plain FP64 on the same samples."""
+    import cueva
+    from cueva import diagnostics, problems
+    res = {}
+    for f in ("plain_fp64", form):
+        p, prims = problems.config(cfg, **FORMS[f])
+        dt = cueva.derived(p)["dt"]
+        with cueva.Solver(p) as s:
+            if packed is not None:
+                s.set_c2p_params(packed)
+            s.upload(prims)
+            del prims
+            res[f] = diagnostics.amplitude_series(s, nsteps, every, dt)
+    t0, a0, stop0 = res["plain_fp64"]
+    win = diagnostics.growth_window(a0)
+    out = {"steps": nsteps, "every": every, "amplitude": "(max vy - min vy)/2 over the interior"}
+    if win is None:
+        out["note"] = "amplitude did not reach 1.5x its initial value"
+        return out
+    out["window_t"] = [float(t0[win[0]]), float(t0[win[1] - 1])]
+    for f, (t, a, stop) in res.items():
+        out[f] = {"rate": diagnostics.growth_rate(t, a, win) if len(a) >= win[1] else None,
+                  "A_first": float(a[0]), "A_last": float(a[-1]),
+                  "stopped": None if stop is None else [stop[0], str(stop[1])]}
+    if out[form]["rate"] is not None:
+        out["rel_diff_vs_plain_fp64"] = out[form]["rate"] / out["plain_fp64"]["rate"] - 1.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,7 +221,7 @@ def main():
     ap.add_argument("--form", default=None, choices=sorted(FORMS),
                     help="time one form only (default: plain FP64 and mixed + learned c2p)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the 100-step accuracy and growth runs")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -256,6 +289,8 @@ def main():
         if not args.no_accuracy:
             log("accuracy: %d steps of %s against plain FP64" % (ACCURACY_STEPS, head))
             out["rel_l1_vs_plain_fp64_after_%d_steps" % ACCURACY_STEPS] = accuracy(cfg, head, packed)
+            log("growth: %d steps of plain FP64 and %s" % (GROWTH_STEPS, head))
+            out["kh_vy_growth"] = growth(cfg, head, packed)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(cfg)

@@ -175,6 +175,12 @@ int cueva_max_speed(cueva_ctx *ctx, double *vmax);
  * restarts from primitives only). */
 int cueva_upload_cons(cueva_ctx *ctx, const double *cons5);
 
+/* Minimum (out[0]) and maximum (out[1]) of one device plane over the
+ * interior points; group and index as cueva_download_plane.  No reference
+ * counterpart: the KH transverse-velocity amplitude (max vy - min vy)/2 of a
+ * growth-rate series without copying the plane to the host. */
+int cueva_plane_range(cueva_ctx *ctx, int group, int index, double out[2]);
+
 /* Copy one device plane to the host (host layout).  group: 0 primitives
  * (14, restart order), 1 D tau Sx Sy Sz W, 2 stage planes, 3 x-face fluxes,
  * 4 y-face fluxes, 5+l stage-l right-hand sides (17).  For tests. */

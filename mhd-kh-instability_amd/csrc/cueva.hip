@@ -20,7 +20,7 @@ hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, 
 const void* march_kernel_fp32(int rec_order, int solver, bool final_stage);
 // learned primitive recovery (cueva_mlp.hip)
 hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
-                              DevStatus* st);
+                              DevStatus* st, bool stage0);
 }
 
 using namespace cueva;
@@ -338,11 +338,14 @@ static dim3 interior_grid(const Geom& g, dim3 blk) {
 
 static int launch_ok() { return hipGetLastError() == hipSuccess ? CUEVA_OK : CUEVA_E_HIP; }
 
-static int fill_ghosts(cueva_ctx* c, double* const* planes, int n) {
+// copy every ghost point of the planes from its interior source point; the
+// last nzero planes (at most 20) are set to 0 there instead
+static int fill_ghosts(cueva_ctx* c, double* const* planes, int n, int nzero = 0) {
   PlaneList pl;
   memset(&pl, 0, sizeof(pl));
   for (int start = 0; start < n; start += 20) {
     pl.n = n - start < 20 ? n - start : 20;
+    pl.nzero = start + pl.n == n ? nzero : 0;
     for (int k = 0; k < pl.n; k++) pl.a[k] = planes[start + k];
     int64_t cnt = ghost_count(c->g);
     LAUNCH(K_GHOSTS, k_fill_ghosts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), c->g, pl);
@@ -516,15 +519,19 @@ static int read_status(cueva_ctx* c, DevStatus* h) {
   return CUEVA_OK;
 }
 
-static int launch_c2p(cueva_ctx* c, int cls = K_OTHER) {
+// stage0: also the next step's stage-0 state at the interior points
+static int launch_c2p(cueva_ctx* c, int cls = K_OTHER, bool stage0 = false) {
   if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
     if (!c->has_mlp) return CUEVA_E_ARG;
     prof_begin(c, cls);
-    hipError_t e = launch_c2p_learned(c->stream, c->g, c->P, c->S, c->mlpw, c->dst);
+    hipError_t e = launch_c2p_learned(c->stream, c->g, c->P, c->S, c->mlpw, c->dst, stage0);
     prof_end(c);
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
-  LAUNCH(cls, k_c2p, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
+  if (stage0)
+    LAUNCH(cls, k_c2p<true>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
+  else
+    LAUNCH(cls, k_c2p<false>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
   return launch_ok() ? CUEVA_E_HIP : CUEVA_OK;
 }
 
@@ -620,20 +627,34 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   const int halo = (c->rec_order - 1) / 2 + 1;  // stencil radius K plus the face offset
   const bool learned = c->prm.c2p_mode == CUEVA_C2P_LEARNED;
   if (learned && !c->has_mlp) return CUEVA_E_ARG;
+  bool s0_ready = false;  // this step's stage-0 state was formed by the previous step's recovery pass
   for (int h = 0; h < nsteps; h++) {
     for (int l = 0; l < c->S.lmax; l++) {
       const bool last = l == c->S.lmax - 1;
-      dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
-              (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
-      LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo);
-      if (launch_ok()) return CUEVA_E_HIP;
+      if (!(l == 0 && s0_ready)) {
+        dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
+                (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
+        LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo);
+        if (launch_ok()) return CUEVA_E_HIP;
+      }
       if (c->march) {
         // reconstruction + Riemann + flux difference (+ final update) in one pass
         int rc = launch_march(c, l, last);
         if (rc) return rc;
         if (last) {
           adopt_update(c, l);
-          if ((rc = launch_c2p(c, K_FINAL))) return rc;
+          // the primitive recovery also forms the next step's stage-0 state
+          // (pointwise, so its ghost band is a copy; HLLE's J ghosts are 0)
+          const bool fuse = h + 1 < nsteps;
+          if ((rc = launch_c2p(c, K_FINAL, fuse))) return rc;
+          if (fuse) {
+            double* const sp[17] = {c->P.s[S_PSI], c->P.s[S_PHI], c->P.s[S_BX], c->P.s[S_BY], c->P.s[S_BZ],
+                                    c->P.s[S_Q], c->P.s[S_D], c->P.s[S_TAU], c->P.s[S_SX], c->P.s[S_SY],
+                                    c->P.s[S_SZ], c->P.s[S_EX], c->P.s[S_EY], c->P.s[S_EZ],
+                                    c->P.s[S_JX], c->P.s[S_JY], c->P.s[S_JZ]};
+            if ((rc = fill_ghosts(c, sp, 17, c->prm.flux_solver == CUEVA_HLLE ? 3 : 0))) return rc;
+          }
+          s0_ready = fuse;
         }
         continue;
       }
@@ -686,16 +707,34 @@ extern "C" int cueva_upload_cons(cueva_ctx* c, const double* cons5) {
   return copy_planes_h2d(c, c->P.c, 5, cons5);
 }
 
+// device plane of (group, index) as cueva_download_plane numbers them, or null
+static double* plane_of(cueva_ctx* c, int group, int index) {
+  if (index < 0) return nullptr;
+  if (group == 0 && index < P_N) return c->P.p[index];
+  if (group == 1 && index < C_N) return c->P.c[index];
+  if (group == 2 && index < S_N) return c->P.s[index];
+  if (group == 3 && index < FX_N) return c->P.fx[index];
+  if (group == 4 && index < FY_N) return c->P.fy[index];
+  if (group >= 5 && group < 5 + MAXL && index < R_N) return c->P.rhs[group - 5][index];
+  return nullptr;
+}
+
+extern "C" int cueva_plane_range(cueva_ctx* c, int group, int index, double out[2]) {
+  if (!c || !out) return CUEVA_E_ARG;
+  const double* a = plane_of(c, group, index);
+  if (!a) return CUEVA_E_ARG;
+  double* res = c->red + (size_t)RED_BLOCKS * G_N;
+  LAUNCH(K_OTHER, k_range_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), c->g, a, c->red);
+  LAUNCH(K_OTHER, k_range_final, dim3(1), dim3(RED_THREADS), (const double*)c->red, res);
+  HIPCHK(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
+}
+
 extern "C" int cueva_download_plane(cueva_ctx* c, int group, int index, double* out) {
-  if (!c || !out || index < 0) return CUEVA_E_ARG;
-  double* src;
-  if (group == 0 && index < P_N) src = c->P.p[index];
-  else if (group == 1 && index < C_N) src = c->P.c[index];
-  else if (group == 2 && index < S_N) src = c->P.s[index];
-  else if (group == 3 && index < FX_N) src = c->P.fx[index];
-  else if (group == 4 && index < FY_N) src = c->P.fy[index];
-  else if (group >= 5 && group < 5 + MAXL && index < R_N) src = c->P.rhs[group - 5][index];
-  else return CUEVA_E_ARG;
+  if (!c || !out) return CUEVA_E_ARG;
+  double* src = plane_of(c, group, index);
+  if (!src) return CUEVA_E_ARG;
   int rc = copy_planes_d2h(c, &src, 1, out);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));

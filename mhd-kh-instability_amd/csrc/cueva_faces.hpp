@@ -270,12 +270,12 @@ k_faces_march(Geom g, Planes P, Scheme S, int l, int seg, DevStatus* st) {
       put_x(z);
       put_y(z);
     }
-    double dxf[FX_N];
+    T dfx[FX_N];
     if constexpr (!XLDS) {
 #pragma unroll
       for (int a = 0; a < FX_N; a++) {
         const T left = __shfl_up(fx[a], 1, 64);
-        dxf[a] = (double)ddx<T>(S, fx[a] - left);
+        dfx[a] = ddx<T>(S, fx[a] - left);
       }
     }
     __syncthreads();
@@ -283,16 +283,16 @@ k_faces_march(Geom g, Planes P, Scheme S, int l, int seg, DevStatus* st) {
     if (cell_ok) {
       if constexpr (XLDS) {
 #pragma unroll
-        for (int a = 0; a < FX_N; a++) dxf[a] = (double)ddx<T>(S, fxb[w][a][lane] - fxb[w][a][lane - 1]);
+        for (int a = 0; a < FX_N; a++) dfx[a] = ddx<T>(S, fxb[w][a][lane] - fxb[w][a][lane - 1]);
       }
-      double dyf[FY_N];
+      T dfy[FY_N];
 #pragma unroll
       for (int a = 0; a < FY_N; a++) {
         const T below = w == 0 ? fyp[a][lane] : fyb[b][w - 1][a][lane];
-        dyf[a] = (double)ddy<T>(S, fyb[b][w][a][lane] - below);
+        dfy[a] = ddy<T>(S, fyb[b][w][a][lane] - below);
       }
       const int64_t k = g.at(i, j);
-      const Rhs r = rhs_of(P, S, k, dxf, dyf);
+      const Rhs r = rhs_of<T>(P, S, k, dfx, dfy);
       if constexpr (!FINAL) {
 #pragma unroll
         for (int t = 0; t < R_N; t++) P.rhs[l][t][k] = r.r[t];

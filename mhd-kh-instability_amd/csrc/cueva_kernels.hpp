@@ -21,6 +21,7 @@ namespace cueva {
 struct PlaneList {
   double* a[20];
   int n;
+  int nzero;  // the last nzero planes get 0 at every ghost point instead of a copy
 };
 
 __global__ void k_fill_ghosts(Geom g, PlaneList pl) {
@@ -36,7 +37,7 @@ __global__ void k_fill_ghosts(Geom g, PlaneList pl) {
   else return;
   int64_t dst = g.at(i, j);
   int64_t src = g.at(src_index(i, im, g.bcx), src_index(j, jm, g.bcy));
-  for (int k = 0; k < pl.n; k++) pl.a[k][dst] = pl.a[k][src];
+  for (int k = 0; k < pl.n; k++) pl.a[k][dst] = k < pl.n - pl.nzero ? pl.a[k][src] : 0.0;
 }
 
 inline int64_t ghost_count(const Geom& g) { return 12LL * (g.imax + 13) + 12LL * (g.jmax + 1); }
@@ -152,37 +153,20 @@ __global__ void k_varfin(Geom g, Planes P, Scheme S, int which) {
   }
 }
 
-// primitive recovery of one point (Cardano, 12_varprimitive.f95) and the
-// store of its primitives and W
-__device__ inline void c2p_apply(Planes& P, const Scheme& S, DevStatus* st, int i, int j, int64_t k,
-                                 const C2PIn& in) {
-  C2POut o;
-  int code = c2p_cardano_full(S.gamma, in, o, S.crmath != 0);
-  if (code) { report(st, code, i, j); return; }
-  if (o.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
-  if (o.fix_p) atomicAdd(&st->cnt[1], 1ULL);
-  if (o.fix_v) atomicAdd(&st->cnt[2], 1ULL);
-  P.p[P_VX][k] = o.vx; P.p[P_VY][k] = o.vy; P.p[P_VZ][k] = o.vz;
-  P.p[P_RHO][k] = o.rho; P.p[P_P][k] = o.p;
-  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
-}
-
 // primitive recovery over the interior; reads D tau S and E B of the new step
 __device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevStatus* st, int i, int j,
                                  int64_t k) {
-  C2PIn in;
-  in.d = P.c[C_D][k]; in.tau = P.c[C_TAU][k];
-  in.sx = P.c[C_SX][k]; in.sy = P.c[C_SY][k]; in.sz = P.c[C_SZ][k];
-  in.ex = P.p[P_EX][k]; in.ey = P.p[P_EY][k]; in.ez = P.p[P_EZ][k];
-  in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
-  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
-  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
-  c2p_apply(P, S, st, i, j, k, in);
+  C2POut o;
+  c2p_apply(P, S, st, i, j, k, c2p_load(P, k), o);
 }
 
+// STAGE0: also the next step's stage-0 state (fused step)
+template <bool STAGE0 = false>
 __global__ void k_c2p(Geom g, Planes P, Scheme S, DevStatus* st) {
   INTERIOR_IJ
-  c2p_store(g, P, S, st, i, j, k);
+  const C2PIn in = c2p_load(P, k);
+  C2POut o;
+  if (c2p_apply(P, S, st, i, j, k, in, o) && STAGE0) stage0_after_c2p(P, S, k, in, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -201,12 +185,12 @@ template <> __device__ inline const float* flux_y<float>(const Planes& P, int a)
 template <typename FT>
 __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k) {
   const int64_t kw = g.at(i - 1, j), ks = g.at(i, j - 1);
-  double dxf[FX_N], dyf[FY_N];
+  FT dfx[FX_N], dfy[FY_N];
 #pragma unroll
-  for (int a = 0; a < FX_N; a++) dxf[a] = (double)ddx<FT>(S, flux_x<FT>(P, a)[k] - flux_x<FT>(P, a)[kw]);
+  for (int a = 0; a < FX_N; a++) dfx[a] = ddx<FT>(S, flux_x<FT>(P, a)[k] - flux_x<FT>(P, a)[kw]);
 #pragma unroll
-  for (int a = 0; a < FY_N; a++) dyf[a] = (double)ddy<FT>(S, flux_y<FT>(P, a)[k] - flux_y<FT>(P, a)[ks]);
-  return rhs_of(P, S, k, dxf, dyf);
+  for (int a = 0; a < FY_N; a++) dfy[a] = ddy<FT>(S, flux_y<FT>(P, a)[k] - flux_y<FT>(P, a)[ks]);
+  return rhs_of<FT>(P, S, k, dfx, dfy);
 }
 
 template <typename FT>
@@ -223,8 +207,6 @@ __global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l) {
 
 // Stage state at every point the stencils read: the point's ghost map gives
 // the interior point whose values the reference's ghost fills copy.
-// Runs sumint (cumulative, as the reference never resets the sums within a
-// step), varint, electricvarint and flowinit at that interior point.
 __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - halo;
   const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - halo;
@@ -232,41 +214,11 @@ __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
   const int si = src_index(i, g.imax, g.bcx), sj = src_index(j, g.jmax, g.bcy);
   const int64_t k = g.at(i, j), ks = g.at(si, sj);
   const bool ghost = (si != i) || (sj != j);
-  double vx = P.p[P_VX][ks], vy = P.p[P_VY][ks], vz = P.p[P_VZ][ks];
-  double ex0 = P.p[P_EX][ks], ey0 = P.p[P_EY][ks], ez0 = P.p[P_EZ][ks];
-  double bx0 = P.p[P_BX][ks], by0 = P.p[P_BY][ks], bz0 = P.p[P_BZ][ks];
-  Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, P.p[P_RHO][ks], P.p[P_P][ks]);
-  double un[A_N] = {P.p[P_PSI][ks], P.p[P_PHI][ks], bx0, by0, bz0, ex0, ey0, ez0, P.p[P_Q][ks],
-                    cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
-  double stv[A_N];
-#pragma unroll
-  for (int a = 0; a < A_N; a++) {
-    double acc = 0.0;
-    for (int lp = 1; lp <= l; lp++)
-      for (int m = 0; m < lp; m++) {
-        double f = P.rhs[m][a][ks];
-        if (a >= A_EX && a <= A_EZ)
-          acc = acc + (S.Ab[lp][m] * f - S.Abt[lp][m] * P.rhs[m][R_SEX + (a - A_EX)][ks]);
-        else
-          acc = acc + S.Ab[lp][m] * f;
-      }
-    stv[a] = un[a] + S.dt * acc;
-  }
-  double ex, ey, ez;
-  e_solve(S.Abt[l][l], S.sigma, S.dt, cs.w, vx, vy, vz, stv[A_BX], stv[A_BY], stv[A_BZ], stv[A_EX], stv[A_EY],
-          stv[A_EZ], ex, ey, ez);
-  double jx, jy, jz;
-  ohm_source(S.sigma, cs.w, vx, vy, vz, ex, ey, ez, stv[A_BX], stv[A_BY], stv[A_BZ], jx, jy, jz);
-  jx = jx + stv[A_Q] * vx;
-  jy = jy + stv[A_Q] * vy;
-  jz = jz + stv[A_Q] * vz;
-  if (ghost && S.flux_solver == 6) jx = jy = jz = 0.0;  // 24_boundary_flux_conserved.f95:13
-  P.s[S_PSI][k] = stv[A_PSI]; P.s[S_PHI][k] = stv[A_PHI];
-  P.s[S_BX][k] = stv[A_BX]; P.s[S_BY][k] = stv[A_BY]; P.s[S_BZ][k] = stv[A_BZ];
-  P.s[S_EX][k] = ex; P.s[S_EY][k] = ey; P.s[S_EZ][k] = ez;
-  P.s[S_Q][k] = stv[A_Q]; P.s[S_D][k] = stv[A_D]; P.s[S_TAU][k] = stv[A_TAU];
-  P.s[S_SX][k] = stv[A_SX]; P.s[S_SY][k] = stv[A_SY]; P.s[S_SZ][k] = stv[A_SZ];
-  P.s[S_JX][k] = jx; P.s[S_JY][k] = jy; P.s[S_JZ][k] = jz;
+  double out[S_N];
+  stage_point(P, S, l, ks, ghost, P.p[P_VX][ks], P.p[P_VY][ks], P.p[P_VZ][ks], P.p[P_EX][ks], P.p[P_EY][ks],
+              P.p[P_EZ][ks], P.p[P_BX][ks], P.p[P_BY][ks], P.p[P_BZ][ks], P.p[P_RHO][ks], P.p[P_P][ks],
+              P.p[P_PSI][ks], P.p[P_PHI][ks], P.p[P_Q][ks], out);
+  store_stage(P, k, out);
 }
 
 // Last stage right-hand side, sumfin, varfin and electricfield at one

@@ -21,6 +21,7 @@
 #include "cueva.h"
 #include "cueva_common.hpp"
 #include "cueva_mlp.hpp"
+#include "cueva_update.hpp"
 
 namespace cueva {
 namespace mlp {
@@ -178,7 +179,9 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ p
 
 // ---------------------------------------------------------------------------
 // learned primitive recovery over the interior, one thread per cell
+// (STAGE0: also the next step's stage-0 state, as k_c2p<true>)
 // ---------------------------------------------------------------------------
+template <bool STAGE0>
 __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S, const float* __restrict__ wp,
                                                      DevStatus* st) {
   using namespace mlp;
@@ -209,12 +212,17 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
   P.p[P_VX][k] = out.vx; P.p[P_VY][k] = out.vy; P.p[P_VZ][k] = out.vz;
   P.p[P_RHO][k] = out.rho; P.p[P_P][k] = out.p;
   P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(out.vx) + sq(out.vy)) + sq(out.vz)));
+  if (STAGE0) stage0_after_c2p(P, S, k, in, out);
 }
 
 hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
-                              DevStatus* st) {
+                              DevStatus* st, bool stage0) {
   const int64_t ncell = (int64_t)(g.imax + 1) * (g.jmax + 1);
-  hipLaunchKernelGGL(k_c2p_learned, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, g, P, S, wp, st);
+  const dim3 grid((unsigned)((ncell + 255) / 256));
+  if (stage0)
+    hipLaunchKernelGGL(k_c2p_learned<true>, grid, dim3(256), 0, s, g, P, S, wp, st);
+  else
+    hipLaunchKernelGGL(k_c2p_learned<false>, grid, dim3(256), 0, s, g, P, S, wp, st);
   return hipGetLastError();
 }
 

@@ -206,4 +206,56 @@ __global__ void __launch_bounds__(RED_THREADS) k_speed_final(const double* parti
   }
 }
 
+// minimum and maximum of one plane over the interior: partial[2b], [2b+1]
+__global__ void __launch_bounds__(RED_THREADS) k_range_partial(Geom g, const double* __restrict__ a,
+                                                              double* partial) {
+  double lo = INFINITY, hi = -INFINITY;
+  const int ni = g.imax + 1;
+  const int64_t ncell = (int64_t)ni * (g.jmax + 1);
+  for (int64_t n = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; n < ncell; n += (int64_t)RED_BLOCKS * RED_THREADS) {
+    const int j = (int)(n / ni), i = (int)(n - (int64_t)j * ni);
+    const double x = a[g.at(i, j)];
+    lo = x < lo ? x : lo;
+    hi = x > hi ? x : hi;
+  }
+  __shared__ double rlo[RED_THREADS / 64], rhi[RED_THREADS / 64];
+  lo = -wave_max(-lo);
+  hi = wave_max(hi);
+  if ((threadIdx.x & 63) == 0) { rlo[threadIdx.x >> 6] = lo; rhi[threadIdx.x >> 6] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 1; t < RED_THREADS / 64; t++) {
+      lo = rlo[t] < lo ? rlo[t] : lo;
+      hi = rhi[t] > hi ? rhi[t] : hi;
+    }
+    lo = rlo[0] < lo ? rlo[0] : lo;
+    hi = rhi[0] > hi ? rhi[0] : hi;
+    partial[2 * blockIdx.x] = lo;
+    partial[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+__global__ void __launch_bounds__(RED_THREADS) k_range_final(const double* partial, double* out) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (int b = threadIdx.x; b < RED_BLOCKS; b += RED_THREADS) {
+    lo = partial[2 * b] < lo ? partial[2 * b] : lo;
+    hi = partial[2 * b + 1] > hi ? partial[2 * b + 1] : hi;
+  }
+  __shared__ double rlo[RED_THREADS / 64], rhi[RED_THREADS / 64];
+  lo = -wave_max(-lo);
+  hi = wave_max(hi);
+  if ((threadIdx.x & 63) == 0) { rlo[threadIdx.x >> 6] = lo; rhi[threadIdx.x >> 6] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    lo = rlo[0];
+    hi = rhi[0];
+    for (int t = 1; t < RED_THREADS / 64; t++) {
+      lo = rlo[t] < lo ? rlo[t] : lo;
+      hi = rhi[t] > hi ? rhi[t] : hi;
+    }
+    out[0] = lo;
+    out[1] = hi;
+  }
+}
+
 }  // namespace cueva

@@ -18,28 +18,34 @@ template <typename FT> __device__ inline FT ddy(const Scheme& S, FT a) {
 }
 
 // Right-hand side of one interior point, 17_fluxandsource.f95:442-542:
-// dxf[a] = (Fx(i+1/2) - Fx(i-1/2))/dx and dyf likewise (formed in the flux
-// arithmetic, widened), combined with the FP64 sources of the stage state.
-__device__ inline Rhs rhs_of(const Planes& P, const Scheme& S, int64_t k, const double* dxf, const double* dyf) {
+// dfx[a] = (Fx(i+1/2) - Fx(i-1/2))/dx and dfy likewise, formed in the flux
+// arithmetic FT (held in FT until used: a float difference widens exactly),
+// combined with the FP64 sources of the stage state.
+template <typename FT>
+__device__ inline Rhs rhs_of(const Planes& P, const Scheme& S, int64_t k, const FT* dfx, const FT* dfy) {
 #pragma clang fp contract(off)  // FP64: as the reference build, also inside the FP32 face unit
+#define DX(a) ((double)dfx[a])
+#define DY(a) ((double)dfy[a])
   double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
   double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
   double q = P.s[S_Q][k];
   Rhs o;
-  o.r[0] = ((((-dxf[FX_EXPSI]) - dyf[FY_EYPSI]) - 0.0) + q) - S.kpsi * P.s[S_PSI][k];
-  o.r[1] = (((-dxf[FX_BXPHI]) - dyf[FY_BYPHI]) - 0.0) - S.kphi * P.s[S_PHI][k];
-  o.r[2] = ((-dyf[FY_EZBX]) + 0.0) - dxf[FX_PHIBX];
-  o.r[3] = ((-0.0) + dxf[FX_EZBY]) - dyf[FY_PHIBY];
-  o.r[4] = ((-dxf[FX_EYBZ]) + dyf[FY_EXBZ]) - 0.0;
-  o.r[5] = ((dyf[FY_BZEX] - 0.0) - dxf[FX_PSIEX]) - q * vx;
-  o.r[6] = ((0.0 - dxf[FX_BZEY]) - dyf[FY_PSIEY]) - q * vy;
-  o.r[7] = ((dxf[FX_BYEZ] - dyf[FY_BXEZ]) - 0.0) - q * vz;
-  o.r[8] = ((-dxf[FX_J]) - dyf[FY_J]) - 0.0;
-  o.r[9] = ((-dxf[FX_FD]) - dyf[FY_FD]) - 0.0;
-  o.r[10] = ((-dxf[FX_FTAU]) - dyf[FY_FTAU]) - 0.0;
-  o.r[11] = ((-dxf[FX_FSX]) - dyf[FY_FSX]) - 0.0;
-  o.r[12] = ((-dxf[FX_FSY]) - dyf[FY_FSY]) - 0.0;
-  o.r[13] = ((-dxf[FX_FSZ]) - dyf[FY_FSZ]) - 0.0;
+  o.r[0] = ((((-DX(FX_EXPSI)) - DY(FY_EYPSI)) - 0.0) + q) - S.kpsi * P.s[S_PSI][k];
+  o.r[1] = (((-DX(FX_BXPHI)) - DY(FY_BYPHI)) - 0.0) - S.kphi * P.s[S_PHI][k];
+  o.r[2] = ((-DY(FY_EZBX)) + 0.0) - DX(FX_PHIBX);
+  o.r[3] = ((-0.0) + DX(FX_EZBY)) - DY(FY_PHIBY);
+  o.r[4] = ((-DX(FX_EYBZ)) + DY(FY_EXBZ)) - 0.0;
+  o.r[5] = ((DY(FY_BZEX) - 0.0) - DX(FX_PSIEX)) - q * vx;
+  o.r[6] = ((0.0 - DX(FX_BZEY)) - DY(FY_PSIEY)) - q * vy;
+  o.r[7] = ((DX(FX_BYEZ) - DY(FY_BXEZ)) - 0.0) - q * vz;
+  o.r[8] = ((-DX(FX_J)) - DY(FY_J)) - 0.0;
+  o.r[9] = ((-DX(FX_FD)) - DY(FY_FD)) - 0.0;
+  o.r[10] = ((-DX(FX_FTAU)) - DY(FY_FTAU)) - 0.0;
+  o.r[11] = ((-DX(FX_FSX)) - DY(FY_FSX)) - 0.0;
+  o.r[12] = ((-DX(FX_FSY)) - DY(FY_FSY)) - 0.0;
+  o.r[13] = ((-DX(FX_FSZ)) - DY(FY_FSZ)) - 0.0;
+#undef DX
+#undef DY
   ohm_source(S.sigma, w, vx, vy, vz, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k], P.s[S_BY][k],
              P.s[S_BZ][k], o.r[R_SEX], o.r[R_SEY], o.r[R_SEZ]);
   return o;
@@ -81,6 +87,88 @@ __device__ inline void store_update(Planes& P, int64_t k, const double* nu) {
   P.p[P_Q][k] = nu[A_Q];
   P.c[C_D][k] = nu[A_D]; P.c[C_TAU][k] = nu[A_TAU];
   P.c[C_SX][k] = nu[A_SX]; P.c[C_SY][k] = nu[A_SY]; P.c[C_SZ][k] = nu[A_SZ];
+}
+
+// Stage-l state of one point from its U^n primitives (values) and the
+// earlier stages' right-hand sides at ks: sumint (cumulative, as the
+// reference never resets the sums within a step), varint, electricvarint and
+// flowinit.  out: S_PSI..S_SZ (S_EXA..S_EZA unused), S_EX..S_JZ.
+__device__ inline void stage_point(const Planes& P, const Scheme& S, int l, int64_t ks, bool ghost, double vx,
+                                   double vy, double vz, double ex0, double ey0, double ez0, double bx0, double by0,
+                                   double bz0, double rho, double p, double psi, double phi, double q,
+                                   double* out) {
+  Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, rho, p);
+  double un[A_N] = {psi, phi, bx0, by0, bz0, ex0, ey0, ez0, q, cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
+  double stv[A_N];
+#pragma unroll
+  for (int a = 0; a < A_N; a++) {
+    double acc = 0.0;
+    for (int lp = 1; lp <= l; lp++)
+      for (int m = 0; m < lp; m++) {
+        double f = P.rhs[m][a][ks];
+        if (a >= A_EX && a <= A_EZ)
+          acc = acc + (S.Ab[lp][m] * f - S.Abt[lp][m] * P.rhs[m][R_SEX + (a - A_EX)][ks]);
+        else
+          acc = acc + S.Ab[lp][m] * f;
+      }
+    stv[a] = un[a] + S.dt * acc;
+  }
+  double ex, ey, ez;
+  e_solve(S.Abt[l][l], S.sigma, S.dt, cs.w, vx, vy, vz, stv[A_BX], stv[A_BY], stv[A_BZ], stv[A_EX], stv[A_EY],
+          stv[A_EZ], ex, ey, ez);
+  double jx, jy, jz;
+  ohm_source(S.sigma, cs.w, vx, vy, vz, ex, ey, ez, stv[A_BX], stv[A_BY], stv[A_BZ], jx, jy, jz);
+  jx = jx + stv[A_Q] * vx;
+  jy = jy + stv[A_Q] * vy;
+  jz = jz + stv[A_Q] * vz;
+  if (ghost && S.flux_solver == 6) jx = jy = jz = 0.0;  // 24_boundary_flux_conserved.f95:13
+  out[S_PSI] = stv[A_PSI]; out[S_PHI] = stv[A_PHI];
+  out[S_BX] = stv[A_BX]; out[S_BY] = stv[A_BY]; out[S_BZ] = stv[A_BZ];
+  out[S_EX] = ex; out[S_EY] = ey; out[S_EZ] = ez;
+  out[S_Q] = stv[A_Q]; out[S_D] = stv[A_D]; out[S_TAU] = stv[A_TAU];
+  out[S_SX] = stv[A_SX]; out[S_SY] = stv[A_SY]; out[S_SZ] = stv[A_SZ];
+  out[S_JX] = jx; out[S_JY] = jy; out[S_JZ] = jz;
+}
+
+__device__ inline void store_stage(Planes& P, int64_t k, const double* v) {
+#pragma unroll
+  for (int t = 0; t < S_N; t++)
+    if (t < S_EXA || t > S_EZA) P.s[t][k] = v[t];
+}
+
+// primitive recovery of one point (Cardano, 12_varprimitive.f95) and the
+// store of its primitives and W; false on a failure (reported)
+__device__ inline bool c2p_apply(Planes& P, const Scheme& S, DevStatus* st, int i, int j, int64_t k,
+                                 const C2PIn& in, C2POut& o) {
+  int code = c2p_cardano_full(S.gamma, in, o, S.crmath != 0);
+  if (code) { report(st, code, i, j); return false; }
+  if (o.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
+  if (o.fix_p) atomicAdd(&st->cnt[1], 1ULL);
+  if (o.fix_v) atomicAdd(&st->cnt[2], 1ULL);
+  P.p[P_VX][k] = o.vx; P.p[P_VY][k] = o.vy; P.p[P_VZ][k] = o.vz;
+  P.p[P_RHO][k] = o.rho; P.p[P_P][k] = o.p;
+  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
+  return true;
+}
+
+__device__ inline C2PIn c2p_load(const Planes& P, int64_t k) {
+  C2PIn in;
+  in.d = P.c[C_D][k]; in.tau = P.c[C_TAU][k];
+  in.sx = P.c[C_SX][k]; in.sy = P.c[C_SY][k]; in.sz = P.c[C_SZ][k];
+  in.ex = P.p[P_EX][k]; in.ey = P.p[P_EY][k]; in.ez = P.p[P_EZ][k];
+  in.bx = P.p[P_BX][k]; in.by = P.p[P_BY][k]; in.bz = P.p[P_BZ][k];
+  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
+  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
+  return in;
+}
+
+// The next step's stage-0 state of an interior point from its new primitives
+// (the stage kernel's arithmetic; the ghost band is then a plain copy).
+__device__ inline void stage0_after_c2p(Planes& P, const Scheme& S, int64_t k, const C2PIn& in, const C2POut& o) {
+  double out[S_N];
+  stage_point(P, S, 0, k, false, o.vx, o.vy, o.vz, in.ex, in.ey, in.ez, in.bx, in.by, in.bz, o.rho, o.p,
+              P.p[P_PSI][k], P.p[P_PHI][k], P.p[P_Q][k], out);
+  store_stage(P, k, out);
 }
 
 }  // namespace cueva

@@ -93,6 +93,7 @@ SIGNATURES = {
     "cueva_kh_init": (_i, [_i, _i, _d, _d, _i, _d, _d, _d, ctypes.c_uint64, _P(_d)]),
     "cueva_upload_cons": (_i, [_c, _P(_d)]),
     "cueva_download_plane": (_i, [_c, _i, _i, _P(_d)]),
+    "cueva_plane_range": (_i, [_c, _i, _i, _P(_d)]),
     "cueva_set_profiling": (_i, [_c, _i]),
     "cueva_kernel_times": (_i, [_c, _P(_d), _P(ctypes.c_int64)]),
     "cueva_time_steps": (_i, [_c, _i, _P(_d)]),
@@ -299,6 +300,12 @@ class Solver:
         out = np.zeros(self.shape)
         check(self.lib.cueva_download_plane(self.ctx, group, index, _dp(out)), "download_plane")
         return out
+
+    def plane_range(self, group, index):
+        """(min, max) of one device plane over the interior."""
+        out = np.zeros(2)
+        check(self.lib.cueva_plane_range(self.ctx, group, index, _dp(out)), "plane_range")
+        return float(out[0]), float(out[1])
 
     # -- timing
     def set_profiling(self, on):

@@ -119,3 +119,31 @@ def test_march_learned_matches_face_planes():
     say("fallbacks", out[False][2], out[True][2])
     assert out[True][1] == out[False][1] and out[True][2] == out[False][2]
     assert np.array_equal(out[True][0], out[False][0])
+
+
+@pytest.mark.parametrize("name", ["hll_mp5", "hllc_mp7", "hlle_mp7_bz", "hll_mp9_ssp3"])
+@pytest.mark.parametrize("precision", [cueva.FP64, cueva.MIXED])
+def test_multi_step_call_matches_face_planes(name, precision):
+    """Within one cueva_step call the recovery pass of step h also forms the
+    stage-0 state of step h+1 (and its ghost band by copy); across calls the
+    stage kernel does.  Both must give the face-plane path's fields."""
+    fs, rec, order, sigma, kw = CASES[name]
+    p, prims = cueva.kh_problem(130, 68, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True,
+                                precision=precision, crmath=1 - precision, **kw)
+    steps = 3 if order == 4 else 6
+    say("case", name, "precision", precision, "steps", steps, "in one call")
+    out = {}
+    for march in (False, True):
+        os.environ["CUEVA_MARCH"] = "1" if march else "0"
+        try:
+            with cueva.Solver(p) as s:
+                s.upload(prims)
+                s.step(steps)
+                c = s.step_counters()
+                out[march] = (interior(*s.download(), p), c)
+        finally:
+            os.environ.pop("CUEVA_MARCH", None)
+    nd = ndiff(out[True][0], out[False][0])
+    say("  differing points", sum(nd.values()))
+    assert out[True][1] == out[False][1]
+    assert np.array_equal(out[True][0], out[False][0]), nd
