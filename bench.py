@@ -108,16 +108,19 @@ FORMS = {
     "mixed_learned": dict(precision=1, crmath=0, c2p_mode=1),
 }
 # compulsory FP64 bytes per cell of the marching face kernel (every form: the
-# stage planes are FP64, the FP32 work stays on chip).  Both launches read
-# the 16 stage + 5 primitive planes the stencils touch; the stage launch
-# writes its 17-plane right-hand side; the last launch also reads the earlier
-# stages' right-hand sides (17 each) and the 11 other U^n planes of the
-# update, and writes U^{n+1} (14 planes).
-MARCH_STAGE_BYTES = (16 + 5 + 17) * 8
+# stage planes are FP64, the FP32 work stays on chip).  Every launch reads the
+# 16 stage + 5 primitive planes the stencils touch.  A stage launch also reads
+# the other 9 U^n planes of the next stage's state and writes its 17-plane
+# right-hand side and that 17-plane state; the last launch reads the same 9
+# U^n planes and the earlier stages' right-hand sides (17 each) and writes
+# U^{n+1} (14 planes).
+MARCH_STAGE_BYTES = (16 + 5 + 9 + 17 + 17) * 8
 
 
 def march_last_bytes(lmax):
-    return (16 + 5 + 17 * (lmax - 1) + 11 + 14) * 8
+    return (16 + 5 + 9 + 17 * (lmax - 1) + 14) * 8
+
+
 ACCURACY_STEPS = 100
 
 

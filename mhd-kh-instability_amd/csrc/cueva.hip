@@ -220,7 +220,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     c->march_rows = e && atoi(e) > 0 ? atoi(e) : 0;
   }
 
-  const int nplanes = P_N + C_N + S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N;
+  const int nplanes = P_N + C_N + 2 * S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N;
   c->bytes = (int64_t)nplanes * g.plane * (int64_t)sizeof(double);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CUEVA_E_HIP; }
   if (hipMalloc(&c->base, (size_t)c->bytes) != hipSuccess) {
@@ -244,6 +244,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   for (int k = 0; k < P_N; k++) c->P.p[k] = take();
   for (int k = 0; k < C_N; k++) c->P.c[k] = take();
   for (int k = 0; k < S_N; k++) c->P.s[k] = take();
+  for (int k = 0; k < S_N; k++) c->P.sn[k] = take();
   for (int k = 0; k < FX_N; k++) c->P.fx[k] = take();
   for (int k = 0; k < FY_N; k++) c->P.fy[k] = take();
   // the FP32 face-flux planes of the mixed form reuse the FP64 ones
@@ -261,7 +262,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     const int nptr = (int)(sizeof(Planes) / sizeof(void*));
     for (int k = 0; k < nptr; k++) {
       const char* a = (const char*)ptrs[k];
-      if (a < lo || a + g.plane * (int64_t)sizeof(float) > hi) {
+      if (a < lo || a + g.plane * (int64_t)sizeof(double) > hi) {
         hipFree(c->dst);
         hipFree(c->base);
         hipStreamDestroy(c->stream);
@@ -560,6 +561,16 @@ extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t*
 // ---------------------------------------------------------------------------
 static const dim3 MARCH_BLK(64, 4);
 
+// ghost band of a stage state formed at the interior points only: a copy of
+// the source point's (the state is pointwise in the point's own data), except
+// HLLE's J, which the reference keeps 0 there (24_boundary_flux_conserved.f95:13)
+static int fill_stage_ghosts(cueva_ctx* c) {
+  double* const sp[17] = {c->P.s[S_PSI], c->P.s[S_PHI], c->P.s[S_BX], c->P.s[S_BY], c->P.s[S_BZ], c->P.s[S_Q],
+                          c->P.s[S_D], c->P.s[S_TAU], c->P.s[S_SX], c->P.s[S_SY], c->P.s[S_SZ], c->P.s[S_EX],
+                          c->P.s[S_EY], c->P.s[S_EZ], c->P.s[S_JX], c->P.s[S_JY], c->P.s[S_JZ]};
+  return fill_ghosts(c, sp, 17, c->prm.flux_solver == CUEVA_HLLE ? 3 : 0);
+}
+
 // U^{n+1}, left by the last marching launch in rhs[L][0..13], becomes the
 // state by exchanging plane pointers: psi phi B E q into the primitive
 // planes, D tau S into the conserved ones; the planes they held become the
@@ -631,31 +642,34 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   for (int h = 0; h < nsteps; h++) {
     for (int l = 0; l < c->S.lmax; l++) {
       const bool last = l == c->S.lmax - 1;
-      if (!(l == 0 && s0_ready)) {
+      // the marching path forms stage l >= 1 in the previous stage's face pass
+      const bool have_stage = c->march && (l > 0 || s0_ready);
+      if (!have_stage) {
         dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
                 (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
         LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo);
         if (launch_ok()) return CUEVA_E_HIP;
       }
       if (c->march) {
-        // reconstruction + Riemann + flux difference (+ final update) in one pass
+        // reconstruction + Riemann + flux difference in one pass; then the
+        // next stage's state (interior) or U^{n+1}
         int rc = launch_march(c, l, last);
         if (rc) return rc;
-        if (last) {
-          adopt_update(c, l);
-          // the primitive recovery also forms the next step's stage-0 state
-          // (pointwise, so its ghost band is a copy; HLLE's J ghosts are 0)
-          const bool fuse = h + 1 < nsteps;
-          if ((rc = launch_c2p(c, K_FINAL, fuse))) return rc;
-          if (fuse) {
-            double* const sp[17] = {c->P.s[S_PSI], c->P.s[S_PHI], c->P.s[S_BX], c->P.s[S_BY], c->P.s[S_BZ],
-                                    c->P.s[S_Q], c->P.s[S_D], c->P.s[S_TAU], c->P.s[S_SX], c->P.s[S_SY],
-                                    c->P.s[S_SZ], c->P.s[S_EX], c->P.s[S_EY], c->P.s[S_EZ],
-                                    c->P.s[S_JX], c->P.s[S_JY], c->P.s[S_JZ]};
-            if ((rc = fill_ghosts(c, sp, 17, c->prm.flux_solver == CUEVA_HLLE ? 3 : 0))) return rc;
+        if (!last) {
+          for (int t = 0; t < S_N; t++) {
+            double* x = c->P.s[t];
+            c->P.s[t] = c->P.sn[t];
+            c->P.sn[t] = x;
           }
-          s0_ready = fuse;
+          if ((rc = fill_stage_ghosts(c))) return rc;
+          continue;
         }
+        adopt_update(c, l);
+        // the primitive recovery also forms the next step's stage-0 state
+        const bool fuse = h + 1 < nsteps;
+        if ((rc = launch_c2p(c, K_FINAL, fuse))) return rc;
+        if (fuse && (rc = fill_stage_ghosts(c))) return rc;
+        s0_ready = fuse;
         continue;
       }
       int rc = launch_faces(c);

@@ -25,6 +25,7 @@ struct Planes {
   double* p[P_N];
   double* c[C_N];
   double* s[S_N];
+  double* sn[S_N];  // the next stage's state, written by the marching kernel while s is read
   double* fx[FX_N];
   double* fy[FY_N];
   float* fxf[FX_N];  // the same memory as fx/fy, as FP32 planes (mixed form)

@@ -713,6 +713,7 @@ __device__ inline Cons prim_to_cons(double gam, double vx, double vy, double vz,
 __device__ inline void e_solve(double abt, double sig, double dt, double w, double vx, double vy, double vz,
                                double bx, double by, double bz, double eax, double eay, double eaz, double& ex,
                                double& ey, double& ez) {
+#pragma clang fp contract(off)  // FP64: products and sums rounded separately in every unit
   double a = (abt * sig) * dt;
   double matx = ((sq(w) * a + w * sq(a)) + w) + a;
   double rx = bz * vy - by * vz;

@@ -225,7 +225,9 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
 // and wavefront 0 copies the top row it needs next into its own row after
 // the barrier; a buffer is rewritten two tile steps later, after every
 // reader has passed the barrier in between.
-//   FINAL false: the stage right-hand side -> rhs[l] (17_fluxandsource.f95);
+//   FINAL false: the stage right-hand side -> rhs[l] (17_fluxandsource.f95),
+//                and the next stage's state (sumint, varint, electricvarint,
+//                flowinit) -> sn, interior points;
 //   FINAL true:  sumfin + varfin + electricfield of the last stage -> U^{n+1}
 //                into rhs[l][0..13], for the primitive-recovery pass.
 // ---------------------------------------------------------------------------
@@ -296,6 +298,13 @@ k_faces_march(Geom g, Planes P, Scheme S, int l, int seg, DevStatus* st) {
       if constexpr (!FINAL) {
 #pragma unroll
         for (int t = 0; t < R_N; t++) P.rhs[l][t][k] = r.r[t];
+        // the next stage's state of this point (pointwise in U^n and the
+        // stage right-hand sides, the last of which was just stored)
+        double sv[S_N];
+        stage_point(P, S, l + 1, k, false, P.p[P_VX][k], P.p[P_VY][k], P.p[P_VZ][k], P.p[P_EX][k], P.p[P_EY][k],
+                    P.p[P_EZ][k], P.p[P_BX][k], P.p[P_BY][k], P.p[P_BZ][k], P.p[P_RHO][k], P.p[P_P][k],
+                    P.p[P_PSI][k], P.p[P_PHI][k], P.p[P_Q][k], sv);
+        store_stage(P.sn, k, sv);
       } else {
         double nu[A_N];
         final_update_from(P, S, k, r, nu);

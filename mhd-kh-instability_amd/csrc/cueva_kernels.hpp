@@ -218,7 +218,7 @@ __global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
   stage_point(P, S, l, ks, ghost, P.p[P_VX][ks], P.p[P_VY][ks], P.p[P_VZ][ks], P.p[P_EX][ks], P.p[P_EY][ks],
               P.p[P_EZ][ks], P.p[P_BX][ks], P.p[P_BY][ks], P.p[P_BZ][ks], P.p[P_RHO][ks], P.p[P_P][ks],
               P.p[P_PSI][ks], P.p[P_PHI][ks], P.p[P_Q][ks], out);
-  store_stage(P, k, out);
+  store_stage(P.s, k, out);
 }
 
 // Last stage right-hand side, sumfin, varfin and electricfield at one

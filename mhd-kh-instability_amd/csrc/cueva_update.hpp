@@ -97,6 +97,7 @@ __device__ inline void stage_point(const Planes& P, const Scheme& S, int l, int6
                                    double vy, double vz, double ex0, double ey0, double ez0, double bx0, double by0,
                                    double bz0, double rho, double p, double psi, double phi, double q,
                                    double* out) {
+#pragma clang fp contract(off)  // FP64: as the reference build, also inside the FP32 face unit
   Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, rho, p);
   double un[A_N] = {psi, phi, bx0, by0, bz0, ex0, ey0, ez0, q, cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
   double stv[A_N];
@@ -130,10 +131,10 @@ __device__ inline void stage_point(const Planes& P, const Scheme& S, int l, int6
   out[S_JX] = jx; out[S_JY] = jy; out[S_JZ] = jz;
 }
 
-__device__ inline void store_stage(Planes& P, int64_t k, const double* v) {
+__device__ inline void store_stage(double* const* s, int64_t k, const double* v) {
 #pragma unroll
   for (int t = 0; t < S_N; t++)
-    if (t < S_EXA || t > S_EZA) P.s[t][k] = v[t];
+    if (t < S_EXA || t > S_EZA) s[t][k] = v[t];
 }
 
 // primitive recovery of one point (Cardano, 12_varprimitive.f95) and the
@@ -168,7 +169,7 @@ __device__ inline void stage0_after_c2p(Planes& P, const Scheme& S, int64_t k, c
   double out[S_N];
   stage_point(P, S, 0, k, false, o.vx, o.vy, o.vz, in.ex, in.ey, in.ez, in.bx, in.by, in.bz, o.rho, o.p,
               P.p[P_PSI][k], P.p[P_PHI][k], P.p[P_Q][k], out);
-  store_stage(P, k, out);
+  store_stage(P.s, k, out);
 }
 
 }  // namespace cueva
