@@ -142,8 +142,13 @@ int cueva_electricfield(cueva_ctx *ctx);           /* 20_electricfield.f95     *
  * fallback = cells that fell back to Cardano in the learned mode (else 0). */
 int cueva_c2p(cueva_ctx *ctx, int l_swap, int32_t counters[3], int32_t *fallback);
 
-/* The whole step, fused into few kernels: same results bit for bit as the
- * sequence boundary, startconvar, ..., varprimitivecardano, boundary. */
+/* nsteps whole steps (main_cueva_code.f95:176-332 per step), fused into a
+ * few kernels: same results bit for bit as the sequence boundary,
+ * startconvar, ..., varprimitivecardano, boundary.  Per step: one marching
+ * face launch per stage (reconstruction, Riemann solve, flux difference and
+ * the next stage's state, or U^{n+1} on the last stage), one primitive-
+ * recovery launch (which within one call also forms the next step's first
+ * stage state) and ghost fills. */
 int cueva_step(cueva_ctx *ctx, int nsteps);
 /* Fix-up counters and Cardano fallbacks accumulated by cueva_step since the
  * last call (waits for the device). */
