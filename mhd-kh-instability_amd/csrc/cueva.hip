@@ -625,7 +625,14 @@ static int launch_march(cueva_ctx* c, int l, bool fin) {
   int seg = march_seg(c, fin);
   if (!fn || seg <= 0) return CUEVA_E_HIP;
   const dim3 grid((unsigned)march_strips(c->g), (unsigned)((c->g.jmax + 1 + seg - 1) / seg));
-  void* args[] = {(void*)&c->g, (void*)&c->P, (void*)&c->S, (void*)&l, (void*)&seg, (void*)&c->dst};
+  MarchArgs ma;
+  ma.P = c->P;
+  ma.g = c->g;
+  ma.S = c->S;
+  ma.l = l;
+  ma.seg = seg;
+  ma.st = c->dst;
+  void* args[] = {(void*)&ma};
   prof_begin(c, K_FACES);
   hipError_t e = hipLaunchKernel(fn, grid, MARCH_BLK, args, 0, c->stream);
   prof_end(c);

@@ -5,13 +5,16 @@
 // in its own translation unit).
 #pragma once
 
+#include <stddef.h>
+
 #include "cueva_common.hpp"
 #include "cueva_update.hpp"
 
 #ifndef CUEVA_XCD_FACES
 #define CUEVA_XCD_FACES 1
 #endif
-// minimum waves per SIMD the marching kernel is compiled for (0: no bound)
+// minimum waves per SIMD the marching kernel is compiled for (0: the
+// default of march_min_waves)
 #ifndef CUEVA_MARCH_F32_WAVES
 #define CUEVA_MARCH_F32_WAVES 0
 #endif
@@ -235,20 +238,54 @@ constexpr int MARCH_COLS = 63;
 
 inline int march_strips(const Geom& g) { return (g.imax + 1 + MARCH_COLS - 1) / MARCH_COLS; }
 
+// the marching kernel's one argument (read through the kernel-argument
+// pointer inside the loop, so it must be the first and only argument)
+struct MarchArgs {
+  Planes P;
+  Geom g;
+  Scheme S;
+  int l, seg;
+  DevStatus* st;
+};
+static_assert(offsetof(MarchArgs, P) == 0, "argument block layout");
+
+// Waves per SIMD the marching kernel is built for: in FP32 4 for HLLC (3-5
+// VGPRs spill at that bound), 2 otherwise (HLL would spill heavily, HLLE's
+// LDS allows 3 at most); in FP64 no bound (2 where the registers allow).
+template <typename T, int SOLVER>
+constexpr int march_min_waves() {
+  return sizeof(T) == 8 ? (CUEVA_MARCH_F64_WAVES > 0 ? CUEVA_MARCH_F64_WAVES : 1)
+                        : (CUEVA_MARCH_F32_WAVES > 0 ? CUEVA_MARCH_F32_WAVES : (SOLVER == 7 ? 4 : 2));
+}
+
 template <int ORD, int SOLVER, typename T, bool FINAL>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? (CUEVA_MARCH_F32_WAVES > 0 ? CUEVA_MARCH_F32_WAVES : 1)
-                                                     : (CUEVA_MARCH_F64_WAVES > 0 ? CUEVA_MARCH_F64_WAVES : 1))
-k_faces_march(Geom g, Planes P, Scheme S, int l, int seg, DevStatus* st) {
+__global__ void __launch_bounds__(256, (march_min_waves<T, SOLVER>())) k_faces_march(MarchArgs args) {
   constexpr bool XLDS = SOLVER == 6;            // x solved first, its fluxes through LDS
   __shared__ T fyb[2][4][FY_N][64];             // y-face rows of this and the previous tile step
   __shared__ T fyp[FY_N][64];                   // wavefront 0's row below: previous step's top row
   __shared__ T fxb[XLDS ? 4 : 1][FX_N][64];     // x-face rows (HLLE)
   const int lane = threadIdx.x, w = threadIdx.y;
   const int i = (int)blockIdx.x * MARCH_COLS - 1 + lane;
-  const int jc0 = (int)blockIdx.y * seg;
-  const int jend = min(jc0 + seg - 1, g.jmax);
+  const int jc0 = (int)blockIdx.y * args.seg;
+  const int jend = min(jc0 + args.seg - 1, args.g.jmax);
   int b = 0;
   for (int jt = jc0 - 1; jt <= jend; jt += 4, b ^= 1) {  // uniform over the block: the barrier below
+    // The arguments (plane pointers, geometry, scheme constants) are read
+    // again from the kernel's argument block every tile step rather than
+    // held in scalar registers across the loop, where they do not fit and
+    // spill into vector-register lanes (FP64 HLLC: 156 SGPRs spilled, none
+    // this way; FP32 HLLC: 147 VGPRs against 139).  FP64 HLL keeps them:
+    // re-reading costs it a wave per SIMD.
+    constexpr bool REREAD = sizeof(T) == 4 || SOLVER != 5;
+    const __attribute__((address_space(4))) MarchArgs* ap =
+        (const __attribute__((address_space(4))) MarchArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    if constexpr (REREAD) asm volatile("" : "+s"(ap));
+    const MarchArgs& A = REREAD ? *(const MarchArgs*)ap : args;
+    const Planes& P = A.P;
+    const Geom& g = A.g;
+    const Scheme& S = A.S;
+    const int l = A.l;
+    DevStatus* st = A.st;
     const int j = jt + w;
     const bool face_ok = i <= g.imax && j <= jend;
     T fx[FX_N];
