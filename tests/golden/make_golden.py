@@ -2,7 +2,8 @@
 
 Run in the build container (it needs /root/reference and flang):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # the field fixtures
+    python tests/golden/make_golden.py --stops    # where the reference's runs stop
 
 For every case it builds the reference with oracle/refbuild/build_ref.sh,
 feeds it the seeded Kelvin-Helmholtz data of cueva_kh_init (rotated frame,
@@ -122,5 +123,37 @@ def main(only=None):
         print("wrote", name, "stopped" if stopped else "")
 
 
+# name: (fixture case whose reference build is run, steps to try)
+STOPS = {"kh_cfg1_hll_mp5_128x256_stop": ("kh_cfg1_hll_mp5_128x256", 200)}
+
+
+def record_stops():
+    """Where the reference's own run of a case stops: step, cell, stage and
+    message of its first `stop` (12_varprimitive.f95 prints them)."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    for name, (case, nsteps) in STOPS.items():
+        ni, nj, fs, rec, order, sigma = CASES[case][:6]
+        exe = os.path.join(ROOT, "oracle", "_ref", case, "cueva_ref")
+        p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
+                                    rotated=True, seed=SEED)
+        with tempfile.TemporaryDirectory() as tmp:
+            inp = os.path.join(tmp, "in.bin")
+            prims.astype("<f8").tofile(inp)
+            res = subprocess.run([exe, inp, str(nsteps), os.path.join(tmp, "out")], capture_output=True,
+                                 text=True, cwd=tmp)
+        out = res.stdout + res.stderr
+        line = next(x for x in out.splitlines() if "--->" in x and "det" in x)
+        h, i, j, l, det = line.split("--->")[1].split()
+        meta = {"name": name, "case": case, "steps_tried": nsteps, "init_sha256": field_digest(prims),
+                "stop_step": int(h), "cell": [int(i), int(j)], "stage": int(l), "det": float(det.replace("D", "E")),
+                "message": line.strip(), "fortran_stop": "Fortran STOP" in out}
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(meta, fh, indent=1, sort_keys=True)
+        print("wrote", name, meta["message"])
+
+
 if __name__ == "__main__":
-    main(sys.argv[1:] or None)
+    if sys.argv[1:] == ["--stops"]:
+        record_stops()
+    else:
+        main(sys.argv[1:] or None)

@@ -176,3 +176,24 @@ def test_against_reference_fixture(name):
                     % (max(err.values()), max(spread.values())))
                 assert all(err[f] <= bound[f] for f in FIELDS), {f: (err[f], bound[f]) for f in FIELDS
                                                                  if err[f] > bound[f]}
+
+
+def test_gpu_stops_where_the_reference_stops():
+    """The reference's own cfg1 run (128x256, HLL, sigma = 1e3) stops at step
+    135 with det < 0 in Cardano (tests/golden/kh_cfg1_hll_mp5_128x256_stop.json,
+    recorded from the reference binary); the GPU runs the 134 steps before
+    it in one call and reports status 22 on step 135."""
+    stop = json.load(open(os.path.join(HERE, "golden", "kh_cfg1_hll_mp5_128x256_stop.json")))
+    meta, _ = golden(stop["case"])
+    ni, nj = meta["native_grid"]
+    p, prims = cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"], rec_mp=meta["rec_mp"],
+                                order=meta["order"], rotated=True, seed=meta["seed"])
+    say("cfg1 rotated, %d steps then one more" % (stop["stop_step"] - 1))
+    with cueva.Solver(p) as s:
+        s.upload(prims)
+        s.step(stop["stop_step"] - 1)
+        s.step_counters()
+        s.step(1)
+        with pytest.raises(cueva.CuevaError) as e:
+            s.step_counters()
+        assert e.value.code == 22

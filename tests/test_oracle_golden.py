@@ -98,3 +98,18 @@ def test_oracle_bit_exact_128x256_100_steps(name):
 
 def test_fixture_set_complete():
     assert len(SMALL) >= 5 and len(LARGE) >= 2, (SMALL, LARGE)
+
+
+def test_oracle_stops_where_the_reference_stops():
+    """cfg1 (128x256, HLL, sigma = 1e3) cannot run its 200 steps: the
+    reference's Cardano recovery finds det < 0 at step 135
+    (12_varprimitive.f95:374-425); the oracle stops at the same step with
+    status 22 and runs every step before it."""
+    stop = json.load(open(os.path.join(GOLD, "kh_cfg1_hll_mp5_128x256_stop.json")))
+    meta, _ = load_case(stop["case"])
+    p, prims = problem(meta)
+    o = Oracle(p)
+    o.upload(prims)
+    o.step(stop["stop_step"] - 1)
+    with pytest.raises(RuntimeError, match="status 22"):
+        o.step(1)
