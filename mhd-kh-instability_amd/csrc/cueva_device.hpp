@@ -44,8 +44,14 @@ __host__ __device__ inline int src_index(int i, int n, int periodic) {
 template <typename T> __device__ inline T sq(T x) { return x * x; }
 template <typename T> __device__ inline T pw3(T x) { return x * (x * x); }
 template <typename T> __device__ inline T pw4(T x) { return x * (x * (x * x)); }
+// min/max as the reference build evaluates them (FP64, bitwise path); the
+// FP32 form uses the IEEE-754 2019 minimum/maximum (one v_minimum3_f32 /
+// v_maximum3_f32 instead of a compare, a hazard wait and a select; they
+// differ only in the sign of a zero result and propagate every NaN)
 template <typename T> __device__ inline T mn(T a, T b) { return b < a ? b : a; }
 template <typename T> __device__ inline T mx(T a, T b) { return b > a ? b : a; }
+template <> __device__ inline float mn<float>(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+template <> __device__ inline float mx<float>(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 // max(lo, min(hi, x)) as the reference build evaluates it (a NaN passes)
 template <typename T> __device__ inline T clampf(T x, T lo, T hi) {
   T t = x > hi ? hi : x;
