@@ -5,8 +5,9 @@ conserved state to (log W, log(p/C2)).  Training runs on the device through
 the library's own kernels (cueva.Mlp); this module builds the samples and
 drives it:
 
-* harvest(): (conserved state, W, p) of every interior cell of the code's own
-  Kelvin-Helmholtz runs (the plain FP64 solver);
+* harvest(): (conserved state, W, p) of interior cells of the code's own
+  Kelvin-Helmholtz runs (the plain FP64 solver): the benchmark
+  configurations 1 and 2 and two 256x512 variants;
 * synthetic(): states drawn over W in [1, 10] and p/rho in [1e-3, 1e2] with
   random field directions, made conserved by the same relations as
   06_conserved_var.f95:77-140;
@@ -93,30 +94,52 @@ def from_fields(prims, cons, p):
     return x, targets(W, f["p"], C2)
 
 
-def harvest(n_target, seed=2, log=print):
-    """Samples from the code's own KH runs: snapshots of 256x512 runs at
-    v0 = 0.5 and 0.9 through the linear and early nonlinear phase."""
+# harvest runs: (label, problem factory, steps between snapshots, snapshots);
+# the last one (the largest grid) supplies whatever the others fall short of
+def _runs():
     from . import problems
+    return [
+        ("cfg1 128x256 HLL sigma=1e3", lambda: problems.config("cfg1"), 15, 8),
+        ("256x512 HLLC v0=0.5 Bz=0.1", lambda: problems.kh_problem(256, 512, sigma=1.0e6, flux_solver=abi.HLLC,
+                                                                    v0=0.5, bx0=0.01, bz0=0.1), 5, 7),
+        ("256x512 HLLC v0=0.9", lambda: problems.kh_problem(256, 512, sigma=1.0e6, flux_solver=abi.HLLC,
+                                                             v0=0.9, bx0=0.001), 10, 8),
+        ("cfg2 2048x4096 HLLC sigma=1e6", lambda: problems.config("cfg2"), 40, 7),
+    ]
+
+
+def harvest(n_target, seed=2, log=print):
+    """Samples from the code's own KH runs (plain FP64): snapshots of the
+    benchmark configurations 1 and 2 through the phase they can be run
+    (the reference's own solvers stop them at steps 135 and 290), and of
+    256x512 runs with a guide field and at v0 = 0.9, until they stop."""
     rng = np.random.default_rng(seed)
     xs, ys = [], []
-    runs = [dict(v0=0.5, bx0=0.01, bz0=0.0), dict(v0=0.5, bx0=0.01, bz0=0.1), dict(v0=0.9, bx0=0.001, bz0=0.0)]
-    per = max(1, n_target // (len(runs) * 8))
-    for kw in runs:
-        prm, prims = problems.kh_problem(256, 512, sigma=1.0e6, flux_solver=abi.HLLC, **kw)
+    runs = _runs()
+    total = 0
+    for r, (label, make, every, snaps) in enumerate(runs):
+        prm, prims = make()
+        left = len(runs) - r
+        per = max(1, (n_target - total) // (left * snaps))
+        got = 0
         with abi.Solver(prm) as s:
             s.upload(prims)
-            for snap in range(8):
+            del prims
+            for snap in range(snaps):
                 try:
-                    s.step(40)
+                    s.step(every)
                     s.step_counters()
                 except abi.CuevaError as e:  # a run that stops still gave its earlier snapshots
-                    log("harvest: run %s stops (%s)" % (kw, e))
+                    log("harvest: %s stops (%s)" % (label, e))
                     break
                 x, y = from_fields(*s.download(), prm)
                 ok = np.all(np.isfinite(x), axis=1) & np.all(np.isfinite(y), axis=1)
                 idx = rng.choice(np.flatnonzero(ok), size=min(per, int(ok.sum())), replace=False)
                 xs.append(x[idx])
                 ys.append(y[idx])
+                got += idx.size
+        total += got
+        log("harvest: %s: %d samples" % (label, got))
     return np.concatenate(xs), np.concatenate(ys)
 
 
