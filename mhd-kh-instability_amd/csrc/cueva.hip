@@ -559,7 +559,9 @@ extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t*
 // ---------------------------------------------------------------------------
 // fused step
 // ---------------------------------------------------------------------------
-static const dim3 MARCH_BLK(64, 4);
+static int march_waves(const cueva_ctx* c) {
+  return march_block_waves(c->prm.precision == CUEVA_MIXED, c->prm.flux_solver);
+}
 
 // ghost band of a stage state formed at the interior points only: a copy of
 // the source point's (the state is pointwise in the point's own data), except
@@ -596,27 +598,28 @@ static const void* march_kernel_of(cueva_ctx* c, bool fin) {
 }
 
 // Rows per segment of a marching launch: enough segments that the blocks
-// fill the resident slots of every CU march_rounds times, seg = 4m - 1 (the
-// segment's face rows, seg + 1, fill whole tile steps).
+// fill the resident slots of every CU march_rounds times, seg = MW m - 1 (the
+// segment's face rows, seg + 1, fill whole tile steps of MW rows).
 static int march_seg(cueva_ctx* c, bool fin) {
   int& seg = c->march_seg[fin ? 1 : 0];
   if (seg) return seg;
+  const int mw = march_waves(c);
   if (c->march_rows > 0) {
-    seg = (int)(4 * ((c->march_rows + 1 + 3) / 4) - 1);
+    seg = (int)(mw * ((c->march_rows + 1 + mw - 1) / mw) - 1);
     return seg;
   }
   int per_cu = 0, cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel_of(c, fin), 256, 0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel_of(c, fin), 64 * mw, 0) != hipSuccess)
     return 0;
   const int64_t slots = (int64_t)(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
   const int strips = march_strips(c->g), rows = c->g.jmax + 1;
   int64_t want = slots * c->march_rounds / strips;
   if (want < 1) want = 1;
   const int64_t per = (rows + want - 1) / want;
-  const int64_t m = (per + 1 + 3) / 4;
-  seg = (int)(4 * m - 1);
+  const int64_t m = (per + 1 + mw - 1) / mw;
+  seg = (int)(mw * m - 1);
   return seg;
 }
 
@@ -634,7 +637,7 @@ static int launch_march(cueva_ctx* c, int l, bool fin) {
   ma.st = c->dst;
   void* args[] = {(void*)&ma};
   prof_begin(c, K_FACES);
-  hipError_t e = hipLaunchKernel(fn, grid, MARCH_BLK, args, 0, c->stream);
+  hipError_t e = hipLaunchKernel(fn, grid, dim3(64, (unsigned)march_waves(c)), args, 0, c->stream);
   prof_end(c);
   return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
 }

@@ -213,10 +213,10 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
 
 // ---------------------------------------------------------------------------
 // Marching face kernel: reconstruction + Riemann solve + flux difference in
-// one pass.  A block of 4 wavefronts owns a strip of MARCH_COLS cell columns
-// and a segment of `seg` cell rows (seg = 4m - 1).  Lane L of every wavefront
-// computes the face pair of column i0 - 1 + L; per tile step the 4 wavefronts
-// take 4 consecutive face rows.  Fx(i-1/2) is the left lane's x flux (a
+// one pass.  A block of MW wavefronts owns a strip of MARCH_COLS cell
+// columns and a segment of `seg` cell rows (seg = MW m - 1).  Lane L of every
+// wavefront computes the face pair of column i0 - 1 + L; per tile step the
+// MW wavefronts take MW consecutive face rows.  Fx(i-1/2) is the left lane's x flux (a
 // cross-lane shuffle of the registers the x solve leaves; HLLE, which must
 // solve x first, goes through LDS); Fy(j-1/2) is the LDS row of the
 // wavefront below, or for the lowest wavefront the previous tile step's top
@@ -258,18 +258,31 @@ constexpr int march_min_waves() {
                         : (CUEVA_MARCH_F32_WAVES > 0 ? CUEVA_MARCH_F32_WAVES : (SOLVER == 7 ? 4 : 2));
 }
 
+// Wavefronts per block = face rows per tile step.  4 everywhere by default;
+// FP32 HLL/HLLC may take 8 (a tile step then re-reads 5 halo rows per 8 face
+// rows instead of per 4; two 61 KB blocks per CU still give 4 waves per SIMD)
+// but measured the same step time at 2048x4096, so the smaller block stays.
+#ifndef CUEVA_MARCH_F32_BLOCK_WAVES
+#define CUEVA_MARCH_F32_BLOCK_WAVES 4
+#endif
+__host__ __device__ constexpr int march_block_waves(bool fp32, int solver) {
+  return fp32 && solver != 6 ? CUEVA_MARCH_F32_BLOCK_WAVES : 4;
+}
+
 template <int ORD, int SOLVER, typename T, bool FINAL>
-__global__ void __launch_bounds__(256, (march_min_waves<T, SOLVER>())) k_faces_march(MarchArgs args) {
+__global__ void __launch_bounds__(64 * march_block_waves(sizeof(T) == 4, SOLVER), (march_min_waves<T, SOLVER>()))
+k_faces_march(MarchArgs args) {
+  constexpr int MW = march_block_waves(sizeof(T) == 4, SOLVER);
   constexpr bool XLDS = SOLVER == 6;            // x solved first, its fluxes through LDS
-  __shared__ T fyb[2][4][FY_N][64];             // y-face rows of this and the previous tile step
+  __shared__ T fyb[2][MW][FY_N][64];            // y-face rows of this and the previous tile step
   __shared__ T fyp[FY_N][64];                   // wavefront 0's row below: previous step's top row
-  __shared__ T fxb[XLDS ? 4 : 1][FX_N][64];     // x-face rows (HLLE)
+  __shared__ T fxb[XLDS ? MW : 1][FX_N][64];    // x-face rows (HLLE)
   const int lane = threadIdx.x, w = threadIdx.y;
   const int i = (int)blockIdx.x * MARCH_COLS - 1 + lane;
   const int jc0 = (int)blockIdx.y * args.seg;
   const int jend = min(jc0 + args.seg - 1, args.g.jmax);
   int b = 0;
-  for (int jt = jc0 - 1; jt <= jend; jt += 4, b ^= 1) {  // uniform over the block: the barrier below
+  for (int jt = jc0 - 1; jt <= jend; jt += MW, b ^= 1) {  // uniform over the block: the barrier below
     // The arguments (plane pointers, geometry, scheme constants) are read
     // again from the kernel's argument block every tile step rather than
     // held in scalar registers across the loop, where they do not fit and
@@ -351,7 +364,7 @@ __global__ void __launch_bounds__(256, (march_min_waves<T, SOLVER>())) k_faces_m
     }
     if (w == 0) {
 #pragma unroll
-      for (int a = 0; a < FY_N; a++) fyp[a][lane] = fyb[b][3][a][lane];
+      for (int a = 0; a < FY_N; a++) fyp[a][lane] = fyb[b][MW - 1][a][lane];
     }
     if constexpr (XLDS) __builtin_amdgcn_wave_barrier();  // this step's fxb reads before the next step's writes
   }
