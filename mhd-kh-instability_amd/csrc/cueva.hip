@@ -340,12 +340,12 @@ static dim3 interior_grid(const Geom& g, dim3 blk) {
 static int launch_ok() { return hipGetLastError() == hipSuccess ? CUEVA_OK : CUEVA_E_HIP; }
 
 // copy every ghost point of the planes from its interior source point; the
-// last nzero planes (at most 20) are set to 0 there instead
+// last nzero planes (at most 32) are set to 0 there instead
 static int fill_ghosts(cueva_ctx* c, double* const* planes, int n, int nzero = 0) {
   PlaneList pl;
   memset(&pl, 0, sizeof(pl));
-  for (int start = 0; start < n; start += 20) {
-    pl.n = n - start < 20 ? n - start : 20;
+  for (int start = 0; start < n; start += 32) {
+    pl.n = n - start < 32 ? n - start : 32;
     pl.nzero = start + pl.n == n ? nzero : 0;
     for (int k = 0; k < pl.n; k++) pl.a[k] = planes[start + k];
     int64_t cnt = ghost_count(c->g);
@@ -565,12 +565,17 @@ static int march_waves(const cueva_ctx* c) {
 
 // ghost band of a stage state formed at the interior points only: a copy of
 // the source point's (the state is pointwise in the point's own data), except
-// HLLE's J, which the reference keeps 0 there (24_boundary_flux_conserved.f95:13)
-static int fill_stage_ghosts(cueva_ctx* c) {
-  double* const sp[17] = {c->P.s[S_PSI], c->P.s[S_PHI], c->P.s[S_BX], c->P.s[S_BY], c->P.s[S_BZ], c->P.s[S_Q],
-                          c->P.s[S_D], c->P.s[S_TAU], c->P.s[S_SX], c->P.s[S_SY], c->P.s[S_SZ], c->P.s[S_EX],
-                          c->P.s[S_EY], c->P.s[S_EZ], c->P.s[S_JX], c->P.s[S_JY], c->P.s[S_JZ]};
-  return fill_ghosts(c, sp, 17, c->prm.flux_solver == CUEVA_HLLE ? 3 : 0);
+// HLLE's J, which the reference keeps 0 there (24_boundary_flux_conserved.f95:13);
+// with_prims: the primitive planes' ghosts in the same launch
+static int fill_stage_ghosts(cueva_ctx* c, bool with_prims = false) {
+  double* list[P_N + 17];
+  int n = 0;
+  if (with_prims)
+    for (int k = 0; k < P_N; k++) list[n++] = c->P.p[k];
+  const int st[17] = {S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ,
+                      S_EX, S_EY, S_EZ, S_JX, S_JY, S_JZ};
+  for (int k = 0; k < 17; k++) list[n++] = c->P.s[st[k]];
+  return fill_ghosts(c, list, n, c->prm.flux_solver == CUEVA_HLLE ? 3 : 0);
 }
 
 // U^{n+1}, left by the last marching launch in rhs[L][0..13], becomes the
@@ -678,7 +683,8 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
         // the primitive recovery also forms the next step's stage-0 state
         const bool fuse = h + 1 < nsteps;
         if ((rc = launch_c2p(c, K_FINAL, fuse))) return rc;
-        if (fuse && (rc = fill_stage_ghosts(c))) return rc;
+        // ghosts of the new primitives (and of the next stage-1 state)
+        if ((rc = fuse ? fill_stage_ghosts(c, true) : fill_ghosts(c, c->P.p, P_N))) return rc;
         s0_ready = fuse;
         continue;
       }
@@ -705,8 +711,10 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
       }
       if (launch_ok()) return CUEVA_E_HIP;
     }
-    int rc = fill_ghosts(c, c->P.p, P_N);
-    if (rc) return rc;
+    if (!c->march) {
+      int rc = fill_ghosts(c, c->P.p, P_N);
+      if (rc) return rc;
+    }
   }
   return CUEVA_OK;
 }

@@ -265,8 +265,11 @@ constexpr int march_min_waves() {
 #ifndef CUEVA_MARCH_F32_BLOCK_WAVES
 #define CUEVA_MARCH_F32_BLOCK_WAVES 4
 #endif
+#ifndef CUEVA_MARCH_F64_BLOCK_WAVES
+#define CUEVA_MARCH_F64_BLOCK_WAVES 4
+#endif
 __host__ __device__ constexpr int march_block_waves(bool fp32, int solver) {
-  return fp32 && solver != 6 ? CUEVA_MARCH_F32_BLOCK_WAVES : 4;
+  return solver == 6 ? 4 : (fp32 ? CUEVA_MARCH_F32_BLOCK_WAVES : CUEVA_MARCH_F64_BLOCK_WAVES);
 }
 
 template <int ORD, int SOLVER, typename T, bool FINAL>

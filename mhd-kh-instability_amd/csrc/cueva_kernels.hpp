@@ -19,7 +19,7 @@ namespace cueva {
 // ghost fill: every ghost point copies the interior point of the index map
 // ---------------------------------------------------------------------------
 struct PlaneList {
-  double* a[20];
+  double* a[32];
   int n;
   int nzero;  // the last nzero planes get 0 at every ghost point instead of a copy
 };
