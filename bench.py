@@ -198,20 +198,25 @@ def growth(cfg, form, packed, nsteps=GROWTH_STEPS, every=10):
                 s.set_c2p_params(packed)
             s.upload(prims)
             del prims
-            res[f] = diagnostics.amplitude_series(s, nsteps, every, dt)
-    t0, a0, stop0 = res["plain_fp64"]
+            res[f] = diagnostics.kh_series(s, nsteps, every, dt, integral=True)
+    t0, a0, _, stop0 = res["plain_fp64"]
     win = diagnostics.growth_window(a0)
-    out = {"steps": nsteps, "every": every, "amplitude": "(max vy - min vy)/2 over the interior"}
+    out = {"steps": nsteps, "every": every, "amplitude": "(max vy - min vy)/2 over the interior",
+           "integral": "half the log slope of sum(vy^2 dx dy), the reference's measure"}
     if win is None:
         out["note"] = "amplitude did not reach 1.5x its initial value"
         return out
     out["window_t"] = [float(t0[win[0]]), float(t0[win[1] - 1])]
-    for f, (t, a, stop) in res.items():
-        out[f] = {"rate": diagnostics.growth_rate(t, a, win) if len(a) >= win[1] else None,
+    for f, (t, a, vi, stop) in res.items():
+        whole = len(a) >= win[1]
+        out[f] = {"rate": diagnostics.growth_rate(t, a, win) if whole else None,
+                  "rate_integral": diagnostics.growth_rate_of_integral(t, vi, win) if whole else None,
                   "A_first": float(a[0]), "A_last": float(a[-1]),
                   "stopped": None if stop is None else [stop[0], str(stop[1])]}
     if out[form]["rate"] is not None:
         out["rel_diff_vs_plain_fp64"] = out[form]["rate"] / out["plain_fp64"]["rate"] - 1.0
+        out["rel_diff_vs_plain_fp64_integral"] = (out[form]["rate_integral"] /
+                                                  out["plain_fp64"]["rate_integral"] - 1.0)
     return out
 
 

@@ -9,3 +9,4 @@ from .abi import (  # noqa: F401
     plane_shape,
 )
 from .problems import kh_params, kh_problem  # noqa: F401
+from .restart import read_restart, read_restart_text, restart_file_name  # noqa: F401

@@ -77,7 +77,7 @@ def build_fortran(verbose=False):
     out = os.path.join(LIB, "cueva_gpu_driver")
     bdir = os.path.join(LIB, "_fbuild")
     os.makedirs(bdir, exist_ok=True)
-    srcs = ["cueva_gpu.f95", "parameters_kh.f95", "cueva_shims.f95", "main_cueva_gpu.f95"]
+    srcs = ["cueva_gpu.f95", "parameters_kh.f95", "cueva_shims.f95", "cueva_io.f95", "main_cueva_gpu.f95"]
     objs = []
     for s in srcs:
         o = os.path.join(bdir, s.replace(".f95", ".o"))

@@ -101,9 +101,9 @@ contains
 
   ! the global block of lector (22_writer.f95:818-991): one line per call
   ! with h, t and the 37 diagnostics in the reference's unit order 80..217
-  subroutine lector_globals(unit)
+  subroutine lector_globals(unit, g)
     integer, intent(in) :: unit
-    real(c_double) :: g(37)
+    real(c_double), intent(out) :: g(37)
     call check(cueva_reduce_globals(ctx, g), "lector")
     write(unit, '(i8, 38es25.16e3)') h, t_advance, g
   end subroutine lector_globals

@@ -2,28 +2,43 @@
 ! driving the gfx950 library through the reference-named shims.
 !
 !   main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED
-!                  V0 BX0 BZ0 SEED HGLB OUTPREFIX
+!                  V0 BX0 BZ0 SEED HGLB OUTPREFIX [HLOC [RESTART_IN [HINI]]]
 !
 ! IMAX x JMAX is the native KH grid (x periodic, y outflow); ROTATED = 1 runs
 ! it in the frame x' = y, y' = -x (the reference's BOUND 6).  Writes
 ! OUTPREFIX.bin (int32 fix-up counters of the last step, then the 14
 ! primitive planes in restart order and D tau Sx Sy Sz, each
-! (imax+13)*(jmax+13) doubles, x fastest) and, every HGLB steps,
-! a line of global diagnostics to OUTPREFIX_globals.dat.
+! (imax+13)*(jmax+13) doubles, x fastest) and, every HGLB steps (and at the
+! first), a line of global diagnostics to OUTPREFIX_globals.dat, and the
+! reference's data_global/2d_data_*.dat files when that directory exists.
+!
+! HLOC > 0: every HLOC steps, the reference's restart file
+!   data_restart/all_variables_NNNN.dat, NNNN = h/HLOC (stops, as lector
+!   does, when the file exists or cannot be created).
+! RESTART_IN: start from that restart file (the reference's RESTART = 1 run,
+!   main_cueva_code.f95:22-28) instead of the seeded KH data ("-": seeded);
+!   V0 BX0 BZ0 SEED are then unused.
+! HINI: number of the first step, as the reference's hini (default 1): steps
+!   HINI .. HINI+NSTEPS-1 run, from t = (HINI-1) Delt.
 program main_cueva_gpu
   use iso_c_binding
   use cueva_gpu
   use parameters_kh
   use cueva_shims
+  use cueva_io
   implicit none
-  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios
+  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini
   integer(c_int64_t) :: seed
-  real(c_double) :: sigma, v0, bx0, bz0, dv(5)
-  character(len=512) :: arg, prefix
+  real(c_double) :: sigma, v0, bx0, bz0, dv(5), g(37)
+  character(len=512) :: arg, prefix, restart_in
+  character(len=80) :: fname
+  logical :: gfiles
   real(c_double), allocatable :: prims(:), cons(:)
 
-  if (command_argument_count() /= 14) then
-    write(*,*) "usage: main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED V0 BX0 BZ0 SEED HGLB OUTPREFIX"
+  nargs = command_argument_count()
+  if (nargs < 14 .or. nargs > 17) then
+    write(*,*) "usage: main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED V0 BX0 BZ0 SEED HGLB ", &
+               "OUTPREFIX [HLOC [RESTART_IN [HINI]]]"
     stop 2
   end if
   call get_command_argument(1, arg); read(arg, *) ni
@@ -40,6 +55,16 @@ program main_cueva_gpu
   call get_command_argument(12, arg); read(arg, *) seed
   call get_command_argument(13, arg); read(arg, *) hglb
   call get_command_argument(14, prefix)
+  hloc = 0
+  restart_in = "-"
+  hini = 1
+  if (nargs >= 15) then
+    call get_command_argument(15, arg); read(arg, *) hloc
+  end if
+  if (nargs >= 16) call get_command_argument(16, restart_in)
+  if (nargs >= 17) then
+    call get_command_argument(17, arg); read(arg, *) hini
+  end if
 
   call check(cueva_params_default(prm), "params_default")
   if (rotated == 1) then
@@ -60,14 +85,26 @@ program main_cueva_gpu
 
   nplane = (prm%imax + 13) * (prm%jmax + 13)
   allocate(prims(14 * nplane), cons(5 * nplane))
-  call check(cueva_kh_init(prm%imax, prm%jmax, prm%longx, prm%longy, int(rotated, c_int), v0, bx0, bz0, &
-                           seed, prims), "kh_init")
+  if (trim(restart_in) == "-") then
+    call check(cueva_kh_init(prm%imax, prm%jmax, prm%longx, prm%longy, int(rotated, c_int), v0, bx0, bz0, &
+                             seed, prims), "kh_init")
+  else
+    call read_restart(trim(restart_in), int(prm%imax), int(prm%jmax), prims, ios)   ! init_restart
+    if (ios /= 0) then
+      write(*,*) "STOP: cannot read the restart file ", trim(restart_in)
+      stop 3
+    end if
+  end if
   call check(cueva_create(prm, ctx), "create")
-  call check(cueva_upload_prims(ctx, prims), "upload")      ! replaces init_restart
-  if (hglb > 0) open(unit=80, file=trim(prefix)//"_globals.dat", status="replace", action="write")
+  call check(cueva_upload_prims(ctx, prims), "upload")
+  gfiles = .false.
+  if (hglb > 0) then
+    open(unit=70, file=trim(prefix)//"_globals.dat", status="replace", action="write")
+    call open_globals(gfiles)
+  end if
 
-  t_advance = 0.d0
-  do h = 1, hmax
+  t_advance = (hini - 1) * Delt
+  do h = hini, hini + hmax - 1
     call boundary
     call startconvar
     call cleanvarint
@@ -102,8 +139,23 @@ program main_cueva_gpu
     call varprimitivecardano
     call boundary
     t_advance = t_advance + Delt
+    ! lector (22_writer.f95): global diagnostics, then the restart dump
     if (hglb > 0) then
-      if (h == 1 .or. mod(h, hglb) == 0) call lector_globals(80)
+      if (h == 1 .or. mod(h, hglb) == 0) then
+        call lector_globals(70, g)
+        if (gfiles) call write_globals(h, t_advance, g)
+      end if
+    end if
+    if (hloc > 0) then
+      if (mod(h, hloc) == 0) then
+        call check(cueva_download(ctx, prims, cons), "download")
+        fname = restart_file_name(h / hloc)
+        call write_restart(trim(fname), int(prm%imax), int(prm%jmax), prims, ios)
+        if (ios /= 0) then
+          write(*, '("Can''t open file ", a, " for writing.")') trim(fname)
+          stop
+        end if
+      end if
     end if
   end do
 
@@ -117,7 +169,8 @@ program main_cueva_gpu
   write(90) prims
   write(90) cons
   close(90)
-  if (hglb > 0) close(80)
+  if (hglb > 0) close(70)
+  if (gfiles) call close_globals(37)
   call check(cueva_destroy(ctx), "destroy")
   write(*,'(a,i0,a,3i6)') "done ", hmax, " steps; fix-ups of the last step: ", contador_r, contador_p, contador_v
 end program main_cueva_gpu

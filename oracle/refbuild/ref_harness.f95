@@ -7,11 +7,16 @@
 ! (main_cueva_code.f95:47-356) with I/O and the commented-out adaptive dt
 ! left out, and replaces the RESTART read (03_restart_primitive.f95:50-61)
 ! by a binary read of the same 14 fields, so that seeded initial data reach
-! the reference bit for bit.
+! the reference bit for bit (with -restart it makes the reference's own
+! text read instead).
 !
 ! usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug] [-globals]
+!                  [-restart] [-lector H]
 !   INPUT      stream file: 14 arrays (-6:imax+6,-6:jmax+6) of real(8), x fastest,
 !              in the restart order psi phi Bx By Bz Vx Vy Vz Ex Ey Ez q rho p
+!   -restart   INPUT is instead a restart text file, read by the reference's
+!              own init_restart (03_restart_primitive.f95:50-61) on unit 222
+!              and followed by boundary, as main_cueva_code.f95:22-28 does
 !   OUTPREFIX  each dump step h writes OUTPREFIX_hNNNNN.bin:
 !              int32 counters (contador_r, contador_p, contador_v), then 19 arrays:
 !              the 14 fields above followed by D tau Sx Sy Sz
@@ -22,6 +27,10 @@
 !              which writes its global diagnostics (22_writer.f95:818-991) to
 !              the units 80..217 (unconnected: files fort.80 ... in the
 !              working directory)
+!   -lector H  the same with h = H; H = 0 also writes the reference's restart
+!              file data_restart/all_variables_0000.dat (22_writer.f95:1116-1172)
+!              and its local dumps (the directories data_restart, data_print
+!              and data_local must exist in the working directory)
 
 program cueva_ref_harness
 
@@ -33,13 +42,13 @@ program cueva_ref_harness
   implicit none
 
   character(len=1024) :: arg, infile, prefix, fname
-  integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out
+  integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out, hlector
   integer, allocatable :: dumps(:)
-  logical             :: debug, globals
+  logical             :: debug, globals, from_text
 
   nargs = command_argument_count()
   if (nargs < 3) then
-     write(*,*) "usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug]"
+     write(*,*) "usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug] [-globals] [-restart] [-lector H]"
      stop 2
   end if
   call get_command_argument(1, infile)
@@ -50,29 +59,46 @@ program cueva_ref_harness
   ndump = 0
   debug = .false.
   globals = .false.
-  do iarg = 4, nargs
+  from_text = .false.
+  hlector = 1
+  iarg = 4
+  do while (iarg <= nargs)
      call get_command_argument(iarg, arg)
      if (trim(arg) == "-debug") then
         debug = .true.
      else if (trim(arg) == "-globals") then
         globals = .true.
+     else if (trim(arg) == "-restart") then
+        from_text = .true.
+     else if (trim(arg) == "-lector") then
+        globals = .true.
+        iarg = iarg + 1
+        call get_command_argument(iarg, arg)
+        read(arg,*) hlector
      else
         ndump = ndump + 1
         read(arg,*) dumps(ndump)
      end if
+     iarg = iarg + 1
   end do
 
   open(301, file=trim(prefix)//"_fixup_rho.txt", action="write")
   open(302, file=trim(prefix)//"_fixup_p.txt",   action="write")
   open(303, file=trim(prefix)//"_fixup_v.txt",   action="write")
 
-  open(newunit=u_in, file=trim(infile), access="stream", form="unformatted", status="old", action="read")
-  read(u_in) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
-             Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
-             q(:,:,1), rho(:,:,1), p(:,:,1)
-  close(u_in)
-
   t_init = 0
+  if (from_text) then
+     open(unit=222, file=trim(infile), status="old", access="sequential", action="read")
+     call init_restart
+     close(222)
+  else
+     open(newunit=u_in, file=trim(infile), access="stream", form="unformatted", status="old", action="read")
+     read(u_in) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
+                Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
+                q(:,:,1), rho(:,:,1), p(:,:,1)
+     close(u_in)
+  end if
+
   call boundary
   t_advance = (hini-1) * Delt
 
@@ -137,7 +163,7 @@ program cueva_ref_harness
   end do
 
   if (globals) then
-     h = 1
+     h = hlector
      call lector
   end if
 

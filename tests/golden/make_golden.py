@@ -4,6 +4,7 @@ Run in the build container (it needs /root/reference and flang):
 
     python tests/golden/make_golden.py            # the field fixtures
     python tests/golden/make_golden.py --stops    # where the reference's runs stop
+    python tests/golden/make_golden.py --restarts # the reference's restart files
 
 For every case it builds the reference with oracle/refbuild/build_ref.sh,
 feeds it the seeded Kelvin-Helmholtz data of cueva_kh_init (rotated frame,
@@ -152,8 +153,68 @@ def record_stops():
         print("wrote", name, meta["message"])
 
 
+# name: (fixture case, steps before the restart dump, steps after it)
+RESTARTS = {"kh_hllc_mp5_32x64_restart": ("kh_hllc_mp5_32x64", 4, 6),
+            "kh_hlle_mp7_bz_32x64_restart": ("kh_hlle_mp7_bz_32x64", 4, 6)}
+
+
+def _lector_run(exe, inp, nsteps, cwd, extra=()):
+    """Run the reference for nsteps, then its lector with h = 0, which writes
+    its restart file data_restart/all_variables_0000.dat; returns the file's
+    bytes and the binary dump of the last step."""
+    for d in ("data_restart", "data_print", "data_local"):
+        os.makedirs(os.path.join(cwd, d))
+    res = subprocess.run([exe, inp, str(nsteps), os.path.join(cwd, "out"), str(nsteps)] + list(extra) +
+                         ["-lector", "0"], capture_output=True, text=True, cwd=cwd)
+    assert res.returncode == 0 and "Fortran STOP" not in res.stdout + res.stderr, res.stdout + res.stderr
+    text = open(os.path.join(cwd, "data_restart", "all_variables_0000.dat"), "rb").read()
+    dump = open(os.path.join(cwd, "out_h%05d.bin" % nsteps), "rb").read()
+    return text, dump
+
+
+def record_restarts():
+    """The reference's own restart files (22_writer.f95:1116-1172): their
+    SHA-256 after n1 and n1 + n2 steps, with the checks that its text gives
+    back every double exactly and that its own resumed run (init_restart,
+    03_restart_primitive.f95:50-61) is bitwise its uninterrupted one."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    for name, (case, n1, n2) in RESTARTS.items():
+        ni, nj, fs, rec, order, sigma = CASES[case][:6]
+        init_kw = CASES[case][10] if len(CASES[case]) > 10 else {}
+        exe = os.path.join(ROOT, "oracle", "_ref", case, "cueva_ref")
+        p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
+                                    rotated=True, seed=SEED, **init_kw)
+        shape = (p.jmax + 13, p.imax + 13)
+        with tempfile.TemporaryDirectory() as tmp:
+            inp = os.path.join(tmp, "in.bin")
+            prims.astype("<f8").tofile(inp)
+            dirs = [os.path.join(tmp, d) for d in ("first", "resumed", "whole")]
+            for d in dirs:
+                os.makedirs(d)
+            t1, d1 = _lector_run(exe, inp, n1, dirs[0])
+            r1 = os.path.join(tmp, "restart_first.dat")
+            open(r1, "wb").write(t1)
+            t2, d2 = _lector_run(exe, r1, n2, dirs[1], ["-restart"])
+            t3, d3 = _lector_run(exe, inp, n1 + n2, dirs[2])
+        assert d2 == d3, "%s: the reference's resumed run differs from its uninterrupted run" % name
+        assert t2 == t3
+        parsed = cueva.read_restart_text(t1.decode(), p.imax, p.jmax)
+        binary = np.frombuffer(d1[12:], dtype="<f8")[:14 * shape[0] * shape[1]].reshape(14, *shape)
+        assert np.array_equal(parsed, binary), "%s: restart text does not give back the doubles" % name
+        meta = {"name": name, "case": case, "init_sha256": field_digest(prims), "steps": [n1, n2],
+                "restart_sha256": {str(n1): hashlib.sha256(t1).hexdigest(),
+                                   str(n1 + n2): hashlib.sha256(t3).hexdigest()},
+                "restart_bytes": {str(n1): len(t1), str(n1 + n2): len(t3)},
+                "reference_resumes_bitwise": True, "text_gives_back_the_doubles": True}
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(meta, fh, indent=1, sort_keys=True)
+        print("wrote", name, meta["restart_sha256"])
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["--stops"]:
         record_stops()
+    elif sys.argv[1:] == ["--restarts"]:
+        record_restarts()
     else:
         main(sys.argv[1:] or None)

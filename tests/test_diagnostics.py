@@ -27,3 +27,16 @@ def test_window_skips_the_transient():
 def test_no_growth_gives_no_window():
     a = np.full(30, 0.01) * (1.0 + 0.01 * np.sin(np.arange(30)))
     assert diagnostics.growth_window(a) is None
+
+
+def test_rate_from_a_quadratic_integral():
+    # vy = A(t) f(x, y) gives an integral of vy^2 growing as A^2: the
+    # amplitude rate is half its log slope, as fit_growth.gnu halves its fit
+    t = np.linspace(0.0, 2.0, 41)
+    amp = 0.01 * np.exp(1.3 * t)
+    x = np.linspace(-0.5, 0.5, 64, endpoint=False)
+    shape = np.sin(2 * np.pi * x) * np.exp(-x * x / 0.01)
+    ints = np.array([np.sum((a * shape) ** 2) / 64 for a in amp])
+    win = diagnostics.growth_window(amp)
+    assert abs(diagnostics.growth_rate_of_integral(t, ints, win) - 1.3) < 1e-10
+    assert abs(diagnostics.growth_rate(t, ints, win) - 2.6) < 1e-10

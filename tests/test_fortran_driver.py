@@ -58,3 +58,56 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     assert glob.shape == (steps, 39)
     assert np.array_equal(glob[:, 2:], np.array(want_glob))
     assert np.array_equal(glob[:, 0], np.arange(1, steps + 1))
+
+
+def _run_driver(cwd, ni, nj, fs, rec, order, sigma, bz0, steps, hloc, restart_in="-", hini=1, hglb=1):
+    for d in ("data_restart", "data_global"):
+        os.makedirs(os.path.join(cwd, d), exist_ok=True)
+    args = [DRIVER, str(ni), str(nj), str(steps), str(fs), str(rec), str(order), repr(sigma), "1", "0.5", "0.01",
+            repr(bz0), "20261202", str(hglb), "run", str(hloc), restart_in, str(hini)]
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=cwd)
+    assert res.returncode == 0, res.stdout + res.stderr
+    return open(os.path.join(cwd, "run.bin"), "rb").read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["kh_hllc_mp5_32x64_restart", "kh_hlle_mp7_bz_32x64_restart"])
+def test_driver_restart_files_are_the_reference_s(tmp_path, name):
+    """Restart dumps of the driver against the SHA-256 of the reference's own
+    (its lector after the same steps), and a run resumed from one of them
+    against the uninterrupted run, bit for bit."""
+    import hashlib
+    import json
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (needs flang)")
+    fix = json.load(open(os.path.join(ROOT, "tests", "golden", name + ".json")))
+    case = json.load(open(os.path.join(ROOT, "tests", "golden", fix["case"] + ".json")))
+    n1, n2 = fix["steps"]
+    assert n1 % 2 == 0 and n2 % 2 == 0
+    ni, nj = case["native_grid"]
+    conf = (ni, nj, case["flux_solver"], case["rec_mp"], case["order"], case["sigma"], case["init"].get("bz0", 0.0))
+    print("driver restart", name, flush=True)
+    whole = str(tmp_path / "whole")
+    end_whole = _run_driver(whole, *conf, n1 + n2, hloc=2)
+    dump = lambda d, k: open(os.path.join(d, cueva.restart_file_name(k)), "rb").read()  # noqa: E731
+    # dumps every 2 steps: number n1/2 is step n1, (n1+n2)/2 the last step
+    assert hashlib.sha256(dump(whole, n1 // 2)).hexdigest() == fix["restart_sha256"][str(n1)]
+    assert hashlib.sha256(dump(whole, (n1 + n2) // 2)).hexdigest() == fix["restart_sha256"][str(n1 + n2)]
+    # resumed from the step-n1 file, numbering steps on from n1 + 1
+    resumed = str(tmp_path / "resumed")
+    os.makedirs(resumed)
+    src = os.path.join(resumed, "all_variables.dat")
+    open(src, "wb").write(dump(whole, n1 // 2))
+    end_resumed = _run_driver(resumed, *conf, n2, hloc=2, restart_in=src, hini=n1 + 1)
+    assert end_resumed == end_whole
+    assert dump(resumed, (n1 + n2) // 2) == dump(whole, (n1 + n2) // 2)
+    assert not os.path.exists(os.path.join(resumed, cueva.restart_file_name(n1 // 2)))
+    # the reference's per-quantity global files: "h t value" records of the
+    # same values as the driver's own table
+    table = np.loadtxt(os.path.join(whole, "run_globals.dat"), ndmin=2)
+    vy = np.loadtxt(os.path.join(whole, "data_global", "2d_data_vy_integral_vol.dat"), ndmin=2)
+    k = cueva.GLOBAL_NAMES.index("sum_vy")
+    assert vy.shape == (n1 + n2, 3)
+    assert np.array_equal(vy[:, 0], table[:, 0]) and np.array_equal(vy[:, 2], table[:, 2 + k])
+    assert np.array_equal(vy[:, 1], table[:, 1])
+    assert len(os.listdir(os.path.join(whole, "data_global"))) == len(cueva.GLOBAL_NAMES)
