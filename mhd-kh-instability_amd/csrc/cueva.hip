@@ -170,7 +170,8 @@ static int check_params(const cueva_params* p) {
   if (!p) return CUEVA_E_ARG;
   if (p->imax < 8 || p->jmax < 8 || p->imax > 65536 || p->jmax > 65536) return CUEVA_E_ARG;
   if (p->order != 2 && p->order != 4) return CUEVA_E_ARG;
-  if (p->flux_solver != CUEVA_HLL && p->flux_solver != CUEVA_HLLC && p->flux_solver != CUEVA_HLLE)
+  if (p->flux_solver != CUEVA_HLL && p->flux_solver != CUEVA_HLLC && p->flux_solver != CUEVA_HLLE &&
+      p->flux_solver != CUEVA_HLL_MCL)
     return CUEVA_E_ARG;
   if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
@@ -395,7 +396,7 @@ extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
       break;
     case CUEVA_BND_FLUX:
       // guarded by flux_solver == 5 .or. 7 (24_boundary_flux_conserved.f95:13)
-      if (c->prm.flux_solver == CUEVA_HLLE) return CUEVA_OK;
+      if (!j_ghosts_filled(c->prm.flux_solver)) return CUEVA_OK;
       list[n++] = c->P.s[S_JX]; list[n++] = c->P.s[S_JY]; list[n++] = c->P.s[S_JZ];
       break;
     default:
@@ -453,7 +454,9 @@ extern "C" int cueva_flowinit(cueva_ctx* c, int l) {
 template <int ORD>
 static int launch_faces_ord(cueva_ctx* c) {
   const dim3 grid = faces_grid(c->g, BLK);
-  if (c->prm.flux_solver == CUEVA_HLLC)
+  if (c->prm.flux_solver == CUEVA_HLL_MCL)
+    LAUNCH(K_FACES, (k_faces<ORD, 2>), grid, BLK, c->g, c->P, c->S, c->dst);
+  else if (c->prm.flux_solver == CUEVA_HLLC)
     LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, c->dst);
   else if (c->prm.flux_solver == CUEVA_HLLE)
     LAUNCH(K_FACES, (k_faces<ORD, 6>), grid, BLK, c->g, c->P, c->S, c->dst);
@@ -470,6 +473,7 @@ static int launch_faces(cueva_ctx* c) {
     prof_end(c);
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
+  if (c->prm.flux_solver == CUEVA_HLL_MCL) return launch_faces_ord<5>(c);  // the MP order does not enter
   switch (c->rec_order) {
     case 5: return launch_faces_ord<5>(c);
     case 7: return launch_faces_ord<7>(c);
@@ -565,7 +569,8 @@ static int march_waves(const cueva_ctx* c) {
 
 // ghost band of a stage state formed at the interior points only: a copy of
 // the source point's (the state is pointwise in the point's own data), except
-// HLLE's J, which the reference keeps 0 there (24_boundary_flux_conserved.f95:13);
+// J under HLLE and HLL+MCL, which the reference keeps 0 there
+// (24_boundary_flux_conserved.f95:13);
 // with_prims: the primitive planes' ghosts in the same launch
 static int fill_stage_ghosts(cueva_ctx* c, bool with_prims = false) {
   double* list[P_N + 17];
@@ -575,7 +580,7 @@ static int fill_stage_ghosts(cueva_ctx* c, bool with_prims = false) {
   const int st[17] = {S_PSI, S_PHI, S_BX, S_BY, S_BZ, S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ,
                       S_EX, S_EY, S_EZ, S_JX, S_JY, S_JZ};
   for (int k = 0; k < 17; k++) list[n++] = c->P.s[st[k]];
-  return fill_ghosts(c, list, n, c->prm.flux_solver == CUEVA_HLLE ? 3 : 0);
+  return fill_ghosts(c, list, n, j_ghosts_filled(c->prm.flux_solver) ? 0 : 3);
 }
 
 // U^{n+1}, left by the last marching launch in rhs[L][0..13], becomes the

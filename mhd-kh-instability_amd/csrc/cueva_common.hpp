@@ -21,6 +21,10 @@ enum { R_SEX = 14, R_SEY = 15, R_SEZ = 16, R_N = 17 };
 
 constexpr int MAXL = 4;
 
+// boundary_flux_conserved fills the ghosts of J only for flux_solver 5 and 7
+// (24_boundary_flux_conserved.f95:13-14); otherwise they keep their zeros
+__host__ __device__ inline bool j_ghosts_filled(int flux_solver) { return flux_solver == 5 || flux_solver == 7; }
+
 struct Planes {
   double* p[P_N];
   double* c[C_N];

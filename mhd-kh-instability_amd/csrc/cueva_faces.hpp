@@ -81,6 +81,132 @@ __device__ inline void face_fields(const Planes& P, int64_t c, int64_t st, Side<
   face_rec<ORD>(P.s[DIR == 0 ? S_JX : S_JY], c, st, L.jn, R.jn);
 }
 
+// ---------------------------------------------------------------------------
+// HLL with the monotonised-central slope limiter (flux_solver = 2,
+// 16_hll_flow_mcl.f95:554-1402, REC_PRIM = 0): second-order face states of
+// the stage fields and of the t^n velocity, pressure and density, and the
+// Lax-Friedrichs-form HLL flux with signal speed Sp = 1 (const_sp = 0,
+// parameters.f95:274).  The x face uses only x-reconstructed states and the
+// y face only y-reconstructed ones.
+// ---------------------------------------------------------------------------
+// mcl(fa, fb) with slop = 2 (funciones.f95:57-95, parameters.f95:17)
+template <typename T> __device__ inline T mcl_slope(T fa, T fb) {
+  return (T(0.5) * (csign(fa) + csign(fb))) * mn(mn(T(2) * fab(fa), T(2) * fab(fb)), T(0.5) * fab(fa + fb));
+}
+
+// left and right face values between the cells at c and c+st
+// (u_L = u_i + FAC mcl(.), u_R = u_{i+1} - FAC mcl(.), FAC = 0.5); the
+// half-slopes are returned too, for the "_m" states
+template <typename T>
+__device__ inline void mcl_rec(const double* __restrict__ a, int64_t c, int64_t st, T& l, T& r, T* u01 = nullptr,
+                               T* hs = nullptr) {
+  const T um = (T)a[c - st], u0 = (T)a[c], u1 = (T)a[c + st], u2 = (T)a[c + 2 * st];
+  const T sl = T(0.5) * mcl_slope(u1 - u0, u0 - um);
+  const T sr = T(0.5) * mcl_slope(u2 - u1, u1 - u0);
+  l = u0 + sl;
+  r = u1 - sr;
+  if (u01) { u01[0] = u0; u01[1] = u1; hs[0] = sl; hs[1] = sr; }
+}
+
+template <typename T> struct McSide { T psi, phi, ex, ey, ez, bx, by, bz, q, d, tau, sx, sy, sz, vx, vy, vz, p, rho; };
+
+// J, FD, Ftau and the three FS of one side: E^2+B^2 (16_hll_flow_mcl.f95:821),
+// W (:900), E.v (:995), J (:1033), FD (:1050), eps and h (:1124), Ftau
+// (:1181), FS (:1198-1241)
+template <int DIR, typename T>
+__device__ inline void mcl_side_fluxes(T gam, T sigma, const McSide<T>& o, T& jn, T& fd, T& ftau, T& fs1, T& fs2,
+                                       T& fs3) {
+  const T e2b2 = ((((sq(o.ex) + sq(o.ey)) + sq(o.ez)) + sq(o.bx)) + sq(o.by)) + sq(o.bz);
+  const T w = T(1) / sqrt(((T(1) - sq(o.vx)) - sq(o.vy)) - sq(o.vz));
+  const T edv = (o.ex * o.vx + o.ey * o.vy) + o.ez * o.vz;
+  const T eps = o.p / ((gam - T(1)) * o.rho);
+  const T h = o.rho * (T(1) + eps) + o.p;
+  const T hw2 = h * sq(w);
+  if constexpr (DIR == 0) {
+    jn = (sigma * w) * ((o.ex + (o.bz * o.vy - o.by * o.vz)) - edv * o.vx) + o.q * o.vx;
+    fd = o.d * o.vx;
+    ftau = (o.bz * o.ey - o.by * o.ez) + hw2 * o.vx;
+    fs1 = ((((-sq(o.ex)) - sq(o.bx)) + hw2 * sq(o.vx)) + T(0.5) * e2b2) + o.p;
+    fs2 = ((-(o.ex * o.ey)) - o.bx * o.by) + (hw2 * o.vx) * o.vy;
+    fs3 = ((-(o.ex * o.ez)) - o.bx * o.bz) + (hw2 * o.vx) * o.vz;
+  } else {
+    jn = (sigma * w) * ((o.ey + (o.bx * o.vz - o.bz * o.vx)) - edv * o.vy) + o.q * o.vy;
+    fd = o.d * o.vy;
+    ftau = (o.bx * o.ez - o.bz * o.ex) + hw2 * o.vy;
+    fs1 = ((-(o.ex * o.ey)) - o.bx * o.by) + (hw2 * o.vx) * o.vy;
+    fs2 = ((((-sq(o.ey)) - sq(o.by)) + hw2 * sq(o.vy)) + T(0.5) * e2b2) + o.p;
+    fs3 = ((-(o.ey * o.ez)) - o.by * o.bz) + (hw2 * o.vy) * o.vz;
+  }
+}
+
+// 0.5 (A_L + A_R - Sp (U_R - U_L)) with Sp = 1 (16_hll_flow_mcl.f95:1259-1396)
+template <typename T> __device__ inline T mcl_lf(T al, T ar, T ul, T ur) { return T(0.5) * ((al + ar) - (ur - ul)); }
+
+// the fluxes of the face between the cells at c and c+st, DIR 0 = x (FX_*
+// slots), 1 = y (FY_* slots)
+template <int DIR, typename T>
+__device__ inline void mcl_face(const Planes& P, const Scheme& S, int64_t c, int64_t st, T* f) {
+  McSide<T> L, R;
+  T e0[2], es[2], b0[2], bs[2];  // cell values and half-slopes of the "_m" pair
+  mcl_rec(P.s[S_PSI], c, st, L.psi, R.psi);
+  mcl_rec(P.s[S_PHI], c, st, L.phi, R.phi);
+  mcl_rec(P.s[S_EX], c, st, L.ex, R.ex, DIR == 1 ? e0 : (T*)nullptr, DIR == 1 ? es : (T*)nullptr);
+  mcl_rec(P.s[S_EY], c, st, L.ey, R.ey);
+  mcl_rec(P.s[S_EZ], c, st, L.ez, R.ez, DIR == 0 ? e0 : (T*)nullptr, DIR == 0 ? es : (T*)nullptr);
+  mcl_rec(P.s[S_BX], c, st, L.bx, R.bx);
+  mcl_rec(P.s[S_BY], c, st, L.by, R.by, DIR == 0 ? b0 : (T*)nullptr, DIR == 0 ? bs : (T*)nullptr);
+  mcl_rec(P.s[S_BZ], c, st, L.bz, R.bz, DIR == 1 ? b0 : (T*)nullptr, DIR == 1 ? bs : (T*)nullptr);
+  mcl_rec(P.s[S_Q], c, st, L.q, R.q);
+  mcl_rec(P.s[S_D], c, st, L.d, R.d);
+  mcl_rec(P.s[S_TAU], c, st, L.tau, R.tau);
+  mcl_rec(P.s[S_SX], c, st, L.sx, R.sx);
+  mcl_rec(P.s[S_SY], c, st, L.sy, R.sy);
+  mcl_rec(P.s[S_SZ], c, st, L.sz, R.sz);
+  mcl_rec(P.p[P_VX], c, st, L.vx, R.vx);
+  mcl_rec(P.p[P_VY], c, st, L.vy, R.vy);
+  mcl_rec(P.p[P_VZ], c, st, L.vz, R.vz);
+  mcl_rec(P.p[P_P], c, st, L.p, R.p);
+  mcl_rec(P.p[P_RHO], c, st, L.rho, R.rho);
+  // "_m" states: the cell value negated, the slope kept
+  // (16_hll_flow_mcl.f95:634-655, 708-737)
+  const T em_l = (-e0[0]) + es[0], em_r = (-e0[1]) - es[1];
+  const T bm_l = (-b0[0]) + bs[0], bm_r = (-b0[1]) - bs[1];
+  T jl, fdl, ftl, f1l, f2l, f3l, jr, fdr, ftr, f1r, f2r, f3r;
+  mcl_side_fluxes<DIR>((T)S.gamma, (T)S.sigma, L, jl, fdl, ftl, f1l, f2l, f3l);
+  mcl_side_fluxes<DIR>((T)S.gamma, (T)S.sigma, R, jr, fdr, ftr, f1r, f2r, f3r);
+  if constexpr (DIR == 0) {
+    f[FX_EXPSI] = mcl_lf(L.ex, R.ex, L.psi, R.psi);
+    f[FX_BXPHI] = mcl_lf(L.bx, R.bx, L.phi, R.phi);
+    f[FX_PHIBX] = mcl_lf(L.phi, R.phi, L.bx, R.bx);
+    f[FX_EZBY] = -mcl_lf(em_l, em_r, L.by, R.by);
+    f[FX_EYBZ] = mcl_lf(L.ey, R.ey, L.bz, R.bz);
+    f[FX_PSIEX] = mcl_lf(L.psi, R.psi, L.ex, R.ex);
+    f[FX_BZEY] = mcl_lf(L.bz, R.bz, L.ey, R.ey);
+    f[FX_BYEZ] = -mcl_lf(bm_l, bm_r, L.ez, R.ez);
+    f[FX_J] = mcl_lf(jl, jr, L.q, R.q);
+    f[FX_FD] = mcl_lf(fdl, fdr, L.d, R.d);
+    f[FX_FTAU] = mcl_lf(ftl, ftr, L.tau, R.tau);
+    f[FX_FSX] = mcl_lf(f1l, f1r, L.sx, R.sx);
+    f[FX_FSY] = mcl_lf(f2l, f2r, L.sy, R.sy);
+    f[FX_FSZ] = mcl_lf(f3l, f3r, L.sz, R.sz);
+  } else {
+    f[FY_EYPSI] = mcl_lf(L.ey, R.ey, L.psi, R.psi);
+    f[FY_BYPHI] = mcl_lf(L.by, R.by, L.phi, R.phi);
+    f[FY_EZBX] = mcl_lf(L.ez, R.ez, L.bx, R.bx);
+    f[FY_PHIBY] = mcl_lf(L.phi, R.phi, L.by, R.by);
+    f[FY_EXBZ] = -mcl_lf(em_l, em_r, L.bz, R.bz);
+    f[FY_BZEX] = -mcl_lf(bm_l, bm_r, L.ex, R.ex);
+    f[FY_PSIEY] = mcl_lf(L.psi, R.psi, L.ey, R.ey);
+    f[FY_BXEZ] = mcl_lf(L.bx, R.bx, L.ez, R.ez);
+    f[FY_J] = mcl_lf(jl, jr, L.q, R.q);
+    f[FY_FD] = mcl_lf(fdl, fdr, L.d, R.d);
+    f[FY_FTAU] = mcl_lf(ftl, ftr, L.tau, R.tau);
+    f[FY_FSX] = mcl_lf(f1l, f1r, L.sx, R.sx);
+    f[FY_FSY] = mcl_lf(f2l, f2r, L.sy, R.sy);
+    f[FY_FSZ] = mcl_lf(f3l, f3r, L.sz, R.sz);
+  }
+}
+
 // both faces of the face pair (i,j): put_x(Fx(i+1/2,j)) as soon as the x
 // face is solved, put_y(Fy(i,j+1/2)) as soon as the y face is; each flux
 // array is dead after its put, which keeps the other solve's registers free
@@ -92,6 +218,22 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;
+  if constexpr (SOLVER == 2) {  // HLL + MCL: the two faces are independent
+    T f[FX_N > FY_N ? FX_N : FY_N];
+    if constexpr (YFIRST) {
+      mcl_face<1>(P, S, c, sy, f);
+      put_y(f);
+      mcl_face<0>(P, S, c, sx, f);
+      put_x(f);
+    } else {
+      mcl_face<0>(P, S, c, sx, f);
+      put_x(f);
+      mcl_face<1>(P, S, c, sy, f);
+      put_y(f);
+    }
+    (void)st;
+    return;
+  }
   Side<T> xl, xr, yl, yr;
   // kinematics of both faces first: the fallbacks couple them
   face_rec<ORD>(P.p[P_VX], c, sx, xl.vx, xr.vx);
@@ -379,6 +521,7 @@ inline const void* march_kernel(int rec_order, int solver, bool final_stage) {
 #define CUEVA_MK(O, SV)                                                        \
   (final_stage ? reinterpret_cast<const void*>(&k_faces_march<O, SV, T, true>) \
                : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false>))
+  if (solver == 2) return CUEVA_MK(5, 2);  // second order: the MP order does not enter
   switch (rec_order * 10 + solver) {
     case 55: return CUEVA_MK(5, 5);
     case 56: return CUEVA_MK(5, 6);

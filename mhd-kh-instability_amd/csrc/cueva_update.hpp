@@ -122,7 +122,7 @@ __device__ inline void stage_point(const Planes& P, const Scheme& S, int l, int6
   jx = jx + stv[A_Q] * vx;
   jy = jy + stv[A_Q] * vy;
   jz = jz + stv[A_Q] * vz;
-  if (ghost && S.flux_solver == 6) jx = jy = jz = 0.0;  // 24_boundary_flux_conserved.f95:13
+  if (ghost && !j_ghosts_filled(S.flux_solver)) jx = jy = jz = 0.0;  // 24_boundary_flux_conserved.f95:13
   out[S_PSI] = stv[A_PSI]; out[S_PHI] = stv[A_PHI];
   out[S_BX] = stv[A_BX]; out[S_BY] = stv[A_BY]; out[S_BZ] = stv[A_BZ];
   out[S_EX] = ex; out[S_EY] = ey; out[S_EZ] = ez;

@@ -122,7 +122,9 @@ program main_cueva_gpu
       call boundary_electric
       call flowinit
       call boundary_flux_conserved
-      if (prm%flux_solver == CUEVA_HLL) then
+      if (prm%flux_solver == CUEVA_HLL_MCL) then
+        call hll_flow
+      else if (prm%flux_solver == CUEVA_HLL) then
         call hll_flow_mpx
       else if (prm%flux_solver == CUEVA_HLLE) then
         call hlle_flow_mpx

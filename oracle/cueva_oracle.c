@@ -104,7 +104,8 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   memset(c, 0, sizeof(*c));
   if (imax < 8 || jmax < 8) return 1;
   if (lmax != 2 && lmax != 4) return 2;
-  if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC) return 3;
+  if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL)
+    return 3;
   if (rec_order != 5 && rec_order != 7 && rec_order != 9) return 4;
   c->imax = imax;
   c->jmax = jmax;
@@ -1159,12 +1160,22 @@ static int hlle_xy(double gam, const side_t *xl, const side_t *xr, const side_t 
 
 /* face loop: 16_hll_flow_mpx.f95:1358-1958, 16_hllc_flow_mpx.f95:2401-4612,
  * 16_hlle_flow_mpx.f95:2258-4618 */
+static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f);
+
 static int face_loop(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
   double **P = s->prim;
   for (int j = -1; j <= jm; j++)
     for (int i = -1; i <= im; i++) {
       size_t k = IX(s, i, j);
+      if (fs == OR_HLL_MCL) {
+        double fx[FX_N], fy[FY_N];
+        mcl_face(s, i, j, 0, fx);
+        mcl_face(s, i, j, 1, fy);
+        for (int f = 0; f < FX_N; f++) s->fx[f][k] = fx[f];
+        for (int f = 0; f < FY_N; f++) s->fy[f][k] = fy[f];
+        continue;
+      }
       side_t xl, xr, yl, yr;
       load_x(s, s->xL, k, &xl);
       load_x(s, s->xR, k, &xr);
@@ -1214,6 +1225,130 @@ static int face_loop(oracle_state *s) {
       for (int f = 0; f < FY_N; f++) s->fy[f][k] = fy[f];
     }
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* HLL with the monotonised-central slope limiter, flux_solver = 2:     */
+/* hll_flow, 16_hll_flow_mcl.f95:554-1402 (DIM = 2, REC_PRIM = 0).      */
+/* Face states of the stage fields and of the t^n velocity, pressure    */
+/* and density; Sp = 1 (const_sp = 0, parameters.f95:274).               */
+/* ------------------------------------------------------------------ */
+/* funciones.f95:57-95 with slop = 2 (parameters.f95:17) */
+static inline double mcl(double fa, double fb) {
+  return (0.5 * (copysign(1.0, fa) + copysign(1.0, fb))) *
+         fmin2(fmin2(2.0 * fabs(fa), 2.0 * fabs(fb)), 0.5 * fabs(fa + fb));
+}
+
+typedef struct {
+  double psi, phi, ex, ey, ez, bx, by, bz, q, d, tau, sx, sy, sz, vx, vy, vz, p, rho;
+  double ezm, bym, exm, bzm; /* the "_m" states the fluxes use */
+} mcl_side_t;
+
+/* u_L = u(i) + FAC mcl(u(i+1)-u(i), u(i)-u(i-1)), u_R = u(i+1) - FAC
+ * mcl(u(i+2)-u(i+1), u(i+1)-u(i)), FAC = 0.5 (16_hll_flow_mcl.f95:572-575);
+ * the "_m" states negate the cell value only (:634-637) */
+static void mcl_rec(const double *a, size_t km, size_t k0, size_t k1, size_t k2, double *l, double *r,
+                    double *lm, double *rm) {
+  double u0 = a[k0], u1 = a[k1];
+  double sl = 0.5 * mcl(u1 - u0, u0 - a[km]);
+  double sr = 0.5 * mcl(a[k2] - u1, u1 - u0);
+  *l = u0 + sl;
+  *r = u1 - sr;
+  if (lm) {
+    *lm = (-u0) + sl;
+    *rm = (-u1) - sr;
+  }
+}
+
+/* J, FD, Ftau, FS of one side: :821 (E2B2), :900 (W), :995 (E.v),
+ * :1033 (J), :1050 (FD), :1124 (eps, h), :1181 (Ftau), :1198-1241 (FS) */
+static void mcl_side_fluxes(double gam, double sigma, int dir, const mcl_side_t *o, double out[6]) {
+  double e2b2 = ((((sq(o->ex) + sq(o->ey)) + sq(o->ez)) + sq(o->bx)) + sq(o->by)) + sq(o->bz);
+  double w = 1.0 / sqrt(((1.0 - sq(o->vx)) - sq(o->vy)) - sq(o->vz));
+  double edv = (o->ex * o->vx + o->ey * o->vy) + o->ez * o->vz;
+  double eps = o->p / ((gam - 1.0) * o->rho);
+  double h = o->rho * (1.0 + eps) + o->p;
+  double hw2 = h * sq(w);
+  if (dir == 0) {
+    double rx = o->bz * o->vy - o->by * o->vz;
+    out[0] = (sigma * w) * ((o->ex + rx) - edv * o->vx) + o->q * o->vx;
+    out[1] = o->d * o->vx;
+    out[2] = (o->bz * o->ey - o->by * o->ez) + hw2 * o->vx;
+    out[3] = ((((-sq(o->ex)) - sq(o->bx)) + hw2 * sq(o->vx)) + 0.5 * e2b2) + o->p;
+    out[4] = ((-(o->ex * o->ey)) - o->bx * o->by) + (hw2 * o->vx) * o->vy;
+    out[5] = ((-(o->ex * o->ez)) - o->bx * o->bz) + (hw2 * o->vx) * o->vz;
+  } else {
+    double ry = o->bx * o->vz - o->bz * o->vx;
+    out[0] = (sigma * w) * ((o->ey + ry) - edv * o->vy) + o->q * o->vy;
+    out[1] = o->d * o->vy;
+    out[2] = (o->bx * o->ez - o->bz * o->ex) + hw2 * o->vy;
+    out[3] = ((-(o->ex * o->ey)) - o->bx * o->by) + (hw2 * o->vx) * o->vy;
+    out[4] = ((((-sq(o->ey)) - sq(o->by)) + hw2 * sq(o->vy)) + 0.5 * e2b2) + o->p;
+    out[5] = ((-(o->ey * o->ez)) - o->by * o->bz) + (hw2 * o->vy) * o->vz;
+  }
+}
+
+/* 0.5 (A_L + A_R - Sp (U_R - U_L)), Sp = 1 (:1269-1396) */
+static inline double mcl_lf(double al, double ar, double ul, double ur) { return 0.5 * ((al + ar) - (ur - ul)); }
+
+/* the face between cell (i,j) and its +x (dir 0) or +y (dir 1) neighbour */
+static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
+  double *const *S = s->st, *const *P = s->prim;
+  const int di = dir == 0, dj = dir == 1;
+  size_t km = IX(s, i - di, j - dj), k0 = IX(s, i, j), k1 = IX(s, i + di, j + dj),
+         k2 = IX(s, i + 2 * di, j + 2 * dj);
+  mcl_side_t L, R;
+  mcl_rec(S[ST_PSI], km, k0, k1, k2, &L.psi, &R.psi, NULL, NULL);
+  mcl_rec(S[ST_PHI], km, k0, k1, k2, &L.phi, &R.phi, NULL, NULL);
+  mcl_rec(S[ST_EX], km, k0, k1, k2, &L.ex, &R.ex, &L.exm, &R.exm);
+  mcl_rec(S[ST_EY], km, k0, k1, k2, &L.ey, &R.ey, NULL, NULL);
+  mcl_rec(S[ST_EZ], km, k0, k1, k2, &L.ez, &R.ez, &L.ezm, &R.ezm);
+  mcl_rec(S[ST_BX], km, k0, k1, k2, &L.bx, &R.bx, NULL, NULL);
+  mcl_rec(S[ST_BY], km, k0, k1, k2, &L.by, &R.by, &L.bym, &R.bym);
+  mcl_rec(S[ST_BZ], km, k0, k1, k2, &L.bz, &R.bz, &L.bzm, &R.bzm);
+  mcl_rec(S[ST_Q], km, k0, k1, k2, &L.q, &R.q, NULL, NULL);
+  mcl_rec(S[ST_D], km, k0, k1, k2, &L.d, &R.d, NULL, NULL);
+  mcl_rec(S[ST_TAU], km, k0, k1, k2, &L.tau, &R.tau, NULL, NULL);
+  mcl_rec(S[ST_SX], km, k0, k1, k2, &L.sx, &R.sx, NULL, NULL);
+  mcl_rec(S[ST_SY], km, k0, k1, k2, &L.sy, &R.sy, NULL, NULL);
+  mcl_rec(S[ST_SZ], km, k0, k1, k2, &L.sz, &R.sz, NULL, NULL);
+  mcl_rec(P[OR_VX], km, k0, k1, k2, &L.vx, &R.vx, NULL, NULL);
+  mcl_rec(P[OR_VY], km, k0, k1, k2, &L.vy, &R.vy, NULL, NULL);
+  mcl_rec(P[OR_VZ], km, k0, k1, k2, &L.vz, &R.vz, NULL, NULL);
+  mcl_rec(P[OR_P], km, k0, k1, k2, &L.p, &R.p, NULL, NULL);
+  mcl_rec(P[OR_RHO], km, k0, k1, k2, &L.rho, &R.rho, NULL, NULL);
+  double fl[6], fr[6];
+  mcl_side_fluxes(s->c.gamma, s->c.sigma, dir, &L, fl);
+  mcl_side_fluxes(s->c.gamma, s->c.sigma, dir, &R, fr);
+  if (dir == 0) {
+    f[FX_EXPSI] = mcl_lf(L.ex, R.ex, L.psi, R.psi);             /* Exlaxpsi */
+    f[FX_BXPHI] = mcl_lf(L.bx, R.bx, L.phi, R.phi);             /* Bxlaxphi */
+    f[FX_PHIBX] = mcl_lf(L.phi, R.phi, L.bx, R.bx);             /* philaxBx */
+    f[FX_EZBY] = -mcl_lf(L.ezm, R.ezm, L.by, R.by);             /* EzlaxBy, :1292 */
+    f[FX_EYBZ] = mcl_lf(L.ey, R.ey, L.bz, R.bz);                /* EylaxBz */
+    f[FX_PSIEX] = mcl_lf(L.psi, R.psi, L.ex, R.ex);             /* psilaxEx */
+    f[FX_BZEY] = mcl_lf(L.bz, R.bz, L.ey, R.ey);                /* BzlaxEy */
+    f[FX_BYEZ] = -mcl_lf(L.bym, R.bym, L.ez, R.ez);             /* BylaxEz */
+  } else {
+    f[FY_EYPSI] = mcl_lf(L.ey, R.ey, L.psi, R.psi);             /* Eylaxpsi */
+    f[FY_BYPHI] = mcl_lf(L.by, R.by, L.phi, R.phi);             /* Bylaxphi */
+    f[FY_EZBX] = mcl_lf(L.ez, R.ez, L.bx, R.bx);                /* EzlaxBx */
+    f[FY_PHIBY] = mcl_lf(L.phi, R.phi, L.by, R.by);             /* philaxBy */
+    f[FY_EXBZ] = -mcl_lf(L.exm, R.exm, L.bz, R.bz);             /* ExlaxBz, :1299 */
+    f[FY_BZEX] = -mcl_lf(L.bzm, R.bzm, L.ex, R.ex);             /* BzlaxEx, :1308 */
+    f[FY_PSIEY] = mcl_lf(L.psi, R.psi, L.ey, R.ey);             /* psilaxEy */
+    f[FY_BXEZ] = mcl_lf(L.bx, R.bx, L.ez, R.ez);                /* BxlaxEz */
+  }
+  /* J, FD, Ftau, FS sit at the same slots in both directions */
+  _Static_assert((int)FX_J == (int)FY_J && (int)FX_FD == (int)FY_FD && (int)FX_FTAU == (int)FY_FTAU &&
+                     (int)FX_FSX == (int)FY_FSX && (int)FX_FSZ == (int)FY_FSZ,
+                 "slot layout");
+  f[FX_J] = mcl_lf(fl[0], fr[0], L.q, R.q);
+  f[FX_FD] = mcl_lf(fl[1], fr[1], L.d, R.d);
+  f[FX_FTAU] = mcl_lf(fl[2], fr[2], L.tau, R.tau);
+  f[FX_FSX] = mcl_lf(fl[3], fr[3], L.sx, R.sx);
+  f[FX_FSY] = mcl_lf(fl[4], fr[4], L.sy, R.sy);
+  f[FX_FSZ] = mcl_lf(fl[5], fr[5], L.sz, R.sz);
 }
 
 /* fluxandsource, 2D IMEX branch: 17_fluxandsource.f95:442-542 */
@@ -1449,7 +1584,7 @@ int oracle_step(oracle_state *s, int nsteps) {
       varint(s);
       electricvarint(s, l);
       flowinit(s);
-      reconstruct(s);
+      if (s->c.flux_solver != OR_HLL_MCL) reconstruct(s);
       int st = face_loop(s);
       if (st) return st;
       fluxandsource(s, l);

@@ -124,14 +124,16 @@ program cueva_ref_harness
         call boundary_electric
         call flowinit
         call boundary_flux_conserved
-        if (flux_solver == 5) then
+        if (flux_solver == 2) then
+           call hll_flow
+        else if (flux_solver == 5) then
            call hll_flow_mpx
         else if (flux_solver == 6) then
            call hlle_flow_mpx
         else if (flux_solver == 7) then
            call hllc_flow_mpx
         else
-           write(*,*) "harness: flux_solver must be 5, 6 or 7"
+           write(*,*) "harness: flux_solver must be 2, 5, 6 or 7"
            stop 3
         end if
         call fluxandsource

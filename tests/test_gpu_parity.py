@@ -38,6 +38,8 @@ CASES = {
     "hllc_mp9": (cueva.HLLC, 2, 2, 1.0e6, {}),
     "hll_mp9_ssp3": (cueva.HLL, 2, 4, 1.0e3, {}),
     "hlle_mp7_bz": (cueva.HLLE, 1, 2, 1.0e6, {"bz0": 0.1}),
+    "hll_mcl": (cueva.HLL_MCL, 0, 2, 1.0e3, {}),
+    "hll_mcl_bz": (cueva.HLL_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
 }
 
 
@@ -92,7 +94,7 @@ def test_gpu_equals_oracle_bitwise_100_steps(fs, sigma):
     assert sum(nd.values()) == 0, (nd, rel_l1(g, r))
 
 
-@pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz"])
+@pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz", "hll_mcl_bz"])
 def test_routine_entry_points_match_fused_step(case):
     """The per-subroutine entry points (the drop-in path) and cueva_step agree bit for bit."""
     fs, rec, order, sigma, kw = CASES[case]
@@ -138,6 +140,7 @@ BITWISE_DUMPS = {
     "kh_hll_mp5_32x64": (1, 10), "kh_hllc_mp5_32x64": (1, 10), "kh_hll_mp7_32x64": (1, 10),
     "kh_hllc_mp9_32x64": (1, 10), "kh_hll_mp9_ssp3_32x64": (1, 7), "kh_hlle_mp7_bz_32x64": (1, 10),
     "kh_cfg1_hll_mp5_128x256": (1,), "kh_hllc_mp5_128x256": (1, 10),
+    "kh_hll_mcl_32x64": (1, 10), "kh_hll_mcl_bz_32x64": (1, 10), "kh_hll_mcl_ssp3_32x64": (1, 5),
 }
 
 
