@@ -7,6 +7,8 @@
 
 #include <stddef.h>
 
+#include <type_traits>
+
 #include "cueva_common.hpp"
 #include "cueva_update.hpp"
 
@@ -207,6 +209,114 @@ __device__ inline void mcl_face(const Planes& P, const Scheme& S, int64_t c, int
   }
 }
 
+// ---------------------------------------------------------------------------
+// Lax-Friedrichs flux splitting (flux_solver = 1, 15_lax_flow.f95:733-1460):
+// at every cell the split fluxes F+- = A +- Sp U (Sp = 1) of the stage fields
+// and of the cell fluxes flowinit forms (14_flowinit.f95:264-373); the face
+// value is 0.5 (L + R), L = F+(i) + FAC mcl(F+(i+1)-F+(i), F+(i)-F+(i-1)),
+// R = F-(i+1) - FAC mcl(F+(i+2)-F+(i+1), F+(i+1)-F+(i)) (the slope of R is
+// also taken from F+).  flowinit fills only the interior and
+// boundary_flux_conserved no ghosts for this solver, so the cell fluxes are
+// 0 on ghost cells (cleanvarint, 07_clean_var.f95:341-350).
+// ---------------------------------------------------------------------------
+// J, FD, Ftau, FS along DIR at cell (i,j) = k, with the t^n W and enthalpy
+// of startconvar (06_conserved_var.f95:92-109)
+template <int DIR, typename T>
+__device__ inline void lax_cell_fluxes(const Geom& g, const Planes& P, const Scheme& S, int i, int j, int64_t k,
+                                       T* out) {
+  if (i < 0 || i > g.imax || j < 0 || j > g.jmax) {
+#pragma unroll
+    for (int t = 0; t < 6; t++) out[t] = T(0);
+    return;
+  }
+  const T vx = (T)P.p[P_VX][k], vy = (T)P.p[P_VY][k], vz = (T)P.p[P_VZ][k];
+  const T p = (T)P.p[P_P][k], rho = (T)P.p[P_RHO][k];
+  const T ex = (T)P.s[S_EX][k], ey = (T)P.s[S_EY][k], ez = (T)P.s[S_EZ][k];
+  const T bx = (T)P.s[S_BX][k], by = (T)P.s[S_BY][k], bz = (T)P.s[S_BZ][k];
+  const T w = T(1) / sqrt(T(1) - ((sq(vx) + sq(vy)) + sq(vz)));
+  const T eps = p / (((T)S.gamma - T(1)) * rho);
+  const T h = rho * (T(1) + eps) + p;
+  const T e2 = (sq(ex) + sq(ey)) + sq(ez), b2 = (sq(bx) + sq(by)) + sq(bz);
+  const T hw2 = h * sq(w);
+  out[0] = (T)P.s[DIR == 0 ? S_JX : S_JY][k];
+  if constexpr (DIR == 0) {
+    out[1] = (T)P.s[S_D][k] * vx;
+    out[2] = (bz * ey - by * ez) + hw2 * vx;
+    out[3] = ((((-sq(ex)) - sq(bx)) + hw2 * sq(vx)) + T(0.5) * (e2 + b2)) + p;
+    out[4] = ((-(ex * ey)) - bx * by) + (hw2 * vx) * vy;
+    out[5] = ((-(ex * ez)) - bx * bz) + (hw2 * vx) * vz;
+  } else {
+    out[1] = (T)P.s[S_D][k] * vy;
+    out[2] = (bx * ez - bz * ex) + hw2 * vy;
+    out[3] = ((-(ex * ey)) - bx * by) + (hw2 * vx) * vy;
+    out[4] = ((((-sq(ey)) - sq(by)) + hw2 * sq(vy)) + T(0.5) * (e2 + b2)) + p;
+    out[5] = ((-(ey * ez)) - by * bz) + (hw2 * vy) * vz;
+  }
+}
+
+// the face value from the split flux of cells i-1 .. i+2 (a: the flux, u:
+// the field it is split with, sa = -1 for the fluxes the reference splits
+// as -X +- U and negates afterwards)
+template <typename T> __device__ inline T lax_face_value(const T* a, const T* u, T sa = T(1)) {
+  const T p0 = sa * a[0] + u[0], p1 = sa * a[1] + u[1], p2 = sa * a[2] + u[2], p3 = sa * a[3] + u[3];
+  const T m2 = sa * a[2] - u[2];
+  const T l = p1 + T(0.5) * mcl_slope(p2 - p1, p1 - p0);
+  const T r = m2 - T(0.5) * mcl_slope(p3 - p2, p2 - p1);
+  return T(0.5) * (l + r);
+}
+
+template <typename T> __device__ inline void lax_ld(const double* __restrict__ a, const int64_t* k, T* v) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) v[c] = (T)a[k[c]];
+}
+
+// the fluxes of the face between cell (i,j) and its +x (DIR 0) or +y
+// (DIR 1) neighbour
+template <int DIR, typename T>
+__device__ inline void lax_face(const Geom& g, const Planes& P, const Scheme& S, int i, int j, T* f) {
+  int64_t k[4];
+  T fl[4][6];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int ii = DIR == 0 ? i + c - 1 : i, jj = DIR == 1 ? j + c - 1 : j;
+    k[c] = g.at(ii, jj);
+    lax_cell_fluxes<DIR>(g, P, S, ii, jj, k[c], fl[c]);
+  }
+  T psi[4], phi[4], ex[4], ey[4], ez[4], bx[4], by[4], bz[4];
+  lax_ld(P.s[S_PSI], k, psi); lax_ld(P.s[S_PHI], k, phi);
+  lax_ld(P.s[S_EX], k, ex); lax_ld(P.s[S_EY], k, ey); lax_ld(P.s[S_EZ], k, ez);
+  lax_ld(P.s[S_BX], k, bx); lax_ld(P.s[S_BY], k, by); lax_ld(P.s[S_BZ], k, bz);
+  if constexpr (DIR == 0) {  // 15_lax_flow.f95:930-1180
+    f[FX_EXPSI] = lax_face_value(ex, psi);
+    f[FX_BXPHI] = lax_face_value(bx, phi);
+    f[FX_PHIBX] = lax_face_value(phi, bx);
+    f[FX_EZBY] = -lax_face_value(ez, by, T(-1));
+    f[FX_EYBZ] = lax_face_value(ey, bz);
+    f[FX_PSIEX] = lax_face_value(psi, ex);
+    f[FX_BZEY] = lax_face_value(bz, ey);
+    f[FX_BYEZ] = -lax_face_value(by, ez, T(-1));
+  } else {
+    f[FY_EYPSI] = lax_face_value(ey, psi);
+    f[FY_BYPHI] = lax_face_value(by, phi);
+    f[FY_EZBX] = lax_face_value(ez, bx);
+    f[FY_PHIBY] = lax_face_value(phi, by);
+    f[FY_EXBZ] = -lax_face_value(ex, bz, T(-1));
+    f[FY_BZEX] = -lax_face_value(bz, ex, T(-1));
+    f[FY_PSIEY] = lax_face_value(psi, ey);
+    f[FY_BXEZ] = lax_face_value(bx, ez);
+  }
+  static_assert((int)FX_J == (int)FY_J && (int)FX_FSZ == (int)FY_FSZ, "fluid slots at the same place");
+  const int us[6] = {S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ};
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    T a[4], u[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) a[c] = fl[c][t];
+    lax_ld(P.s[us[t]], k, u);
+    f[FX_J + t] = lax_face_value(a, u);
+  }
+}
+
 // both faces of the face pair (i,j): put_x(Fx(i+1/2,j)) as soon as the x
 // face is solved, put_y(Fy(i,j+1/2)) as soon as the y face is; each flux
 // array is dead after its put, which keeps the other solve's registers free
@@ -218,18 +328,21 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;
-  if constexpr (SOLVER == 2) {  // HLL + MCL: the two faces are independent
+  if constexpr (SOLVER == 1 || SOLVER == 2) {  // second-order solvers: the two faces are independent
     T f[FX_N > FY_N ? FX_N : FY_N];
+    auto face = [&](auto dir) {
+      constexpr int D = decltype(dir)::value;
+      if constexpr (SOLVER == 1) lax_face<D>(g, P, S, i, j, f);
+      else mcl_face<D>(P, S, c, D == 0 ? sx : sy, f);
+      if constexpr (D == 0) put_x(f);
+      else put_y(f);
+    };
     if constexpr (YFIRST) {
-      mcl_face<1>(P, S, c, sy, f);
-      put_y(f);
-      mcl_face<0>(P, S, c, sx, f);
-      put_x(f);
+      face(std::integral_constant<int, 1>());
+      face(std::integral_constant<int, 0>());
     } else {
-      mcl_face<0>(P, S, c, sx, f);
-      put_x(f);
-      mcl_face<1>(P, S, c, sy, f);
-      put_y(f);
+      face(std::integral_constant<int, 0>());
+      face(std::integral_constant<int, 1>());
     }
     (void)st;
     return;
@@ -521,7 +634,8 @@ inline const void* march_kernel(int rec_order, int solver, bool final_stage) {
 #define CUEVA_MK(O, SV)                                                        \
   (final_stage ? reinterpret_cast<const void*>(&k_faces_march<O, SV, T, true>) \
                : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false>))
-  if (solver == 2) return CUEVA_MK(5, 2);  // second order: the MP order does not enter
+  if (solver == 1) return CUEVA_MK(5, 1);  // second order: the MP order does not enter
+  if (solver == 2) return CUEVA_MK(5, 2);
   switch (rec_order * 10 + solver) {
     case 55: return CUEVA_MK(5, 5);
     case 56: return CUEVA_MK(5, 6);

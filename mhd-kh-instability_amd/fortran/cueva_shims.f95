@@ -63,6 +63,10 @@ contains
   end subroutine boundary_flux_conserved
 
   ! the three MPx solvers: the library takes the solver from prm%flux_solver
+  subroutine lax_flow                 ! 15_lax_flow.f95 (Lax-Friedrichs)
+    call check(cueva_flux(ctx, int(l, c_int)), "lax_flow")
+  end subroutine lax_flow
+
   subroutine hll_flow                 ! 16_hll_flow_mcl.f95 (HLL + MCL)
     call check(cueva_flux(ctx, int(l, c_int)), "hll_flow")
   end subroutine hll_flow

@@ -122,7 +122,9 @@ program main_cueva_gpu
       call boundary_electric
       call flowinit
       call boundary_flux_conserved
-      if (prm%flux_solver == CUEVA_HLL_MCL) then
+      if (prm%flux_solver == CUEVA_LAX) then
+        call lax_flow
+      else if (prm%flux_solver == CUEVA_HLL_MCL) then
         call hll_flow
       else if (prm%flux_solver == CUEVA_HLL) then
         call hll_flow_mpx

@@ -104,7 +104,8 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   memset(c, 0, sizeof(*c));
   if (imax < 8 || jmax < 8) return 1;
   if (lmax != 2 && lmax != 4) return 2;
-  if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL)
+  if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL &&
+      flux_solver != OR_LAX)
     return 3;
   if (rec_order != 5 && rec_order != 7 && rec_order != 9) return 4;
   c->imax = imax;
@@ -1161,6 +1162,7 @@ static int hlle_xy(double gam, const side_t *xl, const side_t *xr, const side_t 
 /* face loop: 16_hll_flow_mpx.f95:1358-1958, 16_hllc_flow_mpx.f95:2401-4612,
  * 16_hlle_flow_mpx.f95:2258-4618 */
 static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f);
+static void lax_face(const oracle_state *s, int i, int j, int dir, double *f);
 
 static int face_loop(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
@@ -1168,10 +1170,15 @@ static int face_loop(oracle_state *s) {
   for (int j = -1; j <= jm; j++)
     for (int i = -1; i <= im; i++) {
       size_t k = IX(s, i, j);
-      if (fs == OR_HLL_MCL) {
+      if (fs == OR_HLL_MCL || fs == OR_LAX) {
         double fx[FX_N], fy[FY_N];
-        mcl_face(s, i, j, 0, fx);
-        mcl_face(s, i, j, 1, fy);
+        if (fs == OR_LAX) {
+          lax_face(s, i, j, 0, fx);
+          lax_face(s, i, j, 1, fy);
+        } else {
+          mcl_face(s, i, j, 0, fx);
+          mcl_face(s, i, j, 1, fy);
+        }
         for (int f = 0; f < FX_N; f++) s->fx[f][k] = fx[f];
         for (int f = 0; f < FY_N; f++) s->fy[f][k] = fy[f];
         continue;
@@ -1349,6 +1356,101 @@ static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
   f[FX_FSX] = mcl_lf(fl[3], fr[3], L.sx, R.sx);
   f[FX_FSY] = mcl_lf(fl[4], fr[4], L.sy, R.sy);
   f[FX_FSZ] = mcl_lf(fl[5], fr[5], L.sz, R.sz);
+}
+
+/* ------------------------------------------------------------------ */
+/* Lax-Friedrichs flux splitting, flux_solver = 1: lax_flow,            */
+/* 15_lax_flow.f95:733-1460 (DIM = 2).  At every cell the split fluxes  */
+/* F+- = A +- Sp U (Sp = 1) of the stage fields and of the cell fluxes  */
+/* flowinit forms (14_flowinit.f95:264-373); the face value is          */
+/* 0.5 (L + R), L = F+(i) + FAC mcl(F+(i+1)-F+(i), F+(i)-F+(i-1)),        */
+/* R = F-(i+1) - FAC mcl(F+(i+2)-F+(i+1), F+(i+1)-F+(i)): the slope of R */
+/* is taken from F+ as well.  flowinit fills only the interior and      */
+/* boundary_flux_conserved fills no ghosts for this solver, so the cell  */
+/* fluxes are 0 on ghost cells (cleanvarint, 07_clean_var.f95:341-350).  */
+/* ------------------------------------------------------------------ */
+/* the six fluid cell fluxes along dir at cell k (14_flowinit.f95:291-373) */
+static void lax_cell_fluxes(const oracle_state *s, int i, int j, int dir, double out[6]) {
+  if (i < 0 || i > s->c.imax || j < 0 || j > s->c.jmax) {
+    for (int t = 0; t < 6; t++) out[t] = 0.0;
+    return;
+  }
+  double *const *S = s->st, *const *P = s->prim;
+  size_t k = IX(s, i, j);
+  double vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k], p = P[OR_P][k], rho = P[OR_RHO][k];
+  double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
+  double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k];
+  double w = s->W[k];                                           /* 06_conserved_var.f95:92 */
+  double eps = p / ((s->c.gamma - 1.0) * rho);                  /* :105 */
+  double h = rho * (1.0 + eps) + p;                             /* :109 */
+  double e2 = (sq(ex) + sq(ey)) + sq(ez), b2 = (sq(bx) + sq(by)) + sq(bz);
+  double hw2 = h * sq(w);
+  out[0] = S[dir == 0 ? ST_JX : ST_JY][k];
+  if (dir == 0) {
+    out[1] = S[ST_D][k] * vx;
+    out[2] = (bz * ey - by * ez) + hw2 * vx;
+    out[3] = ((((-sq(ex)) - sq(bx)) + hw2 * sq(vx)) + 0.5 * (e2 + b2)) + p;
+    out[4] = ((-(ex * ey)) - bx * by) + (hw2 * vx) * vy;
+    out[5] = ((-(ex * ez)) - bx * bz) + (hw2 * vx) * vz;
+  } else {
+    out[1] = S[ST_D][k] * vy;
+    out[2] = (bx * ez - bz * ex) + hw2 * vy;
+    out[3] = ((-(ex * ey)) - bx * by) + (hw2 * vx) * vy;
+    out[4] = ((((-sq(ey)) - sq(by)) + hw2 * sq(vy)) + 0.5 * (e2 + b2)) + p;
+    out[5] = ((-(ey * ez)) - by * bz) + (hw2 * vy) * vz;
+  }
+}
+
+/* one face value from the split fluxes of cells i-1 .. i+2 (a: the flux,
+ * u: the conserved field it is split with) */
+static inline double lax_face_value(const double a[4], const double u[4]) {
+  double p0 = a[0] + u[0], p1 = a[1] + u[1], p2 = a[2] + u[2], p3 = a[3] + u[3], m2 = a[2] - u[2];
+  double l = p1 + 0.5 * mcl(p2 - p1, p1 - p0);
+  double r = m2 - 0.5 * mcl(p3 - p2, p2 - p1);
+  return 0.5 * (l + r);
+}
+
+static void lax_face(const oracle_state *s, int i, int j, int dir, double *f) {
+  double *const *S = s->st;
+  const int di = dir == 0, dj = dir == 1;
+  size_t k[4];
+  double fl[4][6];
+  for (int c = 0; c < 4; c++) {
+    k[c] = IX(s, i + (c - 1) * di, j + (c - 1) * dj);
+    lax_cell_fluxes(s, i + (c - 1) * di, j + (c - 1) * dj, dir, fl[c]);
+  }
+#define LAX_FIELD(name, plane) double name[4]; for (int c = 0; c < 4; c++) name[c] = S[plane][k[c]];
+  LAX_FIELD(psi, ST_PSI) LAX_FIELD(phi, ST_PHI) LAX_FIELD(ex, ST_EX) LAX_FIELD(ey, ST_EY) LAX_FIELD(ez, ST_EZ)
+  LAX_FIELD(bx, ST_BX) LAX_FIELD(by, ST_BY) LAX_FIELD(bz, ST_BZ) LAX_FIELD(q, ST_Q) LAX_FIELD(d, ST_D)
+  LAX_FIELD(tau, ST_TAU) LAX_FIELD(sx, ST_SX) LAX_FIELD(sy, ST_SY) LAX_FIELD(sz, ST_SZ)
+#undef LAX_FIELD
+  double nex[4], nez[4], nby[4], nbz[4], a[4];
+  for (int c = 0; c < 4; c++) { nex[c] = -ex[c]; nez[c] = -ez[c]; nby[c] = -by[c]; nbz[c] = -bz[c]; }
+  if (dir == 0) { /* 15_lax_flow.f95:930-1180 */
+    f[FX_EXPSI] = lax_face_value(ex, psi);
+    f[FX_BXPHI] = lax_face_value(bx, phi);
+    f[FX_PHIBX] = lax_face_value(phi, bx);
+    f[FX_EZBY] = -lax_face_value(nez, by);
+    f[FX_EYBZ] = lax_face_value(ey, bz);
+    f[FX_PSIEX] = lax_face_value(psi, ex);
+    f[FX_BZEY] = lax_face_value(bz, ey);
+    f[FX_BYEZ] = -lax_face_value(nby, ez);
+  } else {
+    f[FY_EYPSI] = lax_face_value(ey, psi);
+    f[FY_BYPHI] = lax_face_value(by, phi);
+    f[FY_EZBX] = lax_face_value(ez, bx);
+    f[FY_PHIBY] = lax_face_value(phi, by);
+    f[FY_EXBZ] = -lax_face_value(nex, bz);
+    f[FY_BZEX] = -lax_face_value(nbz, ex);
+    f[FY_PSIEY] = lax_face_value(psi, ey);
+    f[FY_BXEZ] = lax_face_value(bx, ez);
+  }
+  const double *us[6] = {q, d, tau, sx, sy, sz};
+  const int slot[6] = {FX_J, FX_FD, FX_FTAU, FX_FSX, FX_FSY, FX_FSZ};
+  for (int t = 0; t < 6; t++) {
+    for (int c = 0; c < 4; c++) a[c] = fl[c][t];
+    f[slot[t]] = lax_face_value(a, us[t]);
+  }
 }
 
 /* fluxandsource, 2D IMEX branch: 17_fluxandsource.f95:442-542 */
@@ -1584,7 +1686,7 @@ int oracle_step(oracle_state *s, int nsteps) {
       varint(s);
       electricvarint(s, l);
       flowinit(s);
-      if (s->c.flux_solver != OR_HLL_MCL) reconstruct(s);
+      if (s->c.flux_solver != OR_HLL_MCL && s->c.flux_solver != OR_LAX) reconstruct(s);
       int st = face_loop(s);
       if (st) return st;
       fluxandsource(s, l);

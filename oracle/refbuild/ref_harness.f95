@@ -124,7 +124,9 @@ program cueva_ref_harness
         call boundary_electric
         call flowinit
         call boundary_flux_conserved
-        if (flux_solver == 2) then
+        if (flux_solver == 1) then
+           call lax_flow
+        else if (flux_solver == 2) then
            call hll_flow
         else if (flux_solver == 5) then
            call hll_flow_mpx
@@ -133,7 +135,7 @@ program cueva_ref_harness
         else if (flux_solver == 7) then
            call hllc_flow_mpx
         else
-           write(*,*) "harness: flux_solver must be 2, 5, 6 or 7"
+           write(*,*) "harness: flux_solver must be 1, 2, 5, 6 or 7"
            stop 3
         end if
         call fluxandsource

@@ -40,6 +40,8 @@ CASES = {
     "hlle_mp7_bz": (cueva.HLLE, 1, 2, 1.0e6, {"bz0": 0.1}),
     "hll_mcl": (cueva.HLL_MCL, 0, 2, 1.0e3, {}),
     "hll_mcl_bz": (cueva.HLL_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
+    "lax": (cueva.LAX, 0, 2, 1.0e3, {}),
+    "lax_bz": (cueva.LAX, 0, 2, 1.0e6, {"bz0": 0.1}),
 }
 
 
@@ -94,7 +96,7 @@ def test_gpu_equals_oracle_bitwise_100_steps(fs, sigma):
     assert sum(nd.values()) == 0, (nd, rel_l1(g, r))
 
 
-@pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz", "hll_mcl_bz"])
+@pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz", "hll_mcl_bz", "lax_bz"])
 def test_routine_entry_points_match_fused_step(case):
     """The per-subroutine entry points (the drop-in path) and cueva_step agree bit for bit."""
     fs, rec, order, sigma, kw = CASES[case]
@@ -141,6 +143,7 @@ BITWISE_DUMPS = {
     "kh_hllc_mp9_32x64": (1, 10), "kh_hll_mp9_ssp3_32x64": (1, 7), "kh_hlle_mp7_bz_32x64": (1, 10),
     "kh_cfg1_hll_mp5_128x256": (1,), "kh_hllc_mp5_128x256": (1, 10),
     "kh_hll_mcl_32x64": (1, 10), "kh_hll_mcl_bz_32x64": (1, 10), "kh_hll_mcl_ssp3_32x64": (1, 5),
+    "kh_lax_32x64": (1, 10), "kh_lax_bz_32x64": (1, 10),
 }
 
 
