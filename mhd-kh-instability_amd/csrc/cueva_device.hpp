@@ -191,6 +191,10 @@ template <typename T> struct Side {
   T vx, vy, vz, p, rho, jn;  // jn: normal current (Jx on x faces, Jy on y faces)
   T w, e2b2, h;
   T fd, ftau, fs1, fs2, fs3;  // normal physical fluxes
+  // the reference's "_m" states of the tangential E and B pair (x faces: Ez,
+  // By; y faces: Ex, Bz), read only by the HLLC solve of the MC-limited
+  // solver; with MPx they are the negated states
+  T em, bm;
 };
 
 // W, E^2+B^2, enthalpy and normal fluxes (16_hll_flow_mpx.f95:1463-1466,
@@ -301,7 +305,8 @@ template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, in
 }
 
 // contact speed, 16_hllc_flow_mpx.f95:3185-3334
-template <typename T> __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail) {
+template <typename T>
+__device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail, T zero = T(1.0e-9)) {
   T vlast;
   T det = sq(b) - (T(4) * a) * c;
   T qr = -(T(0.5) * (b + csign(b) * fab(sqrt(det))));
@@ -309,7 +314,7 @@ template <typename T> __device__ inline T hllc_contact(T a, T b, T c, bool seed_
   if (det < T(0)) rt2 = c / b;
   T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
   T v = hllc_vel(a, -b, c, rt2, v0, fail, vlast);
-  if (fab(v) <= T(1.0e-9))
+  if (fab(v) <= zero)
     v = T(0);
   else if (fab(v) > T(1))
     v = clampf(v, T(-1) + T(1.0e-6), T(1) - T(1.0e-6));
@@ -318,15 +323,20 @@ template <typename T> __device__ inline T hllc_contact(T a, T b, T c, bool seed_
 
 // HLLC x face, 16_hllc_flow_mpx.f95:2811-4295.  Returns 0 or a status.
 // seed_ref: first Newton point as the flang -O2 reference build (R.bz).
-template <typename T> __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref) {
+// MC: the MC-limited solver (16_hllc_flow_mcl.f95:2480-4140): its own "_m"
+// states, |v*| <= 1e-6 -> 0, and Newton from the documented seed.
+template <typename T, bool MC = false>
+__device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref) {
   const T S1 = T(-1), S2 = T(1);
+  const T ezmL = MC ? L.em : -L.ez, ezmR = MC ? R.em : -R.ez;
+  const T bymL = MC ? L.bm : -L.by, bymR = MC ? R.bm : -R.by;
   T psi_h = hll_state(L.psi, R.psi, L.ex, R.ex);
   T phi_h = hll_state(L.phi, R.phi, L.bx, R.bx);
   T ex_h = hll_state(L.ex, R.ex, L.psi, R.psi);
   T ey_h = hll_state(L.ey, R.ey, L.bz, R.bz);
-  T ez_h = hll_state(L.ez, R.ez, -L.by, -R.by);
+  T ez_h = hll_state(L.ez, R.ez, bymL, bymR);
   T bx_h = hll_state(L.bx, R.bx, L.phi, R.phi);
-  T by_h = hll_state(L.by, R.by, -L.ez, -R.ez);
+  T by_h = hll_state(L.by, R.by, ezmL, ezmR);
   T bz_h = hll_state(L.bz, R.bz, L.ey, R.ey);
   T dd_h = hll_state(L.d, R.d, L.fd, R.fd);
   T tau_h = hll_state(L.tau, R.tau, L.ftau, R.ftau);
@@ -336,11 +346,11 @@ template <typename T> __device__ inline int hllc_x(const Side<T>& L, const Side<
   T psi_f = hll_flux(L.ex, R.ex, L.psi, R.psi);
   T phi_f = hll_flux(L.bx, R.bx, L.phi, R.phi);
   T bx_f = hll_flux(L.phi, R.phi, L.bx, R.bx);
-  T by_f = hll_flux(-L.ez, -R.ez, L.by, R.by);
+  T by_f = hll_flux(ezmL, ezmR, L.by, R.by);
   T bz_f = hll_flux(L.ey, R.ey, L.bz, R.bz);
   T ex_f = hll_flux(L.psi, R.psi, L.ex, R.ex);
   T ey_f = hll_flux(L.bz, R.bz, L.ey, R.ey);
-  T ez_f = hll_flux(-L.by, -R.by, L.ez, R.ez);
+  T ez_f = hll_flux(bymL, bymR, L.ez, R.ez);
   T q_f = hll_flux(L.jn, R.jn, L.q, R.q);
   T dd_f = hll_flux(L.fd, R.fd, L.d, R.d);
   T tau_f = hll_flux(L.ftau, R.ftau, L.tau, R.tau);
@@ -357,7 +367,7 @@ template <typename T> __device__ inline int hllc_x(const Side<T>& L, const Side<
   T b = ((e2p + b2p) - tau_h) - sx_f;
   T c = sx_h - erx;
   int fail = 0;
-  T v = hllc_contact(a, b, c, seed_ref, R.bz, fail);
+  T v = MC ? hllc_contact(a, b, c, false, R.bz, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, R.bz, fail);
   if (fail || v != v) return 7;
   T pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
   T psiR = (S2 * psi_h - v * ex_f) / (S2 - v);
@@ -418,15 +428,18 @@ template <typename T> __device__ inline int hllc_x(const Side<T>& L, const Side<
 
 // HLLC y face, 16_hllc_flow_mpx.f95:2950-3095, 3338-3489, 3536-3574,
 // 3742-3901, 4305-4606.  seed_ref: first Newton point = -b (the b argument).
-template <typename T> __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref) {
+template <typename T, bool MC = false>
+__device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref) {
   const T S1 = T(-1), S2 = T(1);
+  const T exmL = MC ? L.em : -L.ex, exmR = MC ? R.em : -R.ex;
+  const T bzmL = MC ? L.bm : -L.bz, bzmR = MC ? R.bm : -R.bz;
   T psi_h = hll_state(L.psi, R.psi, L.ey, R.ey);
   T phi_h = hll_state(L.phi, R.phi, L.by, R.by);
   T ey_h = hll_state(L.ey, R.ey, L.psi, R.psi);
   T ez_h = hll_state(L.ez, R.ez, L.bx, R.bx);
-  T ex_h = hll_state(L.ex, R.ex, -L.bz, -R.bz);
+  T ex_h = hll_state(L.ex, R.ex, bzmL, bzmR);
   T by_h = hll_state(L.by, R.by, L.phi, R.phi);
-  T bz_h = hll_state(L.bz, R.bz, -L.ex, -R.ex);
+  T bz_h = hll_state(L.bz, R.bz, exmL, exmR);
   T bx_h = hll_state(L.bx, R.bx, L.ez, R.ez);
   T dd_h = hll_state(L.d, R.d, L.fd, R.fd);
   T tau_h = hll_state(L.tau, R.tau, L.ftau, R.ftau);
@@ -436,11 +449,11 @@ template <typename T> __device__ inline int hllc_y(const Side<T>& L, const Side<
   T psi_f = hll_flux(L.ey, R.ey, L.psi, R.psi);
   T phi_f = hll_flux(L.by, R.by, L.phi, R.phi);
   T by_f = hll_flux(L.phi, R.phi, L.by, R.by);
-  T bz_f = hll_flux(-L.ex, -R.ex, L.bz, R.bz);
+  T bz_f = hll_flux(exmL, exmR, L.bz, R.bz);
   T bx_f = hll_flux(L.ez, R.ez, L.bx, R.bx);
   T ey_f = hll_flux(L.psi, R.psi, L.ey, R.ey);
   T ez_f = hll_flux(L.bx, R.bx, L.ez, R.ez);
-  T ex_f = hll_flux(-L.bz, -R.bz, L.ex, R.ex);
+  T ex_f = hll_flux(bzmL, bzmR, L.ex, R.ex);
   T q_f = hll_flux(L.jn, R.jn, L.q, R.q);
   T dd_f = hll_flux(L.fd, R.fd, L.d, R.d);
   T tau_f = hll_flux(L.ftau, R.ftau, L.tau, R.tau);
@@ -457,7 +470,7 @@ template <typename T> __device__ inline int hllc_y(const Side<T>& L, const Side<
   T b = ((e2p + b2p) - tau_h) - sy_f;
   T c = sy_h - ery;
   int fail = 0;
-  T v = hllc_contact(a, b, c, seed_ref, -b, fail);
+  T v = MC ? hllc_contact(a, b, c, false, -b, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, -b, fail);
   if (fail || v != v) return 7;
   T pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
   T psiR = (S2 * psi_h - v * ey_f) / (S2 - v);

@@ -209,6 +209,50 @@ __device__ inline void mcl_face(const Planes& P, const Scheme& S, int64_t c, int
   }
 }
 
+// The face states of HLLC with the MC limiter (flux_solver = 4,
+// 16_hllc_flow_mcl.f95:1760-2479, the states and fluxes of hll_flow), as
+// the HLLC solve reads them
+template <int DIR, typename T>
+__device__ inline void mcl_sides(const Planes& P, const Scheme& S, int64_t c, int64_t st, Side<T>& L, Side<T>& R) {
+  McSide<T> l, r;
+  T e0[2], es[2], b0[2], bs[2];
+  mcl_rec(P.s[S_PSI], c, st, l.psi, r.psi);
+  mcl_rec(P.s[S_PHI], c, st, l.phi, r.phi);
+  mcl_rec(P.s[S_EX], c, st, l.ex, r.ex, DIR == 1 ? e0 : (T*)nullptr, DIR == 1 ? es : (T*)nullptr);
+  mcl_rec(P.s[S_EY], c, st, l.ey, r.ey);
+  mcl_rec(P.s[S_EZ], c, st, l.ez, r.ez, DIR == 0 ? e0 : (T*)nullptr, DIR == 0 ? es : (T*)nullptr);
+  mcl_rec(P.s[S_BX], c, st, l.bx, r.bx);
+  mcl_rec(P.s[S_BY], c, st, l.by, r.by, DIR == 0 ? b0 : (T*)nullptr, DIR == 0 ? bs : (T*)nullptr);
+  mcl_rec(P.s[S_BZ], c, st, l.bz, r.bz, DIR == 1 ? b0 : (T*)nullptr, DIR == 1 ? bs : (T*)nullptr);
+  mcl_rec(P.s[S_Q], c, st, l.q, r.q);
+  mcl_rec(P.s[S_D], c, st, l.d, r.d);
+  mcl_rec(P.s[S_TAU], c, st, l.tau, r.tau);
+  mcl_rec(P.s[S_SX], c, st, l.sx, r.sx);
+  mcl_rec(P.s[S_SY], c, st, l.sy, r.sy);
+  mcl_rec(P.s[S_SZ], c, st, l.sz, r.sz);
+  mcl_rec(P.p[P_VX], c, st, l.vx, r.vx);
+  mcl_rec(P.p[P_VY], c, st, l.vy, r.vy);
+  mcl_rec(P.p[P_VZ], c, st, l.vz, r.vz);
+  mcl_rec(P.p[P_P], c, st, l.p, r.p);
+  mcl_rec(P.p[P_RHO], c, st, l.rho, r.rho);
+  Side<T>* sd[2] = {&L, &R};
+  const McSide<T>* ms[2] = {&l, &r};
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    Side<T>& o = *sd[t];
+    const McSide<T>& m = *ms[t];
+    o.psi = m.psi; o.phi = m.phi; o.ex = m.ex; o.ey = m.ey; o.ez = m.ez;
+    o.bx = m.bx; o.by = m.by; o.bz = m.bz; o.q = m.q; o.d = m.d; o.tau = m.tau;
+    o.sx = m.sx; o.sy = m.sy; o.sz = m.sz; o.vx = m.vx; o.vy = m.vy; o.vz = m.vz; o.p = m.p; o.rho = m.rho;
+    mcl_side_fluxes<DIR>((T)S.gamma, (T)S.sigma, m, o.jn, o.fd, o.ftau, o.fs1, o.fs2, o.fs3);
+  }
+  // "_m" states: the cell value negated, the slope kept
+  L.em = (-e0[0]) + es[0];
+  R.em = (-e0[1]) - es[1];
+  L.bm = (-b0[0]) + bs[0];
+  R.bm = (-b0[1]) - bs[1];
+}
+
 // ---------------------------------------------------------------------------
 // Lax-Friedrichs flux splitting (flux_solver = 1, 15_lax_flow.f95:733-1460):
 // at every cell the split fluxes F+- = A +- Sp U (Sp = 1) of the stage fields
@@ -328,12 +372,20 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;
-  if constexpr (SOLVER == 1 || SOLVER == 2) {  // second-order solvers: the two faces are independent
+  if constexpr (SOLVER == 1 || SOLVER == 2 || SOLVER == 4) {  // second-order solvers: the faces are independent
     T f[FX_N > FY_N ? FX_N : FY_N];
     auto face = [&](auto dir) {
       constexpr int D = decltype(dir)::value;
-      if constexpr (SOLVER == 1) lax_face<D>(g, P, S, i, j, f);
-      else mcl_face<D>(P, S, c, D == 0 ? sx : sy, f);
+      if constexpr (SOLVER == 1) {
+        lax_face<D>(g, P, S, i, j, f);
+      } else if constexpr (SOLVER == 2) {
+        mcl_face<D>(P, S, c, D == 0 ? sx : sy, f);
+      } else {
+        Side<T> sl, sr;
+        mcl_sides<D>(P, S, c, D == 0 ? sx : sy, sl, sr);
+        const int code = D == 0 ? hllc_x<T, true>(sl, sr, f, false) : hllc_y<T, true>(sl, sr, f, false);
+        if (code) report(st, code, i, j);
+      }
       if constexpr (D == 0) put_x(f);
       else put_y(f);
     };
@@ -344,7 +396,6 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
       face(std::integral_constant<int, 0>());
       face(std::integral_constant<int, 1>());
     }
-    (void)st;
     return;
   }
   Side<T> xl, xr, yl, yr;
@@ -636,6 +687,7 @@ inline const void* march_kernel(int rec_order, int solver, bool final_stage) {
                : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false>))
   if (solver == 1) return CUEVA_MK(5, 1);  // second order: the MP order does not enter
   if (solver == 2) return CUEVA_MK(5, 2);
+  if (solver == 4) return CUEVA_MK(5, 4);
   switch (rec_order * 10 + solver) {
     case 55: return CUEVA_MK(5, 5);
     case 56: return CUEVA_MK(5, 6);

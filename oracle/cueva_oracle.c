@@ -105,7 +105,7 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   if (imax < 8 || jmax < 8) return 1;
   if (lmax != 2 && lmax != 4) return 2;
   if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL &&
-      flux_solver != OR_LAX)
+      flux_solver != OR_LAX && flux_solver != OR_HLLC_MCL)
     return 3;
   if (rec_order != 5 && rec_order != 7 && rec_order != 9) return 4;
   c->imax = imax;
@@ -234,6 +234,11 @@ void oracle_get_counters(const oracle_state *s, int64_t counters3[3]) {
 
 void oracle_get_rhs(const oracle_state *s, int l, int which, double *out) {
   memcpy(out, s->rhs[l][which], s->n * sizeof(double));
+}
+
+/* the face fluxes of the last face pass: dir 0 = x (FX order), 1 = y */
+void oracle_get_face(const oracle_state *s, int dir, int slot, double *out) {
+  memcpy(out, dir == 0 ? s->fx[slot] : s->fy[slot], s->n * sizeof(double));
 }
 
 /* ------------------------------------------------------------------ */
@@ -715,6 +720,10 @@ static inline double hll_flux(double fl, double fr, double ul, double ur) {
   return 0.5 * ((fl + fr) - (ur - ul));
 }
 
+/* |v*| at or below which the contact speed is set to 0: 1e-9 in
+ * 16_hllc_flow_mpx.f95:3322, 1e-6 in 16_hllc_flow_mcl.f95:2596 */
+static double g_hllc_zero = 1.0e-9;
+
 /* contact speed of 16_hllc_flow_mpx.f95:3185-3334.  v0 is the point of the
  * first Newton evaluation (see oracle_hllc_vel). */
 static double hllc_contact(double a, double b, double c, double v0) {
@@ -723,7 +732,7 @@ static double hllc_contact(double a, double b, double c, double v0) {
   double rt2 = c / qr;
   if (det < 0.0) rt2 = c / b;
   double v = oracle_hllc_vel(a, -b, c, rt2, v0 == v0 ? v0 : clampf(rt2, -1.0, 1.0));
-  if (fabs(v) <= 1.0e-9)
+  if (fabs(v) <= g_hllc_zero)
     v = 0.0;
   else if (fabs(v) > 1.0)
     v = clampf(v, -1.0 + 1.0e-6, 1.0 - 1.0e-6);
@@ -1163,6 +1172,7 @@ static int hlle_xy(double gam, const side_t *xl, const side_t *xr, const side_t 
  * 16_hlle_flow_mpx.f95:2258-4618 */
 static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f);
 static void lax_face(const oracle_state *s, int i, int j, int dir, double *f);
+static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy);
 
 static int face_loop(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
@@ -1170,9 +1180,12 @@ static int face_loop(oracle_state *s) {
   for (int j = -1; j <= jm; j++)
     for (int i = -1; i <= im; i++) {
       size_t k = IX(s, i, j);
-      if (fs == OR_HLL_MCL || fs == OR_LAX) {
+      if (fs == OR_HLL_MCL || fs == OR_LAX || fs == OR_HLLC_MCL) {
         double fx[FX_N], fy[FY_N];
-        if (fs == OR_LAX) {
+        if (fs == OR_HLLC_MCL) {
+          int st = hllc_mcl_faces(s, i, j, fx, fy);
+          if (st) return st;
+        } else if (fs == OR_LAX) {
           lax_face(s, i, j, 0, fx);
           lax_face(s, i, j, 1, fy);
         } else {
@@ -1356,6 +1369,75 @@ static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
   f[FX_FSX] = mcl_lf(fl[3], fr[3], L.sx, R.sx);
   f[FX_FSY] = mcl_lf(fl[4], fr[4], L.sy, R.sy);
   f[FX_FSZ] = mcl_lf(fl[5], fr[5], L.sz, R.sz);
+}
+
+/* ------------------------------------------------------------------ */
+/* HLLC with the MC limiter, flux_solver = 4: hllc_flow,                */
+/* 16_hllc_flow_mcl.f95:1751-4140 (DIM = 2).  The face states and their */
+/* physical fluxes are those of hll_flow (:1760-2479 repeats              */
+/* 16_hll_flow_mcl.f95:563-1258 but for log lines); the Riemann solve is */
+/* that of hllc_flow_mpx statement for statement (:2480-4140) with the   */
+/* "_m" states of the MC reconstruction and |v*| <= 1e-6 -> 0 (:2596).    */
+/* Its get_hllc_vel is an internal copy (:4072-4143) whose first Newton   */
+/* evaluation reads v before setting it, like the module one; the value   */
+/* this build of the reference reads there is not reproduced (no side     */
+/* state, coefficient or earlier iterate matches it at every face), so    */
+/* the solve starts from the documented seed, clamp(rt2): parity with the */
+/* reference is to rounding (about 1e-16 after one step), not bitwise.   */
+/* ------------------------------------------------------------------ */
+static void mcl_sides(const oracle_state *s, int i, int j, int dir, side_t *L, side_t *R) {
+  double *const *S = s->st, *const *P = s->prim;
+  const int di = dir == 0, dj = dir == 1;
+  size_t km = IX(s, i - di, j - dj), k0 = IX(s, i, j), k1 = IX(s, i + di, j + dj),
+         k2 = IX(s, i + 2 * di, j + 2 * dj);
+  memset(L, 0, sizeof(*L));
+  memset(R, 0, sizeof(*R));
+  mcl_rec(S[ST_PSI], km, k0, k1, k2, &L->psi, &R->psi, NULL, NULL);
+  mcl_rec(S[ST_PHI], km, k0, k1, k2, &L->phi, &R->phi, NULL, NULL);
+  mcl_rec(S[ST_EX], km, k0, k1, k2, &L->ex, &R->ex, &L->exm, &R->exm);
+  mcl_rec(S[ST_EY], km, k0, k1, k2, &L->ey, &R->ey, &L->eym, &R->eym);
+  mcl_rec(S[ST_EZ], km, k0, k1, k2, &L->ez, &R->ez, &L->ezm, &R->ezm);
+  mcl_rec(S[ST_BX], km, k0, k1, k2, &L->bx, &R->bx, &L->bxm, &R->bxm);
+  mcl_rec(S[ST_BY], km, k0, k1, k2, &L->by, &R->by, &L->bym, &R->bym);
+  mcl_rec(S[ST_BZ], km, k0, k1, k2, &L->bz, &R->bz, &L->bzm, &R->bzm);
+  mcl_rec(S[ST_Q], km, k0, k1, k2, &L->q, &R->q, NULL, NULL);
+  mcl_rec(S[ST_D], km, k0, k1, k2, &L->d, &R->d, NULL, NULL);
+  mcl_rec(S[ST_TAU], km, k0, k1, k2, &L->tau, &R->tau, NULL, NULL);
+  mcl_rec(S[ST_SX], km, k0, k1, k2, &L->sx, &R->sx, NULL, NULL);
+  mcl_rec(S[ST_SY], km, k0, k1, k2, &L->sy, &R->sy, NULL, NULL);
+  mcl_rec(S[ST_SZ], km, k0, k1, k2, &L->sz, &R->sz, NULL, NULL);
+  mcl_rec(P[OR_VX], km, k0, k1, k2, &L->vx, &R->vx, NULL, NULL);
+  mcl_rec(P[OR_VY], km, k0, k1, k2, &L->vy, &R->vy, NULL, NULL);
+  mcl_rec(P[OR_VZ], km, k0, k1, k2, &L->vz, &R->vz, NULL, NULL);
+  mcl_rec(P[OR_P], km, k0, k1, k2, &L->p, &R->p, NULL, NULL);
+  mcl_rec(P[OR_RHO], km, k0, k1, k2, &L->rho, &R->rho, NULL, NULL);
+  side_t *sd[2] = {L, R};
+  for (int t = 0; t < 2; t++) {
+    mcl_side_t m;
+    side_t *o = sd[t];
+    m.psi = o->psi; m.phi = o->phi; m.ex = o->ex; m.ey = o->ey; m.ez = o->ez;
+    m.bx = o->bx; m.by = o->by; m.bz = o->bz; m.q = o->q; m.d = o->d; m.tau = o->tau;
+    m.sx = o->sx; m.sy = o->sy; m.sz = o->sz; m.vx = o->vx; m.vy = o->vy; m.vz = o->vz;
+    m.p = o->p; m.rho = o->rho;
+    double fl[6];
+    mcl_side_fluxes(s->c.gamma, s->c.sigma, dir, &m, fl);
+    o->j1 = fl[0]; o->fd = fl[1]; o->ftau = fl[2]; o->fs1 = fl[3]; o->fs2 = fl[4]; o->fs3 = fl[5];
+  }
+}
+
+static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy) {
+  side_t xl, xr, yl, yr;
+  mcl_sides(s, i, j, 0, &xl, &xr);
+  mcl_sides(s, i, j, 1, &yl, &yr);
+  const double zero = g_hllc_zero;
+  const int seed = g_hllc_seed_mode;
+  g_hllc_zero = 1.0e-6;
+  g_hllc_seed_mode = 0; /* clamp(rt2) */
+  int st = hllc_x(&xl, &xr, fx);
+  if (!st) st = hllc_y(&yl, &yr, fy);
+  g_hllc_zero = zero;
+  g_hllc_seed_mode = seed;
+  return st;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1686,7 +1768,7 @@ int oracle_step(oracle_state *s, int nsteps) {
       varint(s);
       electricvarint(s, l);
       flowinit(s);
-      if (s->c.flux_solver != OR_HLL_MCL && s->c.flux_solver != OR_LAX) reconstruct(s);
+      if (s->c.flux_solver == OR_HLL || s->c.flux_solver == OR_HLLE || s->c.flux_solver == OR_HLLC) reconstruct(s);
       int st = face_loop(s);
       if (st) return st;
       fluxandsource(s, l);

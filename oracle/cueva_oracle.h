@@ -30,7 +30,7 @@ enum {
 enum { OR_BC_COPY = 0, OR_BC_PERIODIC = 1 };
 
 /* Riemann solvers (parameters.f95:21 flux_solver 5/6/7). */
-enum { OR_LAX = 1, OR_HLL_MCL = 2, OR_HLL = 5, OR_HLLE = 6, OR_HLLC = 7 };
+enum { OR_LAX = 1, OR_HLL_MCL = 2, OR_HLLC_MCL = 4, OR_HLL = 5, OR_HLLE = 6, OR_HLLC = 7 };
 
 typedef struct {
   int32_t imax, jmax;
@@ -69,6 +69,7 @@ int oracle_step(oracle_state *s, int nsteps);
 /* Debug view of the last completed stage l (0-based) of the last step:
  * which = 0..16 stage RHS (14 flux divergences then 3 sources). */
 void oracle_get_rhs(const oracle_state *s, int l, int which, double *out);
+void oracle_get_face(const oracle_state *s, int dir, int slot, double *out);
 
 /* The 37 global diagnostics of 22_writer.f95:818-991 in the reference's
  * order and summation order (i outer, j inner). */

@@ -11,7 +11,7 @@
 # No reference source is written into the repository.
 #
 # usage: build_ref.sh NAME IMAX JMAX ORDER FLUX_SOLVER REC_MP BOUND SIGMA CFL LX LY
-#   FLUX_SOLVER 1=Lax-Friedrichs, 2=HLL+MCL, 5=HLL 6=HLLE 7=HLLC (+MPx);  REC_MP 0=MP5 1=MP7 2=MP9
+#   FLUX_SOLVER 1=Lax-Friedrichs, 2=HLL+MCL, 4=HLLC+MCL, 5=HLL 6=HLLE 7=HLLC (+MPx);  REC_MP 0=MP5 1=MP7 2=MP9
 #   ORDER 2=SSP2(2,2,2) 4=SSP3(4,3,3);  BOUND 1 copy, 2 periodic, 6 periodic-y/copy-x
 set -euo pipefail
 

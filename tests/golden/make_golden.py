@@ -49,6 +49,8 @@ CASES = {
     "kh_hll_mcl_ssp3_32x64": (32, 64, 2, 0, 4, 1.0e3, 5, [1, 5], [1, 5], None),
     "kh_lax_32x64": (32, 64, 1, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None),
     "kh_lax_bz_32x64": (32, 64, 1, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
+    "kh_hllc_mcl_32x64": (32, 64, 4, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None),
+    "kh_hllc_mcl_bz_32x64": (32, 64, 4, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
 }
 SEED = 20261202
 
