@@ -306,7 +306,8 @@ template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, in
 
 // contact speed, 16_hllc_flow_mpx.f95:3185-3334
 template <typename T>
-__device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail, T zero = T(1.0e-9)) {
+__device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail, T zero = T(1.0e-9),
+                                 T* vlast_out = nullptr) {
   T vlast;
   T det = sq(b) - (T(4) * a) * c;
   T qr = -(T(0.5) * (b + csign(b) * fab(sqrt(det))));
@@ -314,6 +315,7 @@ __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail
   if (det < T(0)) rt2 = c / b;
   T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
   T v = hllc_vel(a, -b, c, rt2, v0, fail, vlast);
+  if (vlast_out) *vlast_out = vlast;
   if (fab(v) <= zero)
     v = T(0);
   else if (fab(v) > T(1))
@@ -324,9 +326,13 @@ __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail
 // HLLC x face, 16_hllc_flow_mpx.f95:2811-4295.  Returns 0 or a status.
 // seed_ref: first Newton point as the flang -O2 reference build (R.bz).
 // MC: the MC-limited solver (16_hllc_flow_mcl.f95:2480-4140): its own "_m"
-// states, |v*| <= 1e-6 -> 0, and Newton from the documented seed.
+// states, |v*| <= 1e-6 -> 0, and, when seed_ref, Newton's first point mcref
+// (the reference build's: the y face's (E x B)_y of the HLL states for the
+// x solve, the x solve's last iterate for the y solve); *vlast receives the
+// solve's last iterate.
 template <typename T, bool MC = false>
-__device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref) {
+__device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref, T mcref = T(0),
+                             T* vlast = nullptr) {
   const T S1 = T(-1), S2 = T(1);
   const T ezmL = MC ? L.em : -L.ez, ezmR = MC ? R.em : -R.ez;
   const T bymL = MC ? L.bm : -L.by, bymR = MC ? R.bm : -R.by;
@@ -367,7 +373,7 @@ __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed
   T b = ((e2p + b2p) - tau_h) - sx_f;
   T c = sx_h - erx;
   int fail = 0;
-  T v = MC ? hllc_contact(a, b, c, false, R.bz, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, R.bz, fail);
+  T v = MC ? hllc_contact(a, b, c, seed_ref, mcref, fail, T(1.0e-6), vlast) : hllc_contact(a, b, c, seed_ref, R.bz, fail);
   if (fail || v != v) return 7;
   T pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
   T psiR = (S2 * psi_h - v * ex_f) / (S2 - v);
@@ -429,7 +435,7 @@ __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed
 // HLLC y face, 16_hllc_flow_mpx.f95:2950-3095, 3338-3489, 3536-3574,
 // 3742-3901, 4305-4606.  seed_ref: first Newton point = -b (the b argument).
 template <typename T, bool MC = false>
-__device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref) {
+__device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref, T mcref = T(0)) {
   const T S1 = T(-1), S2 = T(1);
   const T exmL = MC ? L.em : -L.ex, exmR = MC ? R.em : -R.ex;
   const T bzmL = MC ? L.bm : -L.bz, bzmR = MC ? R.bm : -R.bz;
@@ -470,7 +476,7 @@ __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed
   T b = ((e2p + b2p) - tau_h) - sy_f;
   T c = sy_h - ery;
   int fail = 0;
-  T v = MC ? hllc_contact(a, b, c, false, -b, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, -b, fail);
+  T v = MC ? hllc_contact(a, b, c, seed_ref, mcref, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, -b, fail);
   if (fail || v != v) return 7;
   T pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
   T psiR = (S2 * psi_h - v * ey_f) / (S2 - v);
@@ -572,10 +578,10 @@ template <typename T> __device__ inline T hlle_flux(T a, T b, T fl, T fr, T ul, 
 
 // (E x B)_y of the y face's HLL states: the first Newton point of the x
 // contact solve in the flang -O2 reference build (still in the register)
-template <typename T> __device__ inline T hlle_y_ery(const Side<T>& L, const Side<T>& R) {
+template <typename T, bool MC = false> __device__ inline T hlle_y_ery(const Side<T>& L, const Side<T>& R) {
   T ez_h = hll_state(L.ez, R.ez, L.bx, R.bx);
-  T ex_h = hll_state(L.ex, R.ex, -L.bz, -R.bz);
-  T bz_h = hll_state(L.bz, R.bz, -L.ex, -R.ex);
+  T ex_h = hll_state(L.ex, R.ex, MC ? L.bm : -L.bz, MC ? R.bm : -R.bz);
+  T bz_h = hll_state(L.bz, R.bz, MC ? L.em : -L.ex, MC ? R.em : -R.ex);
   T bx_h = hll_state(L.bx, R.bx, L.ez, R.ez);
   return bx_h * ez_h - bz_h * ex_h;
 }

@@ -372,20 +372,31 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;
-  if constexpr (SOLVER == 1 || SOLVER == 2 || SOLVER == 4) {  // second-order solvers: the faces are independent
+  if constexpr (SOLVER == 4) {  // HLLC + MC limiter: x solve first (the y solve starts from its last iterate)
+    T f[FX_N > FY_N ? FX_N : FY_N];
+    const bool seed = S.hllc_seed_ref != 0;
+    T xseed = T(0), vlast = T(0);
+    Side<T> sl, sr;
+    if (seed) {
+      mcl_sides<1>(P, S, c, sy, sl, sr);
+      xseed = hlle_y_ery<T, true>(sl, sr);
+    }
+    mcl_sides<0>(P, S, c, sx, sl, sr);
+    int code = hllc_x<T, true>(sl, sr, f, seed, xseed, &vlast);
+    if (code) report(st, code, i, j);
+    put_x(f);
+    mcl_sides<1>(P, S, c, sy, sl, sr);
+    code = hllc_y<T, true>(sl, sr, f, seed, vlast);
+    if (code) report(st, code, i, j);
+    put_y(f);
+    return;
+  }
+  if constexpr (SOLVER == 1 || SOLVER == 2) {  // second-order solvers: the faces are independent
     T f[FX_N > FY_N ? FX_N : FY_N];
     auto face = [&](auto dir) {
       constexpr int D = decltype(dir)::value;
-      if constexpr (SOLVER == 1) {
-        lax_face<D>(g, P, S, i, j, f);
-      } else if constexpr (SOLVER == 2) {
-        mcl_face<D>(P, S, c, D == 0 ? sx : sy, f);
-      } else {
-        Side<T> sl, sr;
-        mcl_sides<D>(P, S, c, D == 0 ? sx : sy, sl, sr);
-        const int code = D == 0 ? hllc_x<T, true>(sl, sr, f, false) : hllc_y<T, true>(sl, sr, f, false);
-        if (code) report(st, code, i, j);
-      }
+      if constexpr (SOLVER == 1) lax_face<D>(g, P, S, i, j, f);
+      else mcl_face<D>(P, S, c, D == 0 ? sx : sy, f);
       if constexpr (D == 0) put_x(f);
       else put_y(f);
     };

@@ -693,6 +693,10 @@ static inline double clampf(double x, double lo, double hi) {
 
 /* last Newton iterate of the latest hllc_vel call, before the final clamp */
 static double g_hllc_last_iterate = 0.0;
+/* HLLC with the MC limiter: first Newton points of its x and y solves (see
+ * hllc_mcl_faces); NAN when not in use */
+static double g_mcl_seed_x = NAN;
+static int g_mcl_seed_y = 0;
 
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
   double oldv = clampf(rt, -1.0, 1.0);
@@ -786,7 +790,7 @@ static int hllc_x(const side_t *L, const side_t *R, double f[FX_N]) {
   double a = tau_f - erx;
   double b = ((e2p + b2p) - tau_h) - sx_f;
   double c = sx_h - erx;
-  double v = hllc_contact(a, b, c, g_hllc_seed_mode ? R->bz : NAN);
+  double v = hllc_contact(a, b, c, g_mcl_seed_x == g_mcl_seed_x ? g_mcl_seed_x : (g_hllc_seed_mode ? R->bz : NAN));
   if (v != v) return 7;
 
   double pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
@@ -892,7 +896,7 @@ static int hllc_y(const side_t *L, const side_t *R, double f[FY_N]) {
   double a = tau_f - ery;
   double b = ((e2p + b2p) - tau_h) - sy_f;
   double c = sy_h - ery;
-  double v = hllc_contact(a, b, c, g_hllc_seed_mode ? -b : NAN);
+  double v = hllc_contact(a, b, c, g_mcl_seed_y ? g_hllc_last_iterate : (g_hllc_seed_mode ? -b : NAN));
   if (v != v) return 7;
 
   double pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
@@ -1379,11 +1383,11 @@ static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
 /* that of hllc_flow_mpx statement for statement (:2480-4140) with the   */
 /* "_m" states of the MC reconstruction and |v*| <= 1e-6 -> 0 (:2596).    */
 /* Its get_hllc_vel is an internal copy (:4072-4143) whose first Newton   */
-/* evaluation reads v before setting it, like the module one; the value   */
-/* this build of the reference reads there is not reproduced (no side     */
-/* state, coefficient or earlier iterate matches it at every face), so    */
-/* the solve starts from the documented seed, clamp(rt2): parity with the */
-/* reference is to rounding (about 1e-16 after one step), not bitwise.   */
+/* evaluation reads v before setting it (:4113), like the module one: in  */
+/* the flang -O2 build that is the register the caller last wrote, which */
+/* for the x solve holds the y face's (E x B)_y of the HLL states and for */
+/* the y solve the x solve's last iterate (read from the assembly of the  */
+/* binary built here; hllc_seed = 0 starts both from clamp(rt2) instead). */
 /* ------------------------------------------------------------------ */
 static void mcl_sides(const oracle_state *s, int i, int j, int dir, side_t *L, side_t *R) {
   double *const *S = s->st, *const *P = s->prim;
@@ -1425,6 +1429,16 @@ static void mcl_sides(const oracle_state *s, int i, int j, int dir, side_t *L, s
   }
 }
 
+/* (E x B)_y of the y face's HLL states (16_hllc_flow_mcl.f95:2601-2609,
+ * :2774), the x solve's first Newton point in the reference build */
+static double mcl_y_ery(const side_t *L, const side_t *R) {
+  double ez = hll_state(L->ez, R->ez, L->bx, R->bx);
+  double ex = hll_state(L->ex, R->ex, L->bzm, R->bzm);
+  double bz = hll_state(L->bz, R->bz, L->exm, R->exm);
+  double bx = hll_state(L->bx, R->bx, L->ez, R->ez);
+  return bx * ez - bz * ex;
+}
+
 static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy) {
   side_t xl, xr, yl, yr;
   mcl_sides(s, i, j, 0, &xl, &xr);
@@ -1432,11 +1446,17 @@ static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, doubl
   const double zero = g_hllc_zero;
   const int seed = g_hllc_seed_mode;
   g_hllc_zero = 1.0e-6;
-  g_hllc_seed_mode = 0; /* clamp(rt2) */
+  if (seed) {
+    g_mcl_seed_x = mcl_y_ery(&yl, &yr);
+    g_mcl_seed_y = 1;
+  }
+  g_hllc_seed_mode = 0; /* otherwise clamp(rt2) */
   int st = hllc_x(&xl, &xr, fx);
   if (!st) st = hllc_y(&yl, &yr, fy);
   g_hllc_zero = zero;
   g_hllc_seed_mode = seed;
+  g_mcl_seed_x = NAN;
+  g_mcl_seed_y = 0;
   return st;
 }
 
