@@ -14,7 +14,7 @@ for p in (PKG, ROOT):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: longer CPU test")
-    # build what is missing (the GPU box receives these prebuilt)
+    # build what is missing (a tree copied with its built libraries skips this)
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
     if not os.path.exists(os.path.join(PKG, "lib", "libcueva.so")):
