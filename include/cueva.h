@@ -58,7 +58,7 @@ enum { CUEVA_BC_COPY = 0, CUEVA_BC_PERIODIC = 1 };
  * slope limiter (16_hll_flow_mcl.f95); 4 = HLLC with the same limiter
  * (16_hllc_flow_mcl.f95); 1 = Lax-Friedrichs flux splitting with the same
  * limiter (15_lax_flow.f95). */
-enum { CUEVA_LAX = 1, CUEVA_HLL_MCL = 2, CUEVA_HLLC_MCL = 4, CUEVA_HLL = 5, CUEVA_HLLE = 6, CUEVA_HLLC = 7 };
+enum { CUEVA_LAX = 1, CUEVA_HLL_MCL = 2, CUEVA_HLLE_MCL = 3, CUEVA_HLLC_MCL = 4, CUEVA_HLL = 5, CUEVA_HLLE = 6, CUEVA_HLLC = 7 };
 
 /* Arithmetic of the fluxes: plain FP64, or FP32 reconstruction/Riemann/flux
  * difference with FP64 primitive recovery, E solve and sums. */
@@ -77,7 +77,7 @@ enum { CUEVA_BND_PRIM = 0, CUEVA_BND_CONS = 1, CUEVA_BND_ELEC = 2, CUEVA_BND_FLU
 typedef struct cueva_params {
   int32_t imax, jmax;     /* cells per axis (parameters.f95:110-111)           */
   int32_t order;          /* 2 = IMEX SSP2(2,2,2), 4 = SSP3(4,3,3) (:10)        */
-  int32_t flux_solver;    /* CUEVA_HLL/_HLLE/_HLLC/_HLL_MCL/_HLLC_MCL/_LAX (:21) */
+  int32_t flux_solver;    /* CUEVA_HLL/_HLLE/_HLLC/_HLL_MCL/_HLLE_MCL/_HLLC_MCL/_LAX (:21) */
   int32_t rec_mp;         /* 0 MP5, 1 MP7, 2 MP9 (:33)                          */
   int32_t bc_x, bc_y;     /* CUEVA_BC_* per axis                                */
   int32_t precision;      /* CUEVA_FP64 / CUEVA_MIXED                           */
@@ -134,10 +134,10 @@ int cueva_sumint(cueva_ctx *ctx, int l, int m);    /* 09_sumint.f95:5          *
 int cueva_varint(cueva_ctx *ctx, int l);           /* 10_varint.f95:5          */
 int cueva_electricvarint(cueva_ctx *ctx, int l);   /* 11_elecvarint.f95:3      */
 int cueva_flowinit(cueva_ctx *ctx, int l);         /* 14_flowinit.f95          */
-/* hll_flow_mpx / hlle_flow_mpx / hllc_flow_mpx / hll_flow / hllc_flow /
- * lax_flow per params.flux_solver (16_hll_flow_mpx.f95, 16_hlle_flow_mpx.f95,
- * 16_hllc_flow_mpx.f95, 16_hll_flow_mcl.f95:5, 16_hllc_flow_mcl.f95,
- * 15_lax_flow.f95:5). */
+/* hll_flow_mpx / hlle_flow_mpx / hllc_flow_mpx / hll_flow / hlle_flow /
+ * hllc_flow / lax_flow per params.flux_solver (16_hll_flow_mpx.f95,
+ * 16_hlle_flow_mpx.f95, 16_hllc_flow_mpx.f95, 16_hll_flow_mcl.f95:5,
+ * 16_hlle_flow_mcl.f95:1638, 16_hllc_flow_mcl.f95, 15_lax_flow.f95:5). */
 int cueva_flux(cueva_ctx *ctx, int l);
 int cueva_fluxandsource(cueva_ctx *ctx, int l);    /* 17_fluxandsource.f95:6   */
 int cueva_sumfin(cueva_ctx *ctx, int l);           /* 18_sumfin.f95:5          */

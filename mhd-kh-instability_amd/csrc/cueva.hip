@@ -171,7 +171,8 @@ static int check_params(const cueva_params* p) {
   if (p->imax < 8 || p->jmax < 8 || p->imax > 65536 || p->jmax > 65536) return CUEVA_E_ARG;
   if (p->order != 2 && p->order != 4) return CUEVA_E_ARG;
   if (p->flux_solver != CUEVA_HLL && p->flux_solver != CUEVA_HLLC && p->flux_solver != CUEVA_HLLE &&
-      p->flux_solver != CUEVA_HLL_MCL && p->flux_solver != CUEVA_LAX && p->flux_solver != CUEVA_HLLC_MCL)
+      p->flux_solver != CUEVA_HLL_MCL && p->flux_solver != CUEVA_LAX && p->flux_solver != CUEVA_HLLC_MCL &&
+      p->flux_solver != CUEVA_HLLE_MCL)
     return CUEVA_E_ARG;
   if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
@@ -458,6 +459,8 @@ static int launch_faces_ord(cueva_ctx* c) {
     LAUNCH(K_FACES, (k_faces<ORD, 1>), grid, BLK, c->g, c->P, c->S, c->dst);
   else if (c->prm.flux_solver == CUEVA_HLLC_MCL)
     LAUNCH(K_FACES, (k_faces<ORD, 4>), grid, BLK, c->g, c->P, c->S, c->dst);
+  else if (c->prm.flux_solver == CUEVA_HLLE_MCL)
+    LAUNCH(K_FACES, (k_faces<ORD, 3>), grid, BLK, c->g, c->P, c->S, c->dst);
   else if (c->prm.flux_solver == CUEVA_HLL_MCL)
     LAUNCH(K_FACES, (k_faces<ORD, 2>), grid, BLK, c->g, c->P, c->S, c->dst);
   else if (c->prm.flux_solver == CUEVA_HLLC)
@@ -478,7 +481,7 @@ static int launch_faces(cueva_ctx* c) {
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
   if (c->prm.flux_solver == CUEVA_HLL_MCL || c->prm.flux_solver == CUEVA_LAX ||
-      c->prm.flux_solver == CUEVA_HLLC_MCL)
+      c->prm.flux_solver == CUEVA_HLLC_MCL || c->prm.flux_solver == CUEVA_HLLE_MCL)
     return launch_faces_ord<5>(c);  // the MP order does not enter
   switch (c->rec_order) {
     case 5: return launch_faces_ord<5>(c);

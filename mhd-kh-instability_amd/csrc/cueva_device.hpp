@@ -284,8 +284,10 @@ template <typename T> __device__ inline void hll_y(const Side<T>& L, const Side<
 // [-1,1].  The reference evaluates its first step at the iterate before
 // assigning it; v0 is that point (clamp(rt) in the seed mode).  Tolerances
 // are the reference's for double and scaled to the precision for float.
-// vlast receives the last iterate before the final clamp.
-template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, int& fail, T& vlast) {
+// vlast receives the last iterate before the final clamp.  maxiter: 100
+// (funciones.f95), 200 in the copy inside 16_hlle_flow_mcl.f95:4385.
+template <typename T>
+__device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, int& fail, T& vlast, int maxiter = 100) {
   const T minerr = sizeof(T) == 8 ? T(1.0e-12) : T(2.0e-6);
   T oldv = clampf(rt, T(-1), T(1));
   T v = v0;
@@ -297,7 +299,7 @@ template <typename T> __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, in
     T err = fab(oldv - v);
     oldv = v;
     iter = iter + 1;
-    if (iter >= 100) { fail = 1; vlast = v; return v; }
+    if (iter >= maxiter) { fail = 1; vlast = v; return v; }
     if (!(err > minerr)) break;
   }
   vlast = v;
@@ -707,6 +709,161 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
       return 8;
     }
   }
+  return 0;
+}
+
+// One face of HLLE with the MC limiter (flux_solver 3, hlle_flow of
+// 16_hlle_flow_mcl.f95, DIM = 2), from MC-limited sides with their physical
+// fluxes (mcl_sides).  DIR 0: x face (:2374-2388 EM HLL states, :2417-2455
+// EM HLL fluxes, :2510-2665 wave speeds and fluid HLL states and fluxes,
+// :2924-3056 contact, :3174-3212 star states, :3256-3474 star fluxes,
+// :3654-4034 selection); DIR 1: y face (:2395-2409, 2463-2501, 2673-2831,
+// 3064-3166, 3178, 3219-3243, 3478-3643, 4044-4340).  Unlike hlle_face:
+//  - the normal star momentum carries the star pressure and the star tau
+//    is written from the HLL tau state;
+//  - fluid star fluxes are F_K + s_K (U*_K - U_K) with the sides' own
+//    physical fluxes F_K, which also leave the fan when s1 > 0 or s2 < 0;
+//  - the EM star "fluxes" are the EM HLL states (the HLL fluxes when
+//    v* = 0);
+//  - Newton allows 200 iterations.
+// vref: first Newton point when seed_ref; vlast: the solve's last iterate.
+template <int DIR, typename T>
+__device__ inline int hlle_mcl_face(T gam, Side<T>& L, Side<T>& R, T* f, bool seed_ref, T vref, T& vlast) {
+  T e[3], b[3], psi_h, phi_h;
+  if constexpr (DIR == 0) {
+    psi_h = hll_state(L.psi, R.psi, L.ex, R.ex);
+    phi_h = hll_state(L.phi, R.phi, L.bx, R.bx);
+    e[0] = hll_state(L.ex, R.ex, L.psi, R.psi);
+    e[1] = hll_state(L.ey, R.ey, L.bz, R.bz);
+    e[2] = hll_state(L.ez, R.ez, L.bm, R.bm);
+    b[0] = hll_state(L.bx, R.bx, L.phi, R.phi);
+    b[1] = hll_state(L.by, R.by, L.em, R.em);
+    b[2] = hll_state(L.bz, R.bz, L.ey, R.ey);
+  } else {
+    psi_h = hll_state(L.psi, R.psi, L.ey, R.ey);
+    phi_h = hll_state(L.phi, R.phi, L.by, R.by);
+    e[1] = hll_state(L.ey, R.ey, L.psi, R.psi);
+    e[2] = hll_state(L.ez, R.ez, L.bx, R.bx);
+    e[0] = hll_state(L.ex, R.ex, L.bm, R.bm);
+    b[1] = hll_state(L.by, R.by, L.phi, R.phi);
+    b[2] = hll_state(L.bz, R.bz, L.em, R.em);
+    b[0] = hll_state(L.bx, R.bx, L.ez, R.ez);
+  }
+  // the sound speeds read the sides' enthalpy (:2243-2256)
+  L.h = L.rho * (T(1) + L.p / ((gam - T(1)) * L.rho)) + L.p;
+  R.h = R.rho * (T(1) + R.p / ((gam - T(1)) * R.rho)) + R.p;
+  T sa, sb;
+  int code = hlle_speeds<DIR>(gam, L, R, sa, sb);
+  if (code) return code;
+  constexpr int n = DIR, t1 = DIR == 0 ? 1 : 2, t2 = DIR == 0 ? 2 : 0;
+  T q_f = hlle_flux(sa, sb, L.jn, R.jn, L.q, R.q);
+  T dd_h = hlle_state(sa, sb, L.d, R.d, L.fd, R.fd);
+  T tau_h = hlle_state(sa, sb, L.tau, R.tau, L.ftau, R.ftau);
+  T s_h[3], s_f[3];
+  s_h[0] = hlle_state(sa, sb, L.sx, R.sx, L.fs1, R.fs1);
+  s_h[1] = hlle_state(sa, sb, L.sy, R.sy, L.fs2, R.fs2);
+  s_h[2] = hlle_state(sa, sb, L.sz, R.sz, L.fs3, R.fs3);
+  T dd_f = hlle_flux(sa, sb, L.fd, R.fd, L.d, R.d);
+  T tau_f = hlle_flux(sa, sb, L.ftau, R.ftau, L.tau, R.tau);
+  s_f[0] = hlle_flux(sa, sb, L.fs1, R.fs1, L.sx, R.sx);
+  s_f[1] = hlle_flux(sa, sb, L.fs2, R.fs2, L.sy, R.sy);
+  s_f[2] = hlle_flux(sa, sb, L.fs3, R.fs3, L.sz, R.sz);
+  T er[3];
+  er[0] = b[2] * e[1] - b[1] * e[2];
+  er[1] = b[0] * e[2] - b[2] * e[0];
+  er[2] = b[1] * e[0] - b[0] * e[1];
+  T e2b2s = T(0.5) * (((sq(e[0]) + sq(e[1])) + sq(e[2])) + ((sq(b[0]) + sq(b[1])) + sq(b[2])));
+  constexpr int p1 = DIR == 0 ? 1 : 0;  // the transverse pair is (y,z) or (x,z)
+  T ca = tau_f - er[n];
+  T cb = (((sq(e[p1]) + sq(e[2])) + (sq(b[p1]) + sq(b[2]))) - tau_h) - s_f[n];
+  T cc = s_h[n] - er[n];
+  int fail = 0;
+  T v;
+  {
+    T det = sq(cb) - (T(4) * ca) * cc;
+    T qr = -(T(0.5) * (cb + csign(cb) * fab(sqrt(det))));
+    T rt2 = cc / qr;
+    if (det < T(0)) rt2 = cc / cb;
+    T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
+    v = hllc_vel(ca, -cb, cc, rt2, v0, fail, vlast, 200);
+    if (fab(v) <= T(1.0e-9))
+      v = T(0);
+    else if (fab(v) > T(1))
+      v = clampf(v, T(-1) + T(1.0e-6), T(1) - T(1.0e-6));
+  }
+  if (fail || v != v) return 7;
+  // EM face fluxes: the HLL states (their HLL fluxes when v* = 0)
+  if (v == T(0)) {
+    if constexpr (DIR == 0) {
+      f[FX_EXPSI] = hll_flux(L.ex, R.ex, L.psi, R.psi);
+      f[FX_BXPHI] = hll_flux(L.bx, R.bx, L.phi, R.phi);
+      f[FX_PHIBX] = hll_flux(L.phi, R.phi, L.bx, R.bx);
+      f[FX_EZBY] = -hll_flux(L.em, R.em, L.by, R.by);
+      f[FX_EYBZ] = hll_flux(L.ey, R.ey, L.bz, R.bz);
+      f[FX_PSIEX] = hll_flux(L.psi, R.psi, L.ex, R.ex);
+      f[FX_BZEY] = hll_flux(L.bz, R.bz, L.ey, R.ey);
+      f[FX_BYEZ] = -hll_flux(L.bm, R.bm, L.ez, R.ez);
+    } else {
+      f[FY_EYPSI] = hll_flux(L.ey, R.ey, L.psi, R.psi);
+      f[FY_BYPHI] = hll_flux(L.by, R.by, L.phi, R.phi);
+      f[FY_EZBX] = hll_flux(L.ez, R.ez, L.bx, R.bx);
+      f[FY_PHIBY] = hll_flux(L.phi, R.phi, L.by, R.by);
+      f[FY_EXBZ] = -hll_flux(L.em, R.em, L.bz, R.bz);
+      f[FY_BZEX] = -hll_flux(L.bm, R.bm, L.ex, R.ex);
+      f[FY_PSIEY] = hll_flux(L.psi, R.psi, L.ey, R.ey);
+      f[FY_BXEZ] = hll_flux(L.bx, R.bx, L.ez, R.ez);
+    }
+  } else if constexpr (DIR == 0) {
+    f[FX_EXPSI] = e[0]; f[FX_BXPHI] = b[0]; f[FX_PHIBX] = phi_h; f[FX_EZBY] = e[2];
+    f[FX_EYBZ] = e[1]; f[FX_PSIEX] = psi_h; f[FX_BZEY] = b[2]; f[FX_BYEZ] = b[1];
+  } else {
+    f[FY_EYPSI] = e[1]; f[FY_BYPHI] = b[1]; f[FY_EZBX] = e[2]; f[FY_PHIBY] = phi_h;
+    f[FY_EXBZ] = e[0]; f[FY_BZEX] = b[2]; f[FY_PSIEY] = psi_h; f[FY_BXEZ] = b[0];
+  }
+  // fluid: flux slots J, FD, Ftau, FSx, FSy, FSz are contiguous in both enums
+  T* o = f + (DIR == 0 ? FX_J : FY_J);
+  bool left;
+  if (sa > T(0)) {
+    o[0] = L.jn; o[1] = L.fd; o[2] = L.ftau; o[3] = L.fs1; o[4] = L.fs2; o[5] = L.fs3;
+    return 0;
+  } else if (sa <= T(0) && T(0) <= v) {
+    left = true;
+  } else if (v < T(0) && T(0) <= sb) {
+    left = false;
+  } else if (sb < T(0)) {
+    o[0] = R.jn; o[1] = R.fd; o[2] = R.ftau; o[3] = R.fs1; o[4] = R.fs2; o[5] = R.fs3;
+    return 0;
+  } else {
+    return 8;
+  }
+  o[0] = q_f;
+  if (v == T(0)) {
+    o[1] = dd_f; o[2] = tau_f; o[3] = s_f[0]; o[4] = s_f[1]; o[5] = s_f[2];
+    return 0;
+  }
+  // the star state of the chosen side and its flux F_K + s_K (U*_K - U_K)
+  const Side<T>& K = left ? L : R;
+  const T sk = left ? sa : sb;
+  const T den = left ? v - sa : sb - v;
+  T pstr = (s_f[n] + (sq(e[n]) + sq(b[n]))) - (tau_f - er[n]) * v;
+  T ss[3];
+  if (left) {
+    ss[n] = (((((-(sa * s_h[n]) + s_f[n]) - pstr) + sq(b[n])) + sq(e[n])) + er[n] * v) / den;
+    ss[t1] = ((((-(sa * s_h[t1]) + s_f[t1]) + er[t1] * v) + e[n] * e[t1]) + b[n] * b[t1]) / den;
+    ss[t2] = ((((-(sa * s_h[t2]) + s_f[t2]) + er[t2] * v) + e[n] * e[t2]) + b[n] * b[t2]) / den;
+  } else {
+    ss[n] = (((((sb * s_h[n] - s_f[n]) + pstr) - sq(b[n])) - sq(e[n])) - er[n] * v) / den;
+    ss[t1] = ((((sb * s_h[t1] - s_f[t1]) - er[t1] * v) - e[n] * e[t1]) - b[n] * b[t1]) / den;
+    ss[t2] = ((((sb * s_h[t2] - s_f[t2]) - er[t2] * v) - e[n] * e[t2]) - b[n] * b[t2]) / den;
+  }
+  T taus = left ? ((((-(sa * tau_h)) + tau_f) - (pstr - e2b2s) * v) - er[n]) / den
+                : (((sb * tau_h - tau_f) + (pstr - e2b2s) * v) + er[n]) / den;
+  T dds = left ? (-(sa * dd_h) + dd_f) / den : (sb * dd_h - dd_f) / den;
+  o[1] = K.fd + sk * (dds - K.d);
+  o[2] = K.ftau + sk * (taus - K.tau);
+  o[3] = K.fs1 + sk * (ss[0] - K.sx);
+  o[4] = K.fs2 + sk * (ss[1] - K.sy);
+  o[5] = K.fs3 + sk * (ss[2] - K.sz);
   return 0;
 }
 

@@ -372,7 +372,8 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
   const int64_t sx = 1, sy = g.pitch;
-  if constexpr (SOLVER == 4) {  // HLLC + MC limiter: x solve first (the y solve starts from its last iterate)
+  if constexpr (SOLVER == 3 || SOLVER == 4) {
+    // HLLE / HLLC + MC limiter: x solve first (the y solve starts from its last iterate)
     T f[FX_N > FY_N ? FX_N : FY_N];
     const bool seed = S.hllc_seed_ref != 0;
     T xseed = T(0), vlast = T(0);
@@ -382,11 +383,14 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
       xseed = hlle_y_ery<T, true>(sl, sr);
     }
     mcl_sides<0>(P, S, c, sx, sl, sr);
-    int code = hllc_x<T, true>(sl, sr, f, seed, xseed, &vlast);
+    int code = SOLVER == 3 ? hlle_mcl_face<0>((T)S.gamma, sl, sr, f, seed, xseed, vlast)
+                           : hllc_x<T, true>(sl, sr, f, seed, xseed, &vlast);
     if (code) report(st, code, i, j);
     put_x(f);
     mcl_sides<1>(P, S, c, sy, sl, sr);
-    code = hllc_y<T, true>(sl, sr, f, seed, vlast);
+    T unused;
+    code = SOLVER == 3 ? hlle_mcl_face<1>((T)S.gamma, sl, sr, f, seed, vlast, unused)
+                       : hllc_y<T, true>(sl, sr, f, seed, vlast);
     if (code) report(st, code, i, j);
     put_y(f);
     return;
@@ -698,6 +702,7 @@ inline const void* march_kernel(int rec_order, int solver, bool final_stage) {
                : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false>))
   if (solver == 1) return CUEVA_MK(5, 1);  // second order: the MP order does not enter
   if (solver == 2) return CUEVA_MK(5, 2);
+  if (solver == 3) return CUEVA_MK(5, 3);
   if (solver == 4) return CUEVA_MK(5, 4);
   switch (rec_order * 10 + solver) {
     case 55: return CUEVA_MK(5, 5);

@@ -4,7 +4,7 @@ Host-side Python mirror of the reference's per-stage subroutine interface
 (main_cueva_code.f95:47-356) over the C ABI of include/cueva.h.
 """
 from .abi import (  # noqa: F401
-    BC_COPY, BC_PERIODIC, C2P_CARDANO, C2P_LEARNED, CONS_NAMES, FP64, GLOBAL_NAMES, HLL, HLL_MCL, HLLC, HLLC_MCL, HLLE, LAX, MIXED,
+    BC_COPY, BC_PERIODIC, C2P_CARDANO, C2P_LEARNED, CONS_NAMES, FP64, GLOBAL_NAMES, HLL, HLL_MCL, HLLC, HLLC_MCL, HLLE, HLLE_MCL, LAX, MIXED,
     MLP_NPACK, MLP_NPARAM, PRIM_NAMES, CuevaError, Mlp, Params, Solver, default_params, derived, kh_init, load,
     plane_shape,
 )

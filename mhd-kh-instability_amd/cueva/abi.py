@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libcueva.so")
 HLL, HLLE, HLLC = 5, 6, 7
 HLL_MCL = 2  # HLL with the second-order monotonised-central limiter (16_hll_flow_mcl.f95)
 LAX = 1  # Lax-Friedrichs flux splitting with the same limiter (15_lax_flow.f95)
+HLLE_MCL = 3  # HLLE with the same limiter (16_hlle_flow_mcl.f95)
 HLLC_MCL = 4  # HLLC with the same limiter (16_hllc_flow_mcl.f95)
 # output order of cueva_reduce_globals (the reference's units 80..217)
 GLOBAL_NAMES = ("sum_psi", "sum_phi", "sum_ex", "sum_ey", "sum_ez", "sum_exey", "sum_exez", "sum_eyez",

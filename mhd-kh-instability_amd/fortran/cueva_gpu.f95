@@ -10,8 +10,8 @@ module cueva_gpu
 
   integer(c_int), parameter :: CUEVA_OK = 0
   integer(c_int), parameter :: CUEVA_BC_COPY = 0, CUEVA_BC_PERIODIC = 1
-  integer(c_int), parameter :: CUEVA_LAX = 1, CUEVA_HLL_MCL = 2, CUEVA_HLLC_MCL = 4, CUEVA_HLL = 5, &
-                               CUEVA_HLLE = 6, CUEVA_HLLC = 7
+  integer(c_int), parameter :: CUEVA_LAX = 1, CUEVA_HLL_MCL = 2, CUEVA_HLLE_MCL = 3, CUEVA_HLLC_MCL = 4, &
+                               CUEVA_HLL = 5, CUEVA_HLLE = 6, CUEVA_HLLC = 7
   integer(c_int), parameter :: CUEVA_BND_PRIM = 0, CUEVA_BND_CONS = 1, CUEVA_BND_ELEC = 2, CUEVA_BND_FLUX = 3
   integer(c_int), parameter :: CUEVA_N_GLOBALS = 37
 

@@ -67,6 +67,10 @@ contains
     call check(cueva_flux(ctx, int(l, c_int)), "lax_flow")
   end subroutine lax_flow
 
+  subroutine hlle_flow                ! 16_hlle_flow_mcl.f95 (HLLE + MCL)
+    call check(cueva_flux(ctx, int(l, c_int)), "hlle_flow")
+  end subroutine hlle_flow
+
   subroutine hllc_flow                ! 16_hllc_flow_mcl.f95 (HLLC + MCL)
     call check(cueva_flux(ctx, int(l, c_int)), "hllc_flow")
   end subroutine hllc_flow

@@ -126,6 +126,8 @@ program main_cueva_gpu
         call lax_flow
       else if (prm%flux_solver == CUEVA_HLL_MCL) then
         call hll_flow
+      else if (prm%flux_solver == CUEVA_HLLE_MCL) then
+        call hlle_flow
       else if (prm%flux_solver == CUEVA_HLLC_MCL) then
         call hllc_flow
       else if (prm%flux_solver == CUEVA_HLL) then

@@ -105,7 +105,7 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   if (imax < 8 || jmax < 8) return 1;
   if (lmax != 2 && lmax != 4) return 2;
   if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL &&
-      flux_solver != OR_LAX && flux_solver != OR_HLLC_MCL)
+      flux_solver != OR_LAX && flux_solver != OR_HLLC_MCL && flux_solver != OR_HLLE_MCL)
     return 3;
   if (rec_order != 5 && rec_order != 7 && rec_order != 9) return 4;
   c->imax = imax;
@@ -698,6 +698,8 @@ static double g_hllc_last_iterate = 0.0;
 static double g_mcl_seed_x = NAN;
 static int g_mcl_seed_y = 0;
 
+static int g_hllc_maxiter = 100; /* 200 in 16_hlle_flow_mcl.f95:4385 */
+
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
   double oldv = clampf(rt, -1.0, 1.0);
   double v = v0;
@@ -709,7 +711,7 @@ double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
     double err = fabs(oldv - v);
     oldv = v;
     iter = iter + 1;
-    if (iter >= 100) return NAN; /* the reference stops here */
+    if (iter >= g_hllc_maxiter) return NAN; /* the reference stops here */
     if (!(err > 1.0e-12)) break;
   }
   g_hllc_last_iterate = v;
@@ -1177,6 +1179,7 @@ static int hlle_xy(double gam, const side_t *xl, const side_t *xr, const side_t 
 static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f);
 static void lax_face(const oracle_state *s, int i, int j, int dir, double *f);
 static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy);
+static int hlle_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy);
 
 static int face_loop(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
@@ -1184,10 +1187,10 @@ static int face_loop(oracle_state *s) {
   for (int j = -1; j <= jm; j++)
     for (int i = -1; i <= im; i++) {
       size_t k = IX(s, i, j);
-      if (fs == OR_HLL_MCL || fs == OR_LAX || fs == OR_HLLC_MCL) {
+      if (fs == OR_HLL_MCL || fs == OR_LAX || fs == OR_HLLC_MCL || fs == OR_HLLE_MCL) {
         double fx[FX_N], fy[FY_N];
-        if (fs == OR_HLLC_MCL) {
-          int st = hllc_mcl_faces(s, i, j, fx, fy);
+        if (fs == OR_HLLC_MCL || fs == OR_HLLE_MCL) {
+          int st = fs == OR_HLLC_MCL ? hllc_mcl_faces(s, i, j, fx, fy) : hlle_mcl_faces(s, i, j, fx, fy);
           if (st) return st;
         } else if (fs == OR_LAX) {
           lax_face(s, i, j, 0, fx);
@@ -1457,6 +1460,190 @@ static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, doubl
   g_hllc_seed_mode = seed;
   g_mcl_seed_x = NAN;
   g_mcl_seed_y = 0;
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* HLLE with the MC limiter, flux_solver = 3: hlle_flow,                */
+/* 16_hlle_flow_mcl.f95:1638-4348 (DIM = 2).  The face states and their */
+/* physical fluxes are those of hll_flow (:1647-2373); the wave speeds,  */
+/* fluid HLL states and fluxes and the contact solve follow              */
+/* hlle_flow_mpx, but its star states and fluxes are its own:            */
+/*  - S*_n carries the star pressure and tau* is written in terms of the */
+/*    HLL tau state (:3187-3243);                                         */
+/*  - fluid star fluxes are F_K + s_K (U*_K - U_K) (:3395-3450);          */
+/*  - the EM star "fluxes" are star states (:3342-3364, :3539-3561), so   */
+/*    a face with v* != 0 carries the EM HLL states as its EM fluxes,     */
+/*    and v* = 0 the EM HLL fluxes;                                        */
+/*  - the fluid selection uses s1 > 0, s1 <= 0 <= v*, v* < 0 <= s2,        */
+/*    s2 < 0 with the sides' own physical fluxes outside the fan          */
+/*    (:3719-3800), which this routine does assign in 2D.                 */
+/* Its get_hllc_vel copy allows 200 iterations (:4385).                  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  double psi_h, phi_h, e[3], b[3];   /* EM HLL states, s = -1, +1 */
+  double f_em[8];                    /* EM HLL fluxes in FX or FY slot order 0..7 */
+  double a, b_, q_f, dd_h, dd_f, tau_h, tau_f, s_h[3], s_f[3];
+  double er[3], e2b2s;
+} hlle_mcl_t;
+
+/* the x (dir 0) or y face's HLL states and fluxes from its MC sides */
+static int hlle_mcl_prep(double gam, int dir, const side_t *L, const side_t *R, hlle_mcl_t *F) {
+  if (dir == 0) {
+    F->psi_h = hll_state(L->psi, R->psi, L->ex, R->ex);
+    F->phi_h = hll_state(L->phi, R->phi, L->bx, R->bx);
+    F->e[0] = hll_state(L->ex, R->ex, L->psi, R->psi);
+    F->e[1] = hll_state(L->ey, R->ey, L->bz, R->bz);
+    F->e[2] = hll_state(L->ez, R->ez, L->bym, R->bym);
+    F->b[0] = hll_state(L->bx, R->bx, L->phi, R->phi);
+    F->b[1] = hll_state(L->by, R->by, L->ezm, R->ezm);
+    F->b[2] = hll_state(L->bz, R->bz, L->ey, R->ey);
+    F->f_em[FX_EXPSI] = hll_flux(L->ex, R->ex, L->psi, R->psi);
+    F->f_em[FX_BXPHI] = hll_flux(L->bx, R->bx, L->phi, R->phi);
+    F->f_em[FX_PHIBX] = hll_flux(L->phi, R->phi, L->bx, R->bx);
+    F->f_em[FX_EZBY] = -hll_flux(L->ezm, R->ezm, L->by, R->by);
+    F->f_em[FX_EYBZ] = hll_flux(L->ey, R->ey, L->bz, R->bz);
+    F->f_em[FX_PSIEX] = hll_flux(L->psi, R->psi, L->ex, R->ex);
+    F->f_em[FX_BZEY] = hll_flux(L->bz, R->bz, L->ey, R->ey);
+    F->f_em[FX_BYEZ] = -hll_flux(L->bym, R->bym, L->ez, R->ez);
+  } else {
+    F->psi_h = hll_state(L->psi, R->psi, L->ey, R->ey);
+    F->phi_h = hll_state(L->phi, R->phi, L->by, R->by);
+    F->e[1] = hll_state(L->ey, R->ey, L->psi, R->psi);
+    F->e[2] = hll_state(L->ez, R->ez, L->bx, R->bx);
+    F->e[0] = hll_state(L->ex, R->ex, L->bzm, R->bzm);
+    F->b[1] = hll_state(L->by, R->by, L->phi, R->phi);
+    F->b[2] = hll_state(L->bz, R->bz, L->exm, R->exm);
+    F->b[0] = hll_state(L->bx, R->bx, L->ez, R->ez);
+    F->f_em[FY_EYPSI] = hll_flux(L->ey, R->ey, L->psi, R->psi);
+    F->f_em[FY_BYPHI] = hll_flux(L->by, R->by, L->phi, R->phi);
+    F->f_em[FY_EZBX] = hll_flux(L->ez, R->ez, L->bx, R->bx);
+    F->f_em[FY_PHIBY] = hll_flux(L->phi, R->phi, L->by, R->by);
+    F->f_em[FY_EXBZ] = -hll_flux(L->exm, R->exm, L->bz, R->bz);
+    F->f_em[FY_BZEX] = -hll_flux(L->bzm, R->bzm, L->ex, R->ex);
+    F->f_em[FY_PSIEY] = hll_flux(L->psi, R->psi, L->ey, R->ey);
+    F->f_em[FY_BXEZ] = hll_flux(L->bx, R->bx, L->ez, R->ez);
+  }
+  double a, b;
+  int st = hlle_speeds(gam, L, R, dir, &a, &b);
+  if (st) return st;
+  F->a = a;
+  F->b_ = b;
+  F->q_f = hlle_flux(a, b, L->j1, R->j1, L->q, R->q);
+  F->dd_h = hlle_state(a, b, L->d, R->d, L->fd, R->fd);
+  F->tau_h = hlle_state(a, b, L->tau, R->tau, L->ftau, R->ftau);
+  F->s_h[0] = hlle_state(a, b, L->sx, R->sx, L->fs1, R->fs1);
+  F->s_h[1] = hlle_state(a, b, L->sy, R->sy, L->fs2, R->fs2);
+  F->s_h[2] = hlle_state(a, b, L->sz, R->sz, L->fs3, R->fs3);
+  F->dd_f = hlle_flux(a, b, L->fd, R->fd, L->d, R->d);
+  F->tau_f = hlle_flux(a, b, L->ftau, R->ftau, L->tau, R->tau);
+  F->s_f[0] = hlle_flux(a, b, L->fs1, R->fs1, L->sx, R->sx);
+  F->s_f[1] = hlle_flux(a, b, L->fs2, R->fs2, L->sy, R->sy);
+  F->s_f[2] = hlle_flux(a, b, L->fs3, R->fs3, L->sz, R->sz);
+  const double *e = F->e, *bb = F->b;
+  F->er[0] = bb[2] * e[1] - bb[1] * e[2];
+  F->er[1] = bb[0] * e[2] - bb[2] * e[0];
+  F->er[2] = bb[1] * e[0] - bb[0] * e[1];
+  F->e2b2s = 0.5 * (((sq(e[0]) + sq(e[1])) + sq(e[2])) + ((sq(bb[0]) + sq(bb[1])) + sq(bb[2])));
+  return 0;
+}
+
+/* contact speed of one face (:2924-3056 x, :3064-3166 y) */
+static double hlle_mcl_contact(int dir, const hlle_mcl_t *F, double v0) {
+  const int n = dir, t1 = dir == 0 ? 1 : 0, t2 = 2;
+  double a = F->tau_f - F->er[n];
+  double b = (((sq(F->e[t1]) + sq(F->e[t2])) + (sq(F->b[t1]) + sq(F->b[t2]))) - F->tau_h) - F->s_f[n];
+  double c = F->s_h[n] - F->er[n];
+  return hllc_contact(a, b, c, v0);
+}
+
+/* the fluid and EM face fluxes of one face from its contact speed v */
+static int hlle_mcl_fluxes(int dir, const side_t *L, const side_t *R, const hlle_mcl_t *F, double v, double *f) {
+  const int n = dir, t1 = dir == 0 ? 1 : 2, t2 = dir == 0 ? 2 : 0; /* transverse momenta in reference order */
+  const double a = F->a, b = F->b_;
+  /* EM: the star "fluxes" of a face with v != 0 are its HLL states */
+  if (v == 0.0) {
+    for (int k = 0; k < 8; k++) f[k] = F->f_em[k];
+  } else if (dir == 0) {
+    f[FX_EXPSI] = F->e[0]; f[FX_BXPHI] = F->b[0]; f[FX_PHIBX] = F->phi_h;
+    f[FX_EZBY] = -(-F->e[2]); f[FX_EYBZ] = F->e[1]; f[FX_PSIEX] = F->psi_h;
+    f[FX_BZEY] = F->b[2]; f[FX_BYEZ] = -(-F->b[1]);
+  } else {
+    f[FY_EYPSI] = F->e[1]; f[FY_BYPHI] = F->b[1]; f[FY_EZBX] = F->e[2];
+    f[FY_PHIBY] = F->phi_h; f[FY_EXBZ] = -(-F->e[0]); f[FY_BZEX] = -(-F->b[2]);
+    f[FY_PSIEY] = F->psi_h; f[FY_BXEZ] = F->b[0];
+  }
+  /* fluid star states (:3174-3243) */
+  double pstr = (F->s_f[n] + (sq(F->e[n]) + sq(F->b[n]))) - (F->tau_f - F->er[n]) * v;
+  double sR[3], sL[3];
+  sR[n] = (((((b * F->s_h[n] - F->s_f[n]) + pstr) - sq(F->b[n])) - sq(F->e[n])) - F->er[n] * v) / (b - v);
+  sL[n] = (((((-(a * F->s_h[n]) + F->s_f[n]) - pstr) + sq(F->b[n])) + sq(F->e[n])) + F->er[n] * v) / (v - a);
+  const int ts[2] = {t1, t2};
+  for (int q = 0; q < 2; q++) {
+    int t = ts[q];
+    sR[t] = ((((b * F->s_h[t] - F->s_f[t]) - F->er[t] * v) - F->e[n] * F->e[t]) - F->b[n] * F->b[t]) / (b - v);
+    sL[t] = ((((-(a * F->s_h[t]) + F->s_f[t]) + F->er[t] * v) + F->e[n] * F->e[t]) + F->b[n] * F->b[t]) / (v - a);
+  }
+  double tauR = (((b * F->tau_h - F->tau_f) + (pstr - F->e2b2s) * v) + F->er[n]) / (b - v);
+  double tauL = ((((-(a * F->tau_h)) + F->tau_f) - (pstr - F->e2b2s) * v) - F->er[n]) / (v - a);
+  double ddR = (b * F->dd_h - F->dd_f) / (b - v);
+  double ddL = (-(a * F->dd_h) + F->dd_f) / (v - a);
+  /* fluid star fluxes (:3287-3335, :3395-3450) */
+  double fdR, fdL, ftR, ftL, fsR[3], fsL[3];
+  double sideR[3] = {R->sx, R->sy, R->sz}, sideL[3] = {L->sx, L->sy, L->sz};
+  double fsideR[3] = {R->fs1, R->fs2, R->fs3}, fsideL[3] = {L->fs1, L->fs2, L->fs3};
+  if (v == 0.0) {
+    fdR = fdL = F->dd_f; ftR = ftL = F->tau_f;
+    for (int k = 0; k < 3; k++) fsR[k] = fsL[k] = F->s_f[k];
+  } else {
+    fdR = R->fd + b * (ddR - R->d);
+    fdL = L->fd + a * (ddL - L->d);
+    ftR = R->ftau + b * (tauR - R->tau);
+    ftL = L->ftau + a * (tauL - L->tau);
+    for (int k = 0; k < 3; k++) {
+      fsR[k] = fsideR[k] + b * (sR[k] - sideR[k]);
+      fsL[k] = fsideL[k] + a * (sL[k] - sideL[k]);
+    }
+  }
+  /* selection (:3719-3800 x, :3905-3990 y) */
+  double *o = f + FX_J; /* J, FD, Ftau, FS x, FS y, FS z: the same slots in both directions */
+  if (a > 0.0) {
+    o[0] = L->j1; o[1] = L->fd; o[2] = L->ftau; o[3] = L->fs1; o[4] = L->fs2; o[5] = L->fs3;
+  } else if (a <= 0.0 && 0.0 <= v) {
+    o[0] = F->q_f; o[1] = fdL; o[2] = ftL; o[3] = fsL[0]; o[4] = fsL[1]; o[5] = fsL[2];
+  } else if (v < 0.0 && 0.0 <= b) {
+    o[0] = F->q_f; o[1] = fdR; o[2] = ftR; o[3] = fsR[0]; o[4] = fsR[1]; o[5] = fsR[2];
+  } else if (b < 0.0) {
+    o[0] = R->j1; o[1] = R->fd; o[2] = R->ftau; o[3] = R->fs1; o[4] = R->fs2; o[5] = R->fs3;
+  } else {
+    return 8;
+  }
+  return 0;
+}
+
+/* seeds of the two contact solves in the reference build: see hllc_mcl_faces */
+static int hlle_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy) {
+  side_t xl, xr, yl, yr;
+  mcl_sides(s, i, j, 0, &xl, &xr);
+  mcl_sides(s, i, j, 1, &yl, &yr);
+  /* the sound speeds read the sides' enthalpy (:2243-2256) */
+  side_t *sides[4] = {&xl, &xr, &yl, &yr};
+  for (int k = 0; k < 4; k++) {
+    double eps = sides[k]->p / ((s->c.gamma - 1.0) * sides[k]->rho);
+    sides[k]->h = sides[k]->rho * (1.0 + eps) + sides[k]->p;
+  }
+  hlle_mcl_t X, Y;
+  int st = hlle_mcl_prep(s->c.gamma, 0, &xl, &xr, &X);
+  if (st) return st;
+  st = hlle_mcl_prep(s->c.gamma, 1, &yl, &yr, &Y);
+  if (st) return st;
+  g_hllc_maxiter = 200;
+  double vx = hlle_mcl_contact(0, &X, g_hllc_seed_mode ? Y.er[1] : NAN);
+  double vy = vx == vx ? hlle_mcl_contact(1, &Y, g_hllc_seed_mode ? g_hllc_last_iterate : NAN) : NAN;
+  g_hllc_maxiter = 100;
+  if (vx != vx || vy != vy) return 7;
+  st = hlle_mcl_fluxes(0, &xl, &xr, &X, vx, fx);
+  if (!st) st = hlle_mcl_fluxes(1, &yl, &yr, &Y, vy, fy);
   return st;
 }
 

@@ -30,7 +30,7 @@ enum {
 enum { OR_BC_COPY = 0, OR_BC_PERIODIC = 1 };
 
 /* Riemann solvers (parameters.f95:21 flux_solver 5/6/7). */
-enum { OR_LAX = 1, OR_HLL_MCL = 2, OR_HLLC_MCL = 4, OR_HLL = 5, OR_HLLE = 6, OR_HLLC = 7 };
+enum { OR_LAX = 1, OR_HLL_MCL = 2, OR_HLLE_MCL = 3, OR_HLLC_MCL = 4, OR_HLL = 5, OR_HLLE = 6, OR_HLLC = 7 };
 
 typedef struct {
   int32_t imax, jmax;

@@ -128,6 +128,8 @@ program cueva_ref_harness
            call lax_flow
         else if (flux_solver == 2) then
            call hll_flow
+        else if (flux_solver == 3) then
+           call hlle_flow
         else if (flux_solver == 4) then
            call hllc_flow
         else if (flux_solver == 5) then
@@ -137,7 +139,7 @@ program cueva_ref_harness
         else if (flux_solver == 7) then
            call hllc_flow_mpx
         else
-           write(*,*) "harness: flux_solver must be 1, 2, 4, 5, 6 or 7"
+           write(*,*) "harness: flux_solver must be 1 to 7"
            stop 3
         end if
         call fluxandsource

@@ -51,6 +51,8 @@ CASES = {
     "kh_lax_bz_32x64": (32, 64, 1, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
     "kh_hllc_mcl_32x64": (32, 64, 4, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None),
     "kh_hllc_mcl_bz_32x64": (32, 64, 4, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
+    "kh_hlle_mcl_32x64": (32, 64, 3, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None),
+    "kh_hlle_mcl_bz_32x64": (32, 64, 3, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
 }
 SEED = 20261202
 

@@ -27,8 +27,9 @@ def test_driver_built_and_checks_arguments():
 @pytest.mark.gpu
 @pytest.mark.parametrize("fs,rec,order,kw", [(cueva.HLLC, 0, 2, {}), (cueva.HLLE, 1, 2, {"bz0": 0.1}),
                                              (cueva.HLL, 2, 4, {}), (cueva.HLL_MCL, 0, 2, {"bz0": 0.1}),
-                                             (cueva.LAX, 0, 2, {"bz0": 0.1}), (cueva.HLLC_MCL, 0, 2, {"bz0": 0.1})],
-                         ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl"])
+                                             (cueva.LAX, 0, 2, {"bz0": 0.1}), (cueva.HLLC_MCL, 0, 2, {"bz0": 0.1}),
+                                             (cueva.HLLE_MCL, 0, 2, {"bz0": 0.1})],
+                         ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl", "hlle_mcl"])
 def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built (needs flang)")

@@ -40,6 +40,7 @@ CASES = {
     "hll_mcl_ssp3": (cueva.HLL_MCL, 0, 4, 1.0e3, {}),
     "lax_bz": (cueva.LAX, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hllc_mcl_bz": (cueva.HLLC_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
+    "hlle_mcl_bz": (cueva.HLLE_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
 }
 
 
@@ -88,7 +89,8 @@ def test_march_equals_face_planes_fp64(name, shape):
         assert np.array_equal(got, ref), nd
 
 
-@pytest.mark.parametrize("name", ["hllc_mp5", "hlle_mp7_bz", "hll_mp9_ssp3", "hll_mcl_bz", "lax_bz", "hllc_mcl_bz"])
+@pytest.mark.parametrize("name", ["hllc_mp5", "hlle_mp7_bz", "hll_mp9_ssp3", "hll_mcl_bz", "lax_bz", "hllc_mcl_bz",
+                                  "hlle_mcl_bz"])
 def test_march_matches_face_planes_mixed(name):
     fs, rec, order, sigma, kw = CASES[name]
     p, prims = cueva.kh_problem(130, 68, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True,
@@ -126,7 +128,7 @@ def test_march_learned_matches_face_planes():
 
 
 @pytest.mark.parametrize("name", ["hll_mp5", "hllc_mp7", "hlle_mp7_bz", "hll_mp9_ssp3", "hll_mcl_bz", "lax_bz",
-                                  "hllc_mcl_bz"])
+                                  "hllc_mcl_bz", "hlle_mcl_bz"])
 @pytest.mark.parametrize("precision", [cueva.FP64, cueva.MIXED])
 def test_multi_step_call_matches_face_planes(name, precision):
     """Within one cueva_step call the recovery pass of step h also forms the

@@ -44,6 +44,8 @@ CASES = {
     "lax_bz": (cueva.LAX, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hllc_mcl": (cueva.HLLC_MCL, 0, 2, 1.0e6, {}),
     "hllc_mcl_bz": (cueva.HLLC_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
+    "hlle_mcl": (cueva.HLLE_MCL, 0, 2, 1.0e6, {}),
+    "hlle_mcl_bz": (cueva.HLLE_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
 }
 
 
@@ -99,7 +101,7 @@ def test_gpu_equals_oracle_bitwise_100_steps(fs, sigma):
 
 
 @pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz", "hll_mcl_bz", "lax_bz",
-                                  "hllc_mcl_bz"])
+                                  "hllc_mcl_bz", "hlle_mcl_bz"])
 def test_routine_entry_points_match_fused_step(case):
     """The per-subroutine entry points (the drop-in path) and cueva_step agree bit for bit."""
     fs, rec, order, sigma, kw = CASES[case]
@@ -148,6 +150,7 @@ BITWISE_DUMPS = {
     "kh_hll_mcl_32x64": (1, 10), "kh_hll_mcl_bz_32x64": (1, 10), "kh_hll_mcl_ssp3_32x64": (1, 5),
     "kh_lax_32x64": (1, 10), "kh_lax_bz_32x64": (1, 10),
     "kh_hllc_mcl_32x64": (1, 10), "kh_hllc_mcl_bz_32x64": (1, 10),
+    "kh_hlle_mcl_32x64": (1, 10), "kh_hlle_mcl_bz_32x64": (1, 10),
 }
 
 
