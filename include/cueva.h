@@ -86,7 +86,8 @@ typedef struct cueva_params {
                              build of the reference; 0: the clamped seed       */
   int32_t crmath;         /* 1: correctly rounded acos/cos/pow in the Cardano
                              step (double-double); 0: device math library      */
-  int32_t reserved[5];
+  int32_t primitive_rec;  /* 1: recons_var_primitive before each step (:26)    */
+  int32_t reserved[4];
   double longx, longy;    /* domain lengths Lx, Ly (:174-175)                   */
   double cfl;             /* CFL number (:309)                                  */
   double sigma;           /* conductivity sigma_0 (:196)                        */
@@ -127,6 +128,11 @@ int cueva_download(cueva_ctx *ctx, double *prims14, double *cons5);
  * (04_boundary.f95:1, 24_boundary_conserved.f95:2, 25_boundary_electric.f95:1,
  *  24_boundary_flux_conserved.f95:2); kind = CUEVA_BND_*, l = stage (1-based). */
 int cueva_boundary(cueva_ctx *ctx, int kind, int l);
+/* Second-order diagonal MC reconstruction and re-averaging of the 14
+ * primitives on the interior (recons_var_primitive, 05_reconstruct.f95:6;
+ * the reference calls it, then boundary, before startconvar when
+ * primitive_rec = 1, main_cueva_code.f95:170-175).  Ghosts are not filled. */
+int cueva_recons_var_primitive(cueva_ctx *ctx);
 int cueva_startconvar(cueva_ctx *ctx);             /* 06_conserved_var.f95:5   */
 int cueva_cleanvarint(cueva_ctx *ctx);             /* 07_clean_var.f95:5       */
 int cueva_swap1(cueva_ctx *ctx);                   /* 08_swap1.f95:2           */

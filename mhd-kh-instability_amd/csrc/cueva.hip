@@ -175,6 +175,7 @@ static int check_params(const cueva_params* p) {
       p->flux_solver != CUEVA_HLLE_MCL)
     return CUEVA_E_ARG;
   if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
+  if (p->primitive_rec != 0 && p->primitive_rec != 1) return CUEVA_E_ARG;
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
   if (p->precision != CUEVA_FP64 && p->precision != CUEVA_MIXED) return CUEVA_E_ARG;
   if (p->c2p_mode != CUEVA_C2P_CARDANO && p->c2p_mode != CUEVA_C2P_LEARNED) return CUEVA_E_ARG;
@@ -404,6 +405,18 @@ extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
       return CUEVA_E_ARG;
   }
   return fill_ghosts(c, list, n);
+}
+
+static int recons_prims(cueva_ctx* c) {
+  LAUNCH(K_OTHER, k_recons_prim, interior_grid(c->g, BLK), BLK, c->g, c->P);
+  if (launch_ok()) return CUEVA_E_HIP;
+  LAUNCH(K_OTHER, k_recons_copy, interior_grid(c->g, BLK), BLK, c->g, c->P);
+  return launch_ok();
+}
+
+extern "C" int cueva_recons_var_primitive(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  return recons_prims(c);
 }
 
 extern "C" int cueva_startconvar(cueva_ctx* c) {
@@ -669,6 +682,11 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   if (learned && !c->has_mlp) return CUEVA_E_ARG;
   bool s0_ready = false;  // this step's stage-0 state was formed by the previous step's recovery pass
   for (int h = 0; h < nsteps; h++) {
+    if (c->prm.primitive_rec) {  // main_cueva_code.f95:170-175
+      int rc = recons_prims(c);
+      if (!rc) rc = fill_ghosts(c, c->P.p, P_N);
+      if (rc) return rc;
+    }
     for (int l = 0; l < c->S.lmax; l++) {
       const bool last = l == c->S.lmax - 1;
       // the marching path forms stage l >= 1 in the previous stage's face pass
@@ -695,7 +713,8 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
         }
         adopt_update(c, l);
         // the primitive recovery also forms the next step's stage-0 state
-        const bool fuse = h + 1 < nsteps;
+        // (not when the next step first re-forms the primitives)
+        const bool fuse = h + 1 < nsteps && !c->prm.primitive_rec;
         if ((rc = launch_c2p(c, K_FINAL, fuse))) return rc;
         // ghosts of the new primitives (and of the next stage-1 state)
         if ((rc = fuse ? fill_stage_ghosts(c, true) : fill_ghosts(c, c->P.p, P_N))) return rc;

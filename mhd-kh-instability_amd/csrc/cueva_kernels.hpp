@@ -193,6 +193,34 @@ __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& 
   return rhs_of<FT>(P, S, k, dfx, dfy);
 }
 
+// recons_var_primitive, 05_reconstruct.f95:525-740 (DIM = 2, limit = 2,
+// whose limiter is the MC slope): each interior primitive u becomes
+// 0.5 (R + L) with L = u + 0.5 lim and R = u - 0.5 lim from the diagonal
+// differences u(i+1,j+1) - u and u - u(i-1,j-1).  The 14 results go to the
+// x-face planes (dead between steps); k_recons_copy writes them back, so
+// every read sees the old values.
+__global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > g.imax || j > g.jmax) return;
+  const int64_t c = g.at(i, j), dd = g.pitch + 1;
+#pragma unroll 2
+  for (int k = 0; k < P_N; k++) {
+    const double* a = P.p[k];
+    const double u = a[c];
+    const double lim = mcl_slope(a[c + dd] - u, u - a[c - dd]);
+    const double r = u - 0.5 * lim, l = u + 0.5 * lim;
+    P.fx[k][c] = 0.5 * (r + l);
+  }
+}
+__global__ void __launch_bounds__(256) k_recons_copy(Geom g, Planes P) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > g.imax || j > g.jmax) return;
+  const int64_t c = g.at(i, j);
+#pragma unroll
+  for (int k = 0; k < P_N; k++) P.p[k][c] = P.fx[k][c];
+}
+static_assert(P_N <= FX_N, "the reconstruction stages its results in the x-face planes");
+
 template <typename FT>
 __global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l) {
   INTERIOR_IJ

@@ -54,7 +54,7 @@ class Params(ctypes.Structure):
         ("flux_solver", ctypes.c_int32), ("rec_mp", ctypes.c_int32),
         ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("c2p_mode", ctypes.c_int32), ("hllc_seed", ctypes.c_int32), ("crmath", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("primitive_rec", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
         ("longx", ctypes.c_double), ("longy", ctypes.c_double), ("cfl", ctypes.c_double),
         ("sigma", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
         ("reserved_d", ctypes.c_double * 6),
@@ -77,6 +77,7 @@ SIGNATURES = {
     "cueva_upload_prims": (_i, [_c, _P(_d)]),
     "cueva_download": (_i, [_c, _P(_d), _P(_d)]),
     "cueva_boundary": (_i, [_c, _i, _i]),
+    "cueva_recons_var_primitive": (_i, [_c]),
     "cueva_startconvar": (_i, [_c]),
     "cueva_cleanvarint": (_i, [_c]),
     "cueva_swap1": (_i, [_c]),
@@ -362,6 +363,9 @@ class Solver:
         x = self.ctx
         lmax = 2 if self.p.order == 2 else 4
         check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
+        if self.p.primitive_rec:  # main_cueva_code.f95:170-175
+            check(L.cueva_recons_var_primitive(x), "recons_var_primitive")
+            check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
         check(L.cueva_startconvar(x), "startconvar")
         check(L.cueva_cleanvarint(x), "cleanvarint")
         check(L.cueva_swap1(x), "swap1")

@@ -19,7 +19,8 @@ module cueva_gpu
   type, bind(C) :: cueva_params
     integer(c_int32_t) :: imax, jmax, order, flux_solver, rec_mp, bc_x, bc_y
     integer(c_int32_t) :: precision, c2p_mode, hllc_seed, crmath
-    integer(c_int32_t) :: reserved(5)
+    integer(c_int32_t) :: primitive_rec
+    integer(c_int32_t) :: reserved(4)
     real(c_double) :: longx, longy, cfl, sigma, gamma, dt
     real(c_double) :: reserved_d(6)
   end type cueva_params
@@ -64,6 +65,10 @@ module cueva_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: kind, l
+    end function
+    integer(c_int) function cueva_recons_var_primitive(ctx) bind(C, name="cueva_recons_var_primitive")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
     integer(c_int) function cueva_startconvar(ctx) bind(C, name="cueva_startconvar")
       import :: c_int, c_ptr

@@ -22,6 +22,10 @@ contains
     call check(cueva_boundary(ctx, CUEVA_BND_PRIM, 1_c_int), "boundary")
   end subroutine boundary
 
+  subroutine recons_var_primitive     ! 05_reconstruct.f95:6
+    call check(cueva_recons_var_primitive(ctx), "recons_var_primitive")
+  end subroutine recons_var_primitive
+
   subroutine startconvar              ! 06_conserved_var.f95:5
     call check(cueva_startconvar(ctx), "startconvar")
   end subroutine startconvar

@@ -27,7 +27,7 @@ program main_cueva_gpu
   use cueva_shims
   use cueva_io
   implicit none
-  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini
+  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini, primrec
   integer(c_int64_t) :: seed
   real(c_double) :: sigma, v0, bx0, bz0, dv(5), g(37)
   character(len=512) :: arg, prefix, restart_in
@@ -36,9 +36,9 @@ program main_cueva_gpu
   real(c_double), allocatable :: prims(:), cons(:)
 
   nargs = command_argument_count()
-  if (nargs < 14 .or. nargs > 17) then
+  if (nargs < 14 .or. nargs > 18) then
     write(*,*) "usage: main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED V0 BX0 BZ0 SEED HGLB ", &
-               "OUTPREFIX [HLOC [RESTART_IN [HINI]]]"
+               "OUTPREFIX [HLOC [RESTART_IN [HINI [PRIMITIVE_REC]]]]"
     stop 2
   end if
   call get_command_argument(1, arg); read(arg, *) ni
@@ -65,6 +65,10 @@ program main_cueva_gpu
   if (nargs >= 17) then
     call get_command_argument(17, arg); read(arg, *) hini
   end if
+  primrec = 0
+  if (nargs >= 18) then
+    call get_command_argument(18, arg); read(arg, *) primrec
+  end if
 
   call check(cueva_params_default(prm), "params_default")
   if (rotated == 1) then
@@ -76,7 +80,7 @@ program main_cueva_gpu
     prm%bc_x = CUEVA_BC_PERIODIC; prm%bc_y = CUEVA_BC_COPY
     prm%longx = 1.d0; prm%longy = 2.d0
   end if
-  prm%flux_solver = fs; prm%rec_mp = rec; prm%order = order
+  prm%flux_solver = fs; prm%rec_mp = rec; prm%order = order; prm%primitive_rec = primrec
   prm%sigma = sigma; prm%cfl = 0.4d0
   lmax = merge(2, 4, order == 2)
   hmax = nsteps
@@ -106,6 +110,10 @@ program main_cueva_gpu
   t_advance = (hini - 1) * Delt
   do h = hini, hini + hmax - 1
     call boundary
+    if (prm%primitive_rec == 1) then   ! main_cueva_code.f95:170-175
+      call recons_var_primitive
+      call boundary
+    end if
     call startconvar
     call cleanvarint
     contador_r = 0

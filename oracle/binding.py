@@ -48,6 +48,7 @@ def load():
         lib.oracle_hllc_vel.argtypes = [ctypes.c_double] * 5
         lib.oracle_c2p_point.argtypes = [ctypes.c_double, _P, _P, ctypes.POINTER(ctypes.c_int)]
         lib.oracle_set_hllc_seed_mode.argtypes = [ctypes.c_int]
+        lib.oracle_set_primitive_rec.argtypes = [ctypes.c_int]
         lib.oracle_set_libm_mode.argtypes = [ctypes.c_int]
         lib.oracle_globals.argtypes = [ctypes.c_void_p, _P]
         _lib = lib
@@ -75,6 +76,7 @@ class Oracle:
         if rc:
             raise ValueError("oracle config rejected (%d)" % rc)
         lib.oracle_set_hllc_seed_mode(int(params.hllc_seed))
+        lib.oracle_set_primitive_rec(int(params.primitive_rec))
         if libm is None:
             libm = "cr" if params.crmath else "c"
         lib.oracle_set_libm_mode(1 if libm == "cr" else 0)

@@ -1271,6 +1271,36 @@ typedef struct {
   double ezm, bym, exm, bzm; /* the "_m" states the fluxes use */
 } mcl_side_t;
 
+/* ------------------------------------------------------------------ */
+/* recons_var_primitive, 05_reconstruct.f95:525-740 (DIM = 2), with     */
+/* primitive_rec = 1 (parameters.f95:26) and limit = 2, whose limiter   */
+/* (funciones.f95:100-142) is the MC slope above.  Along the diagonal,  */
+/* L(i,j) = u + 0.5 lim(u(i+1,j+1) - u, u - u(i-1,j-1)) and R(i+1,j+1)  */
+/* from the same differences, and every interior point becomes          */
+/* 0.5 (R + L): u again up to rounding.  Ghosts are left to the caller. */
+/* ------------------------------------------------------------------ */
+static int g_primitive_rec = 0;
+void oracle_set_primitive_rec(int on) { g_primitive_rec = on != 0; }
+
+static void recons_prims(oracle_state *s) {
+  const int64_t dd = IX(s, 1, 1) - IX(s, 0, 0);
+  double *tmp = malloc(s->n * sizeof(double));
+  for (int k = 0; k < OR_NPRIM; k++) {
+    double *a = s->prim[k];
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        size_t c = IX(s, i, j);
+        double u = a[c];
+        double lim = mcl(a[c + dd] - u, u - a[c - dd]);
+        double r = u - 0.5 * lim, l = u + 0.5 * lim;
+        tmp[c] = 0.5 * (r + l);
+      }
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) a[IX(s, i, j)] = tmp[IX(s, i, j)];
+  }
+  free(tmp);
+}
+
 /* u_L = u(i) + FAC mcl(u(i+1)-u(i), u(i)-u(i-1)), u_R = u(i+1) - FAC
  * mcl(u(i+2)-u(i+1), u(i+1)-u(i)), FAC = 0.5 (16_hll_flow_mcl.f95:572-575);
  * the "_m" states negate the cell value only (:634-637) */
@@ -1967,6 +1997,10 @@ static int c2p(oracle_state *s) {
 int oracle_step(oracle_state *s, int nsteps) {
   for (int h = 0; h < nsteps; h++) {
     for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
+    if (g_primitive_rec) { /* main_cueva_code.f95:170-175 */
+      recons_prims(s);
+      for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
+    }
     startconvar(s);
     cleanvarint(s);
     s->cnt[0] = s->cnt[1] = s->cnt[2] = 0;

@@ -78,6 +78,8 @@ void oracle_globals(const oracle_state *s, double out[37]);
 /* Pointwise/linewise pieces, exported for unit tests. */
 void oracle_mp_line(int order, const double *u, int n, double *up, double *um, int positivity);
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0);
+/* 1: recons_var_primitive before each step (primitive_rec = 1) */
+void oracle_set_primitive_rec(int on);
 /* 1: first contact-speed Newton point as in the reference build; 0: clamped seed */
 void oracle_set_hllc_seed_mode(int mode);
 /* 0: acos/cos/pow from the C library (as the reference build); 1: correctly rounded */

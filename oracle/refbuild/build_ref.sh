@@ -52,6 +52,7 @@ sed -E \
   -e "s/^( *doubleprecision,parameter +:: +Ly +=) *[0-9.d]+/\1 $LY/" \
   -e "s/^( *doubleprecision,parameter +:: +sigma_0 +=) *[0-9.d]+/\1 $SIGMA/" \
   -e "s/^( *doubleprecision,parameter +:: +CFL +=) *[0-9.d]+/\1 $CFL/" \
+  -e "s/^( *integer,parameter +:: +primitive_rec +=) *[0-9]+/\1 ${PRIMITIVE_REC:-0}/" \
   "$REF/parameters.f95" > "$WORK/parameters.f95"
 
 # Every substitution must have hit exactly its line.
@@ -66,6 +67,7 @@ check "Lx +=  *$LX( |$)"
 check "Ly +=  *$LY( |$)"
 check "sigma_0 +=  *$SIGMA( |$)"
 check "CFL +=  *$CFL( |$)"
+check "primitive_rec +=  *${PRIMITIVE_REC:-0}( |$)"
 
 cd "$WORK"
 UNITS="scalar threevectors fourvectors funciones imex_scheme mp5 mp7 mp9

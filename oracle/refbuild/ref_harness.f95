@@ -106,6 +106,10 @@ program cueva_ref_harness
 
      call boundary
      t_advance = t_advance + Delt
+     if (primitive_rec == 1) then
+        call recons_var_primitive
+        call boundary
+     end if
 
      call startconvar
      call cleanvarint

@@ -53,6 +53,8 @@ CASES = {
     "kh_hllc_mcl_bz_32x64": (32, 64, 4, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
     "kh_hlle_mcl_32x64": (32, 64, 3, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None),
     "kh_hlle_mcl_bz_32x64": (32, 64, 3, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
+    "kh_hll_mcl_primrec_32x64": (32, 64, 2, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None, {"primitive_rec": 1}),
+    "kh_hllc_mp5_primrec_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"primitive_rec": 1}),
 }
 SEED = 20261202
 
@@ -74,7 +76,8 @@ def main(only=None):
         if not os.path.exists(exe):
             subprocess.run([os.path.join(ROOT, "oracle", "refbuild", "build_ref.sh"), name, str(imax), str(jmax),
                             str(order), str(fs), str(rec), "6", "%.1fd0" % sigma if sigma < 1e4 else "1.d6",
-                            "0.4d0", "2.d0", "1.d0"], check=True)
+                            "0.4d0", "2.d0", "1.d0"], check=True,
+                           env=dict(os.environ, PRIMITIVE_REC=str(init_kw.get("primitive_rec", 0))))
         p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
                                     rotated=True, seed=SEED, **init_kw)
         assert (p.imax, p.jmax) == (imax, jmax)

@@ -28,8 +28,9 @@ def test_driver_built_and_checks_arguments():
 @pytest.mark.parametrize("fs,rec,order,kw", [(cueva.HLLC, 0, 2, {}), (cueva.HLLE, 1, 2, {"bz0": 0.1}),
                                              (cueva.HLL, 2, 4, {}), (cueva.HLL_MCL, 0, 2, {"bz0": 0.1}),
                                              (cueva.LAX, 0, 2, {"bz0": 0.1}), (cueva.HLLC_MCL, 0, 2, {"bz0": 0.1}),
-                                             (cueva.HLLE_MCL, 0, 2, {"bz0": 0.1})],
-                         ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl", "hlle_mcl"])
+                                             (cueva.HLLE_MCL, 0, 2, {"bz0": 0.1}), (cueva.HLLC, 0, 2, {"primitive_rec": 1})],
+                         ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl", "hlle_mcl",
+                              "hllc_mp5_primrec"])
 def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built (needs flang)")
@@ -39,6 +40,8 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     print("driver", fs, rec, order, flush=True)
     args = [DRIVER, str(ni), str(nj), str(steps), str(fs), str(rec), str(order), repr(sigma), "1", "0.5", "0.01",
             repr(bz0), "20261202", "1", prefix]
+    if kw.get("primitive_rec"):
+        args += ["0", "-", "1", "1"]  # HLOC RESTART_IN HINI PRIMITIVE_REC
     res = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True, **kw)
@@ -63,11 +66,11 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     assert np.array_equal(glob[:, 0], np.arange(1, steps + 1))
 
 
-def _run_driver(cwd, ni, nj, fs, rec, order, sigma, bz0, steps, hloc, restart_in="-", hini=1, hglb=1):
+def _run_driver(cwd, ni, nj, fs, rec, order, sigma, bz0, steps, hloc, restart_in="-", hini=1, hglb=1, primrec=0):
     for d in ("data_restart", "data_global"):
         os.makedirs(os.path.join(cwd, d), exist_ok=True)
     args = [DRIVER, str(ni), str(nj), str(steps), str(fs), str(rec), str(order), repr(sigma), "1", "0.5", "0.01",
-            repr(bz0), "20261202", str(hglb), "run", str(hloc), restart_in, str(hini)]
+            repr(bz0), "20261202", str(hglb), "run", str(hloc), restart_in, str(hini), str(primrec)]
     res = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=cwd)
     assert res.returncode == 0, res.stdout + res.stderr
     return open(os.path.join(cwd, "run.bin"), "rb").read()

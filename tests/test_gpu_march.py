@@ -41,6 +41,7 @@ CASES = {
     "lax_bz": (cueva.LAX, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hllc_mcl_bz": (cueva.HLLC_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hlle_mcl_bz": (cueva.HLLE_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
+    "hllc_mp5_primrec": (cueva.HLLC, 0, 2, 1.0e6, {"primitive_rec": 1}),
 }
 
 
@@ -128,7 +129,7 @@ def test_march_learned_matches_face_planes():
 
 
 @pytest.mark.parametrize("name", ["hll_mp5", "hllc_mp7", "hlle_mp7_bz", "hll_mp9_ssp3", "hll_mcl_bz", "lax_bz",
-                                  "hllc_mcl_bz", "hlle_mcl_bz"])
+                                  "hllc_mcl_bz", "hlle_mcl_bz", "hllc_mp5_primrec"])
 @pytest.mark.parametrize("precision", [cueva.FP64, cueva.MIXED])
 def test_multi_step_call_matches_face_planes(name, precision):
     """Within one cueva_step call the recovery pass of step h also forms the

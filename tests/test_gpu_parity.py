@@ -46,6 +46,7 @@ CASES = {
     "hllc_mcl_bz": (cueva.HLLC_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hlle_mcl": (cueva.HLLE_MCL, 0, 2, 1.0e6, {}),
     "hlle_mcl_bz": (cueva.HLLE_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
+    "hllc_mp5_primrec": (cueva.HLLC, 0, 2, 1.0e6, {"primitive_rec": 1}),
 }
 
 
@@ -101,7 +102,7 @@ def test_gpu_equals_oracle_bitwise_100_steps(fs, sigma):
 
 
 @pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz", "hll_mcl_bz", "lax_bz",
-                                  "hllc_mcl_bz", "hlle_mcl_bz"])
+                                  "hllc_mcl_bz", "hlle_mcl_bz", "hllc_mp5_primrec"])
 def test_routine_entry_points_match_fused_step(case):
     """The per-subroutine entry points (the drop-in path) and cueva_step agree bit for bit."""
     fs, rec, order, sigma, kw = CASES[case]
@@ -151,6 +152,7 @@ BITWISE_DUMPS = {
     "kh_lax_32x64": (1, 10), "kh_lax_bz_32x64": (1, 10),
     "kh_hllc_mcl_32x64": (1, 10), "kh_hllc_mcl_bz_32x64": (1, 10),
     "kh_hlle_mcl_32x64": (1, 10), "kh_hlle_mcl_bz_32x64": (1, 10),
+    "kh_hll_mcl_primrec_32x64": (1, 10), "kh_hllc_mp5_primrec_32x64": (1, 10),
 }
 
 
