@@ -17,8 +17,17 @@
  *    is in the restart order psi phi Bx By Bz Vx Vy Vz Ex Ey Ez q rho p
  *    (22_writer.f95:1116-1172).
  *  - Calls on one ctx are ordered on the ctx's HIP stream; only
- *    cueva_download*, cueva_c2p counters, cueva_reduce_globals and
- *    cueva_sync wait for the device.  A ctx is not thread safe.
+ *    cueva_step, cueva_c2p, cueva_download*, the counter, record and error
+ *    queries, cueva_reduce_globals and cueva_sync wait for the device.  A
+ *    ctx is not thread safe.
+ *  - Failures: where the reference prints and stops, the device records the
+ *    first failure in the reference's own order (the launch that failed
+ *    first; in it, the first cell of the reference's loop, j outer, i inner)
+ *    with its step, stage and cell, and every later launch of the step path
+ *    leaves the data alone.  The status is sticky: cueva_step and cueva_c2p
+ *    return it from then on, until cueva_upload_prims loads a new state.
+ *    It surfaces at the next call that waits (cueva_c2p or cueva_step at the
+ *    latest, for a face failure of the per-subroutine sequence).
  */
 #ifndef CUEVA_H
 #define CUEVA_H
@@ -39,14 +48,20 @@ enum cueva_status {
   CUEVA_E_HLLC_VEL = 7,      /* get_hllc_vel did not converge / NaN contact  */
   CUEVA_E_HLLC_STAR = 8,     /* "Revise Vxstar velocity" (NaN contact speed) */
   CUEVA_E_HLLE_NAN = 9,      /* HLLE wave speed is NaN                      */
-  CUEVA_E_C2P_OLDV = 20,     /* superluminal velocity entering Cardano       */
+  CUEVA_E_C2P_OLDV = 20,     /* not returned: the reference clamps a superluminal
+                                previous velocity (12_varprimitive.f95:236-241) */
   CUEVA_E_C2P_IMAG = 21,     /* no admissible resolvent root                 */
   CUEVA_E_C2P_DET = 22,      /* det < 0 in the Cardano quadratic             */
   CUEVA_E_C2P_NOROOT = 23,   /* both quartic roots below 1                   */
-  CUEVA_E_C2P_NAN = 24,      /* no root selected (NaN)                       */
+  CUEVA_E_C2P_NAN = 24,      /* no root selected (NaN quartic roots); the
+                                reference does not stop here but goes on with
+                                the previous cell's W (12_varprimitive.f95:463-500) */
   CUEVA_E_C2P_SUPERLUM = 25, /* superluminal v and magnetisation < 10        */
   CUEVA_E_C2P_DRIFT = 26     /* drift-velocity fix still superluminal        */
 };
+
+/* Kinds of fix-up records (the reference's files on units 301, 302, 303). */
+enum { CUEVA_FIX_RHO = 0, CUEVA_FIX_P = 1, CUEVA_FIX_DRIFT = 2 };
 
 /* Boundary map per axis (the BOUND modes of 04_boundary.f95 are pairs of these:
  * BOUND 1 = copy/copy, BOUND 2 = periodic/periodic, BOUND 6 = copy x / periodic y;
@@ -150,8 +165,13 @@ int cueva_sumfin(cueva_ctx *ctx, int l);           /* 18_sumfin.f95:5          *
 int cueva_varfin(cueva_ctx *ctx);                  /* 19_varfin.f95:5          */
 int cueva_electricfield(cueva_ctx *ctx);           /* 20_electricfield.f95     */
 /* varprimitivecardano with l_swap = 1 (12_varprimitive.f95:6).  Waits for the
- * device; counters = {contador_r, contador_p, contador_v} of this call;
- * fallback = cells that fell back to Cardano in the learned mode (else 0). */
+ * device; counters = {contador_r, contador_p, contador_v} of this call, whose
+ * fix-up records cueva_fixup_records returns; fallback = cells that fell back
+ * to Cardano in the learned mode (else 0).  A superluminal previous velocity
+ * is clamped with the reference's eps1 (1e-6 once an HLL+MPx face solve ran
+ * in this ctx, else 0; 12_varprimitive.f95:236-241, 16_hll_flow_mpx.f95:1375).
+ * Returns the first failure of the step so far (face solves included).
+ * Ends a step of the per-subroutine sequence (the step index advances). */
 int cueva_c2p(cueva_ctx *ctx, int l_swap, int32_t counters[3], int32_t *fallback);
 
 /* nsteps whole steps (main_cueva_code.f95:176-332 per step), fused into a
@@ -163,8 +183,33 @@ int cueva_c2p(cueva_ctx *ctx, int l_swap, int32_t counters[3], int32_t *fallback
  * stage state) and ghost fills. */
 int cueva_step(cueva_ctx *ctx, int nsteps);
 /* Fix-up counters and Cardano fallbacks accumulated by cueva_step since the
- * last call (waits for the device). */
+ * last call (waits for the device); also clears the fix-up records. */
 int cueva_step_counters(cueva_ctx *ctx, int64_t counters[3], int64_t *fallback);
+
+/* The first failure (0 if none; waits for the device) and where it arose:
+ * where = {step index h, stage l (1-based; lmax for the primitive recovery),
+ * i, j} -- the reference's (h, l, i, j) of its stop message
+ * (12_varprimitive.f95:374-425,482-486,684-693; funciones.f95:238-244).
+ * After a failure inside cueva_step the planes hold the state the failing
+ * launch worked on (a face failure: U^n of the failing step; a recovery
+ * failure: U^{n+1}, with the recovered primitives at the cells that
+ * succeeded).  No reference counterpart beyond the stop message itself. */
+int cueva_last_error(cueva_ctx *ctx, int32_t where[4]);
+
+/* The step index the next step carries in failure locations and fix-up
+ * records (the reference's h; 1 after cueva_create, then +1 per fused step
+ * and per cueva_c2p call).  A driver that resumes at step hini sets it. */
+int cueva_set_step_index(cueva_ctx *ctx, int64_t h);
+
+/* Fix-up records of kind CUEVA_FIX_* since the counters were last cleared
+ * (cueva_cleanvarint, cueva_c2p, cueva_step_counters), in the order the
+ * reference writes them to its unit 301/302/303 file
+ * (12_varprimitive.f95:544,601,663): by step, then j, then i.  Per record
+ * hij = {h, i, j} and vals = the two values the reference prints (rho* and
+ * rho^n; p* and p^n; v*^2 and v_drift^2); the running count and x, y of its
+ * record follow from these.  *n = number of records; at most cap are
+ * copied.  CUEVA_E_NOMEM if more than 65536 records of the kind arose. */
+int cueva_fixup_records(cueva_ctx *ctx, int kind, int32_t *hij, double *vals, int64_t cap, int64_t *n);
 
 /* The 37 global diagnostics the reference writes every hglb steps
  * (22_writer.f95:818-991), in the order of its output units 80..217:

@@ -6,7 +6,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
+#include <vector>
 
 #include "cueva.h"
 #include "cueva_kernels.hpp"
@@ -15,15 +17,28 @@
 namespace cueva {
 // FP32 face kernels (cueva_mixed.hip)
 hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
-                             const Scheme& S, DevStatus* st);
+                             const Scheme& S, const StatusRef& st);
 // FP32 marching face kernels (cueva_mixed.hip)
 const void* march_kernel_fp32(int rec_order, int solver, bool final_stage);
 // learned primitive recovery (cueva_mlp.hip)
 hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
-                              DevStatus* st, bool stage0);
+                              const StatusRef& st, bool stage0);
 }
 
 using namespace cueva;
+
+// A launch that can fail, remembered until the status is next read: its
+// launch number, step and stage, and (fused step) the plane roles it ran
+// with, which become the state's again if it was the first to fail.
+struct LaunchRecord {
+  uint64_t seq;
+  int64_t h;
+  int l;
+  bool fused;
+  Planes P;
+};
+constexpr size_t MAX_LAUNCH_RECORDS = 16384;
+constexpr int FIX_CAP = 1 << 16;  // fix-up records kept per kind between reads
 
 struct cueva_ctx {
   cueva_params prm;
@@ -42,7 +57,13 @@ struct cueva_ctx {
   int march_rounds;  // resident-block rounds a marching launch is cut into (CUEVA_MARCH_ROUNDS)
   int march_rows;    // rows per segment when set (CUEVA_MARCH_SEG; rounded up to 4m - 1)
   int march_seg[2];  // rows per segment of the stage / last-stage kernel (0: not sized yet)
-  int err;  // sticky host-side error
+  // failures and fix-up records
+  FixRec* rec;          // device: [3][FIX_CAP], after mlpw
+  uint64_t seq;         // launches numbered so far
+  int64_t hstep;        // step number the next step carries (1 after create)
+  std::vector<LaunchRecord>* launches;
+  int err;              // first failure (sticky until the next upload of primitives)
+  int32_t err_where[4]; // its step, stage, i, j
   // kernel timing (cueva_set_profiling)
   int prof;
   int nev, capev;
@@ -233,8 +254,9 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   }
   const size_t status_bytes = (sizeof(DevStatus) + 255) / 256 * 256;
   const size_t red_bytes = (size_t)(RED_BLOCKS + 1) * G_N * sizeof(double);
-  const size_t mlp_bytes = (size_t)CUEVA_MLP_NPACK * sizeof(float);
-  if (hipMalloc(&c->dst, status_bytes + red_bytes + mlp_bytes) != hipSuccess) {
+  const size_t mlp_bytes = ((size_t)CUEVA_MLP_NPACK * sizeof(float) + 255) / 256 * 256;
+  const size_t rec_bytes = (size_t)3 * FIX_CAP * sizeof(FixRec);
+  if (hipMalloc(&c->dst, status_bytes + red_bytes + mlp_bytes + rec_bytes) != hipSuccess) {
     hipFree(c->base);
     hipStreamDestroy(c->stream);
     delete c;
@@ -242,6 +264,16 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   }
   c->red = (double*)((char*)c->dst + status_bytes);
   c->mlpw = (float*)((char*)c->red + red_bytes);
+  c->rec = (FixRec*)((char*)c->mlpw + mlp_bytes);
+  c->hstep = 1;
+  c->launches = new (std::nothrow) std::vector<LaunchRecord>();
+  if (!c->launches) {
+    hipFree(c->dst);
+    hipFree(c->base);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return CUEVA_E_NOMEM;
+  }
   double* q = c->base;
   auto take = [&]() { double* r = q; q += g.plane; return r; };
   for (int k = 0; k < P_N; k++) c->P.p[k] = take();
@@ -266,6 +298,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     for (int k = 0; k < nptr; k++) {
       const char* a = (const char*)ptrs[k];
       if (a < lo || a + g.plane * (int64_t)sizeof(double) > hi) {
+        delete c->launches;
         hipFree(c->dst);
         hipFree(c->base);
         hipStreamDestroy(c->stream);
@@ -277,6 +310,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   if (hipMemsetAsync(c->base, 0, (size_t)c->bytes, c->stream) != hipSuccess ||
       hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
+    delete c->launches;
     hipFree(c->dst);
     hipFree(c->base);
     hipStreamDestroy(c->stream);
@@ -290,6 +324,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
 extern "C" int cueva_destroy(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
   hipStreamSynchronize(c->stream);
+  delete c->launches;
   for (int k = 0; k < 2 * c->capev; k++) hipEventDestroy(c->ev[k]);
   free(c->ev);
   free(c->evcls);
@@ -342,9 +377,32 @@ static dim3 interior_grid(const Geom& g, dim3 blk) {
 
 static int launch_ok() { return hipGetLastError() == hipSuccess ? CUEVA_OK : CUEVA_E_HIP; }
 
+// Number the next launch.  record: a launch that can fail (face solves,
+// primitive recovery), remembered with its step and stage l (1-based) and,
+// for the fused step, the plane roles it runs with.
+static StatusRef sref(cueva_ctx* c, int l, bool record = false, bool fused = false) {
+  StatusRef r;
+  r.st = c->dst;
+  r.rec = c->rec;
+  r.seq = ++c->seq;
+  r.h = (int)c->hstep;
+  r.cap = FIX_CAP;
+  if (record && c->launches->size() < MAX_LAUNCH_RECORDS) {
+    LaunchRecord lr;
+    lr.seq = r.seq;
+    lr.h = c->hstep;
+    lr.l = l;
+    lr.fused = fused;
+    lr.P = c->P;
+    c->launches->push_back(lr);
+  }
+  return r;
+}
+
 // copy every ghost point of the planes from its interior source point; the
 // last nzero planes (at most 32) are set to 0 there instead
 static int fill_ghosts(cueva_ctx* c, double* const* planes, int n, int nzero = 0) {
+  const StatusRef sr = sref(c, 0);
   PlaneList pl;
   memset(&pl, 0, sizeof(pl));
   for (int start = 0; start < n; start += 32) {
@@ -352,7 +410,7 @@ static int fill_ghosts(cueva_ctx* c, double* const* planes, int n, int nzero = 0
     pl.nzero = start + pl.n == n ? nzero : 0;
     for (int k = 0; k < pl.n; k++) pl.a[k] = planes[start + k];
     int64_t cnt = ghost_count(c->g);
-    LAUNCH(K_GHOSTS, k_fill_ghosts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), c->g, pl);
+    LAUNCH(K_GHOSTS, k_fill_ghosts, dim3((unsigned)((cnt + 255) / 256)), dim3(256), c->g, pl, sr);
     if (launch_ok()) return CUEVA_E_HIP;
   }
   return CUEVA_OK;
@@ -360,6 +418,11 @@ static int fill_ghosts(cueva_ctx* c, double* const* planes, int n, int nzero = 0
 
 extern "C" int cueva_upload_prims(cueva_ctx* c, const double* prims14) {
   if (!c || !prims14) return CUEVA_E_ARG;
+  // a new state: the first failure, counters and records start again
+  HIPCHK(hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->err = 0;
+  c->launches->clear();
   int rc = copy_planes_h2d(c, c->P.p, P_N, prims14);
   if (rc) return rc;
   return fill_ghosts(c, c->P.p, P_N);
@@ -408,9 +471,9 @@ extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
 }
 
 static int recons_prims(cueva_ctx* c) {
-  LAUNCH(K_OTHER, k_recons_prim, interior_grid(c->g, BLK), BLK, c->g, c->P);
+  LAUNCH(K_OTHER, k_recons_prim, interior_grid(c->g, BLK), BLK, c->g, c->P, sref(c, 0));
   if (launch_ok()) return CUEVA_E_HIP;
-  LAUNCH(K_OTHER, k_recons_copy, interior_grid(c->g, BLK), BLK, c->g, c->P);
+  LAUNCH(K_OTHER, k_recons_copy, interior_grid(c->g, BLK), BLK, c->g, c->P, sref(c, 0));
   return launch_ok();
 }
 
@@ -425,12 +488,18 @@ extern "C" int cueva_startconvar(cueva_ctx* c) {
   return launch_ok();
 }
 
+// counters and fix-up records (not the first failure)
+static int clear_counters(cueva_ctx* c) {
+  HIPCHK(hipMemsetAsync((char*)c->dst + STATUS_COUNTERS_OFFSET, 0, sizeof(DevStatus) - STATUS_COUNTERS_OFFSET,
+                        c->stream));
+  return CUEVA_OK;
+}
+
 extern "C" int cueva_cleanvarint(cueva_ctx* c) {
   if (!c) return CUEVA_E_ARG;
   LAUNCH(K_OTHER, k_cleanvarint, interior_grid(c->g, BLK), BLK, c->g, c->P);
   if (launch_ok()) return CUEVA_E_HIP;
-  HIPCHK(hipMemsetAsync(c->dst, 0, sizeof(DevStatus), c->stream));  // contador_* = 0
-  return CUEVA_OK;
+  return clear_counters(c);  // contador_* = 0 (main_cueva_code.f95:189-193)
 }
 
 // swap1 copies the primitives into stage-1 arrays that only REC_PRIM=1 and
@@ -466,40 +535,49 @@ extern "C" int cueva_flowinit(cueva_ctx* c, int l) {
 }
 
 template <int ORD>
-static int launch_faces_ord(cueva_ctx* c) {
+static int launch_faces_ord(cueva_ctx* c, const StatusRef& sr) {
   const dim3 grid = faces_grid(c->g, BLK);
   if (c->prm.flux_solver == CUEVA_LAX)
-    LAUNCH(K_FACES, (k_faces<ORD, 1>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 1>), grid, BLK, c->g, c->P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLC_MCL)
-    LAUNCH(K_FACES, (k_faces<ORD, 4>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 4>), grid, BLK, c->g, c->P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLE_MCL)
-    LAUNCH(K_FACES, (k_faces<ORD, 3>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 3>), grid, BLK, c->g, c->P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLL_MCL)
-    LAUNCH(K_FACES, (k_faces<ORD, 2>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 2>), grid, BLK, c->g, c->P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLC)
-    LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLE)
-    LAUNCH(K_FACES, (k_faces<ORD, 6>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 6>), grid, BLK, c->g, c->P, c->S, sr);
   else
-    LAUNCH(K_FACES, (k_faces<ORD, 5>), grid, BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(K_FACES, (k_faces<ORD, 5>), grid, BLK, c->g, c->P, c->S, sr);
   return launch_ok();
 }
 
-static int launch_faces(cueva_ctx* c) {
+// hll_flow_mpx sets the module variable eps1 that the recovery's clamp of a
+// superluminal previous velocity reads (16_hll_flow_mpx.f95:1375)
+static void note_face_solver(cueva_ctx* c) {
+  if (c->prm.flux_solver == CUEVA_HLL) c->S.eps1 = 1.0e-6;
+}
+
+// the face solves of stage l (1-based); fused: inside cueva_step
+static int launch_faces(cueva_ctx* c, int l, bool fused) {
+  const StatusRef sr = sref(c, l, true, fused);
+  note_face_solver(c);
   if (c->prm.precision == CUEVA_MIXED) {
     const dim3 grid = faces_grid(c->g, BLK);
     prof_begin(c, K_FACES);
-    hipError_t e = launch_faces_fp32(c->rec_order, grid, BLK, c->stream, c->g, c->P, c->S, c->dst);
+    hipError_t e = launch_faces_fp32(c->rec_order, grid, BLK, c->stream, c->g, c->P, c->S, sr);
     prof_end(c);
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
   if (c->prm.flux_solver == CUEVA_HLL_MCL || c->prm.flux_solver == CUEVA_LAX ||
       c->prm.flux_solver == CUEVA_HLLC_MCL || c->prm.flux_solver == CUEVA_HLLE_MCL)
-    return launch_faces_ord<5>(c);  // the MP order does not enter
+    return launch_faces_ord<5>(c, sr);  // the MP order does not enter
   switch (c->rec_order) {
-    case 5: return launch_faces_ord<5>(c);
-    case 7: return launch_faces_ord<7>(c);
-    default: return launch_faces_ord<9>(c);
+    case 5: return launch_faces_ord<5>(c, sr);
+    case 7: return launch_faces_ord<7>(c, sr);
+    default: return launch_faces_ord<9>(c, sr);
   }
 }
 
@@ -510,15 +588,15 @@ extern "C" int cueva_flux(cueva_ctx* c, int l) {
   if ((rc = cueva_boundary(c, CUEVA_BND_CONS, l))) return rc;
   if ((rc = cueva_boundary(c, CUEVA_BND_ELEC, l))) return rc;
   if ((rc = cueva_boundary(c, CUEVA_BND_FLUX, l))) return rc;
-  return launch_faces(c);
+  return launch_faces(c, l, false);
 }
 
 extern "C" int cueva_fluxandsource(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
   if (c->prm.precision == CUEVA_MIXED)
-    LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
+    LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1, sref(c, l));
   else
-    LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
+    LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1, sref(c, l));
   return launch_ok();
 }
 
@@ -540,25 +618,47 @@ extern "C" int cueva_electricfield(cueva_ctx* c) {
   return launch_ok();
 }
 
+// Wait for the device and read the status words.  A first failure found here
+// becomes the context's (code and where: step, stage, i, j); if it arose in
+// the fused step, the plane roles go back to those the failing launch ran
+// with (the launches after it changed no data, they only rotated roles).
 static int read_status(cueva_ctx* c, DevStatus* h) {
   HIPCHK(hipMemcpyAsync(h, c->dst, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (h->seq != 0 && !c->err) {
+    const unsigned long long key = ~h->key;
+    c->err = (int)(key & 0xFF);
+    c->err_where[0] = -1;
+    c->err_where[1] = -1;
+    c->err_where[2] = (int32_t)((key >> 8) & 0xFFFFF) - 8;
+    c->err_where[3] = (int32_t)(key >> 28) - 8;
+    for (const LaunchRecord& lr : *c->launches)
+      if (lr.seq == h->seq) {
+        c->err_where[0] = (int32_t)lr.h;
+        c->err_where[1] = lr.l;
+        if (lr.fused) c->P = lr.P;
+        break;
+      }
+  }
+  c->launches->clear();
   return CUEVA_OK;
 }
 
-// stage0: also the next step's stage-0 state at the interior points
-static int launch_c2p(cueva_ctx* c, int cls = K_OTHER, bool stage0 = false) {
+// Primitive recovery after the last stage (l_swap = 1).  fused: inside
+// cueva_step; stage0: also the next step's stage-0 state at the interior.
+static int launch_c2p(cueva_ctx* c, bool fused, int cls = K_OTHER, bool stage0 = false) {
+  const StatusRef sr = sref(c, c->S.lmax, true, fused);
   if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
     if (!c->has_mlp) return CUEVA_E_ARG;
     prof_begin(c, cls);
-    hipError_t e = launch_c2p_learned(c->stream, c->g, c->P, c->S, c->mlpw, c->dst, stage0);
+    hipError_t e = launch_c2p_learned(c->stream, c->g, c->P, c->S, c->mlpw, sr, stage0);
     prof_end(c);
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
   if (stage0)
-    LAUNCH(cls, k_c2p<true>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(cls, k_c2p<true>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, sr);
   else
-    LAUNCH(cls, k_c2p<false>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, c->dst);
+    LAUNCH(cls, k_c2p<false>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, sr);
   return launch_ok() ? CUEVA_E_HIP : CUEVA_OK;
 }
 
@@ -572,14 +672,18 @@ extern "C" int cueva_set_c2p_params(cueva_ctx* c, const float* packed) {
 
 extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t* fallback) {
   if (!c || l_swap != 1) return CUEVA_E_ARG;
-  int lrc = launch_c2p(c);
-  if (lrc) return lrc;
-  DevStatus h;
-  int rc = read_status(c, &h);
+  if (c->err) return c->err;
+  // the step's counters and records start here (the reference zeroes the
+  // counters after cleanvarint and nothing else counts before the recovery)
+  int rc = clear_counters(c);
   if (rc) return rc;
+  if ((rc = launch_c2p(c, false))) return rc;
+  DevStatus h;
+  if ((rc = read_status(c, &h))) return rc;
+  c->hstep++;  // a step of the per-subroutine sequence ends with this call
   if (counters) for (int k = 0; k < 3; k++) counters[k] = (int32_t)h.cnt[k];
   if (fallback) *fallback = (int32_t)h.fallback;
-  return h.err ? h.err : CUEVA_OK;
+  return c->err;
 }
 
 // ---------------------------------------------------------------------------
@@ -666,7 +770,8 @@ static int launch_march(cueva_ctx* c, int l, bool fin) {
   ma.S = c->S;
   ma.l = l;
   ma.seg = seg;
-  ma.st = c->dst;
+  ma.st = sref(c, l + 1, true, true);
+  note_face_solver(c);
   void* args[] = {(void*)&ma};
   prof_begin(c, K_FACES);
   hipError_t e = hipLaunchKernel(fn, grid, dim3(64, (unsigned)march_waves(c)), args, 0, c->stream);
@@ -674,14 +779,28 @@ static int launch_march(cueva_ctx* c, int l, bool fin) {
   return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
 }
 
+static int fused_steps(cueva_ctx* c, int nsteps, int halo, bool learned);
+
 extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return CUEVA_E_ARG;
-  const Geom& g = c->g;
   const int halo = (c->rec_order - 1) / 2 + 1;  // stencil radius K plus the face offset
   const bool learned = c->prm.c2p_mode == CUEVA_C2P_LEARNED;
   if (learned && !c->has_mlp) return CUEVA_E_ARG;
+  if (c->err) return c->err;
+  int rc0 = fused_steps(c, nsteps, halo, learned);
+  // one wait per call: a failure stops the call's later launches on the
+  // device, and its status, step, stage and cell are known here
+  DevStatus h;
+  int rc = read_status(c, &h);
+  if (rc0) return rc0;
+  if (rc) return rc;
+  return c->err;
+}
+
+static int fused_steps(cueva_ctx* c, int nsteps, int halo, bool learned) {
+  const Geom& g = c->g;
   bool s0_ready = false;  // this step's stage-0 state was formed by the previous step's recovery pass
-  for (int h = 0; h < nsteps; h++) {
+  for (int h = 0; h < nsteps; h++, c->hstep++) {
     if (c->prm.primitive_rec) {  // main_cueva_code.f95:170-175
       int rc = recons_prims(c);
       if (!rc) rc = fill_ghosts(c, c->P.p, P_N);
@@ -694,7 +813,7 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
       if (!have_stage) {
         dim3 gs((unsigned)((g.imax + 1 + 2 * halo + BLK.x - 1) / BLK.x),
                 (unsigned)((g.jmax + 1 + 2 * halo + BLK.y - 1) / BLK.y));
-        LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo);
+        LAUNCH(K_STAGE, k_stage_fused, gs, BLK, c->g, c->P, c->S, l, halo, sref(c, l + 1));
         if (launch_ok()) return CUEVA_E_HIP;
       }
       if (c->march) {
@@ -715,32 +834,36 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
         // the primitive recovery also forms the next step's stage-0 state
         // (not when the next step first re-forms the primitives)
         const bool fuse = h + 1 < nsteps && !c->prm.primitive_rec;
-        if ((rc = launch_c2p(c, K_FINAL, fuse))) return rc;
+        if ((rc = launch_c2p(c, true, K_FINAL, fuse))) return rc;
         // ghosts of the new primitives (and of the next stage-1 state)
         if ((rc = fuse ? fill_stage_ghosts(c, true) : fill_ghosts(c, c->P.p, P_N))) return rc;
         s0_ready = fuse;
         continue;
       }
-      int rc = launch_faces(c);
+      int rc = launch_faces(c, l + 1, true);
       if (rc) return rc;
       const bool mixed = c->prm.precision == CUEVA_MIXED;
       if (!last) {
+        const StatusRef sr = sref(c, l + 1);
         if (mixed)
-          LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
+          LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l, sr);
         else
-          LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l);
-      } else if (learned) {
-        if (mixed)
-          LAUNCH(K_FINAL, k_final_learned<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
-                 (const float*)c->mlpw, c->dst);
-        else
-          LAUNCH(K_FINAL, k_final_learned<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
-                 (const float*)c->mlpw, c->dst);
+          LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, l, sr);
       } else {
-        if (mixed)
-          LAUNCH(K_FINAL, k_final_fused<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
-        else
-          LAUNCH(K_FINAL, k_final_fused<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, c->dst);
+        const StatusRef sr = sref(c, l + 1, true, true);
+        if (learned) {
+          if (mixed)
+            LAUNCH(K_FINAL, k_final_learned<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
+                   (const float*)c->mlpw, sr);
+          else
+            LAUNCH(K_FINAL, k_final_learned<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S,
+                   (const float*)c->mlpw, sr);
+        } else {
+          if (mixed)
+            LAUNCH(K_FINAL, k_final_fused<float>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, sr);
+          else
+            LAUNCH(K_FINAL, k_final_fused<double>, interior_grid(g, BLK), BLK, c->g, c->P, c->S, sr);
+        }
       }
       if (launch_ok()) return CUEVA_E_HIP;
     }
@@ -759,12 +882,55 @@ extern "C" int cueva_step_counters(cueva_ctx* c, int64_t counters[3], int64_t* f
   if (rc) return rc;
   if (counters) for (int k = 0; k < 3; k++) counters[k] = (int64_t)h.cnt[k];
   if (fallback) *fallback = (int64_t)h.fallback;
-  DevStatus z;
-  memset(&z, 0, sizeof(z));
-  z.err = h.err; z.err_i = h.err_i; z.err_j = h.err_j;
-  HIPCHK(hipMemcpyAsync(c->dst, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  if ((rc = clear_counters(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  return h.err ? h.err : CUEVA_OK;
+  return c->err;
+}
+
+extern "C" int cueva_last_error(cueva_ctx* c, int32_t where[4]) {
+  if (!c) return CUEVA_E_ARG;
+  DevStatus h;
+  int rc = read_status(c, &h);
+  if (rc) return rc;
+  if (where)
+    for (int k = 0; k < 4; k++) where[k] = c->err ? c->err_where[k] : 0;
+  return c->err;
+}
+
+extern "C" int cueva_set_step_index(cueva_ctx* c, int64_t h) {
+  if (!c || h < 1 || h > INT32_MAX) return CUEVA_E_ARG;
+  c->hstep = h;
+  return CUEVA_OK;
+}
+
+extern "C" int cueva_fixup_records(cueva_ctx* c, int kind, int32_t* hij, double* vals, int64_t cap, int64_t* n) {
+  if (!c || kind < 0 || kind > 2 || !n || cap < 0 || (cap > 0 && (!hij || !vals))) return CUEVA_E_ARG;
+  DevStatus h;
+  int rc = read_status(c, &h);
+  if (rc) return rc;
+  const int64_t total = (int64_t)h.nrec[kind];
+  *n = total;
+  if (total > FIX_CAP) return CUEVA_E_NOMEM;  // more than the device kept
+  std::vector<FixRec> r((size_t)total);
+  if (total) {
+    HIPCHK(hipMemcpyAsync(r.data(), c->rec + (size_t)kind * FIX_CAP, (size_t)total * sizeof(FixRec),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  // the reference's order: steps in turn, then its loop order (j outer, i inner)
+  std::sort(r.begin(), r.end(), [](const FixRec& a, const FixRec& b) {
+    if (a.h != b.h) return a.h < b.h;
+    if (a.j != b.j) return a.j < b.j;
+    return a.i < b.i;
+  });
+  for (int64_t e = 0; e < total && e < cap; e++) {
+    hij[3 * e] = r[e].h;
+    hij[3 * e + 1] = r[e].i;
+    hij[3 * e + 2] = r[e].j;
+    vals[2 * e] = r[e].a;
+    vals[2 * e + 1] = r[e].b;
+  }
+  return CUEVA_OK;
 }
 
 extern "C" int cueva_upload_cons(cueva_ctx* c, const double* cons5) {

@@ -47,16 +47,78 @@ struct Scheme {
   double wb[MAXL], wbt[MAXL], Ab[MAXL][MAXL], Abt[MAXL][MAXL];
   int hllc_seed_ref;
   int crmath;
+  // the reference's module variable eps1 that clamps a superluminal previous
+  // velocity in the recovery (12_varprimitive.f95:236-241): 1e-6 once
+  // hll_flow_mpx has run in the process (16_hll_flow_mpx.f95:1375), else 0
+  double eps1;
 };
 
 // status words in device memory
 struct DevStatus {
-  int err;          // first error code (0 none)
-  int err_i, err_j; // where
-  int pad;
-  unsigned long long cnt[3];  // contador_r, contador_p, contador_v
+  // the first failure, kept until the host clears it
+  unsigned long long seq;  // launch number of the launch that failed first (0: none)
+  unsigned long long key;  // ~(first failing cell of that launch in the reference's loop order, see fail_key)
+  // counters, cleared per step by the host (main_cueva_code.f95:189-193)
+  unsigned long long cnt[3];   // contador_r, contador_p, contador_v
   unsigned long long fallback;
+  unsigned int nrec[3];        // fix-up records of each kind (may exceed the capacity)
+  unsigned int pad;
 };
+constexpr size_t STATUS_COUNTERS_OFFSET = 2 * sizeof(unsigned long long);
+
+// one record of the reference's fix-up files 301-303 (12_varprimitive.f95:544,601,663)
+struct FixRec {
+  int h, i, j, pad;
+  double a, b;  // rho* and rho^n / p* and p^n / v*^2 and v_drift^2
+};
+
+// what a launch that can fail carries: where to report, its launch number
+// (launches are numbered in stream order) and the step it belongs to
+struct StatusRef {
+  DevStatus* st;
+  FixRec* rec;  // [3][cap]
+  unsigned long long seq;
+  int h;
+  int cap;
+};
+
+// Cells are ordered as the reference's loops visit them (j outer, i inner),
+// so the smallest key of a launch is the cell at which the reference stops.
+__device__ inline unsigned long long fail_key(int code, int i, int j) {
+  return ((unsigned long long)(j + 8) << 28) | ((unsigned long long)(i + 8) << 8) | (unsigned long long)code;
+}
+
+// Record a failure.  Only the launch that failed first reports; within it the
+// first cell in loop order wins.
+__device__ inline void report(const StatusRef& r, int code, int i, int j) {
+  const unsigned long long s0 = atomicCAS(&r.st->seq, 0ULL, r.seq);
+  if (s0 != 0ULL && s0 != r.seq) return;
+  atomicMax(&r.st->key, ~fail_key(code, i, j));
+}
+
+// An earlier launch failed: the step stops there and later launches leave
+// the state alone.  The answer is the same for every thread of a launch (only
+// earlier launches can have set a smaller launch number), so kernels with
+// barriers may return on it.
+__device__ inline bool halted(const StatusRef& r) {
+  const unsigned long long s = __atomic_load_n(&r.st->seq, __ATOMIC_RELAXED);
+  return s != 0ULL && s < r.seq;
+}
+
+// a fix-up: counter and record (kind 0 rho, 1 p, 2 drift)
+__device__ inline void add_fixup(const StatusRef& r, int kind, int i, int j, double a, double b) {
+  atomicAdd(&r.st->cnt[kind], 1ULL);
+  const unsigned n = atomicAdd(&r.st->nrec[kind], 1u);
+  if ((int)n < r.cap) {
+    FixRec& e = r.rec[(size_t)kind * r.cap + n];
+    e.h = r.h;
+    e.i = i;
+    e.j = j;
+    e.pad = 0;
+    e.a = a;
+    e.b = b;
+  }
+}
 
 // XCD-aware tile order for a 1D launch of 8 * ceil(n/8) workgroups over n =
 // tiles_x * tiles_y tiles: the hardware hands workgroup w to XCD w % 8, and
@@ -72,13 +134,6 @@ __device__ inline bool xcd_tile(int tiles_x, int tiles_y, int& bx, int& by) {
   bx = t % tiles_x;
   by = t / tiles_x;
   return true;
-}
-
-__device__ inline void report(DevStatus* st, int code, int i, int j) {
-  if (atomicCAS(&st->err, 0, code) == 0) {
-    st->err_i = i;
-    st->err_j = j;
-  }
 }
 
 // U^n of accumulated field a (in A_* order) at offset k

@@ -233,6 +233,11 @@ enum FyIdx { FY_EYPSI, FY_BYPHI, FY_EZBX, FY_PHIBY, FY_EXBZ, FY_BZEX, FY_PSIEY, 
              FY_FTAU, FY_FSX, FY_FSY, FY_FSZ, FY_N };
 
 // (s2*FL - s1*FR + s1*s2*(UR-UL))/(s2-s1), s1 = -1, s2 = 1 (parameters.f95:283-284)
+// The reference's "_m" slot of a field: the MPx reconstruction of -u, which
+// equals -MPx(u) except that a zero comes out as +0 (16_hll_flow_mpx.f95:
+// 784-917 packs -E and -B into their own slots); 0 - x is exactly that.
+template <typename T> __device__ inline T msl(T x) { return T(0) - x; }
+
 template <typename T> __device__ inline T hllf(T fl, T fr, T ul, T ur) {
   const T S1 = T(-1), S2 = T(1);
   return ((S2 * fl - S1 * fr) + (S1 * S2) * (ur - ul)) / (S2 - S1);
@@ -251,11 +256,11 @@ template <typename T> __device__ inline void hll_x(const Side<T>& L, const Side<
   f[FX_EXPSI] = hllf(L.ex, R.ex, L.psi, R.psi);
   f[FX_BXPHI] = hllf(L.bx, R.bx, L.phi, R.phi);
   f[FX_PHIBX] = hllf(L.phi, R.phi, L.bx, R.bx);
-  f[FX_EZBY] = -hllf(-L.ez, -R.ez, L.by, R.by);
+  f[FX_EZBY] = -hllf(msl(L.ez), msl(R.ez), L.by, R.by);
   f[FX_EYBZ] = hllf(L.ey, R.ey, L.bz, R.bz);
   f[FX_PSIEX] = hllf(L.psi, R.psi, L.ex, R.ex);
   f[FX_BZEY] = hllf(L.bz, R.bz, L.ey, R.ey);
-  f[FX_BYEZ] = -hllf(-L.by, -R.by, L.ez, R.ez);
+  f[FX_BYEZ] = -hllf(msl(L.by), msl(R.by), L.ez, R.ez);
   f[FX_J] = hllf(L.jn, R.jn, L.q, R.q);
   f[FX_FD] = hllf(L.fd, R.fd, L.d, R.d);
   f[FX_FTAU] = hllf(L.ftau, R.ftau, L.tau, R.tau);
@@ -268,8 +273,8 @@ template <typename T> __device__ inline void hll_y(const Side<T>& L, const Side<
   f[FY_BYPHI] = hllf(L.by, R.by, L.phi, R.phi);
   f[FY_EZBX] = hllf(L.ez, R.ez, L.bx, R.bx);
   f[FY_PHIBY] = hllf(L.phi, R.phi, L.by, R.by);
-  f[FY_EXBZ] = -hllf(-L.ex, -R.ex, L.bz, R.bz);
-  f[FY_BZEX] = -hllf(-L.bz, -R.bz, L.ex, R.ex);
+  f[FY_EXBZ] = -hllf(msl(L.ex), msl(R.ex), L.bz, R.bz);
+  f[FY_BZEX] = -hllf(msl(L.bz), msl(R.bz), L.ex, R.ex);
   f[FY_PSIEY] = hllf(L.psi, R.psi, L.ey, R.ey);
   f[FY_BXEZ] = hllf(L.bx, R.bx, L.ez, R.ez);
   f[FY_J] = hllf(L.jn, R.jn, L.q, R.q);
@@ -336,8 +341,8 @@ template <typename T, bool MC = false>
 __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref, T mcref = T(0),
                              T* vlast = nullptr) {
   const T S1 = T(-1), S2 = T(1);
-  const T ezmL = MC ? L.em : -L.ez, ezmR = MC ? R.em : -R.ez;
-  const T bymL = MC ? L.bm : -L.by, bymR = MC ? R.bm : -R.by;
+  const T ezmL = MC ? L.em : msl(L.ez), ezmR = MC ? R.em : msl(R.ez);
+  const T bymL = MC ? L.bm : msl(L.by), bymR = MC ? R.bm : msl(R.by);
   T psi_h = hll_state(L.psi, R.psi, L.ex, R.ex);
   T phi_h = hll_state(L.phi, R.phi, L.bx, R.bx);
   T ex_h = hll_state(L.ex, R.ex, L.psi, R.psi);
@@ -439,8 +444,8 @@ __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed
 template <typename T, bool MC = false>
 __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed_ref, T mcref = T(0)) {
   const T S1 = T(-1), S2 = T(1);
-  const T exmL = MC ? L.em : -L.ex, exmR = MC ? R.em : -R.ex;
-  const T bzmL = MC ? L.bm : -L.bz, bzmR = MC ? R.bm : -R.bz;
+  const T exmL = MC ? L.em : msl(L.ex), exmR = MC ? R.em : msl(R.ex);
+  const T bzmL = MC ? L.bm : msl(L.bz), bzmR = MC ? R.bm : msl(R.bz);
   T psi_h = hll_state(L.psi, R.psi, L.ey, R.ey);
   T phi_h = hll_state(L.phi, R.phi, L.by, R.by);
   T ey_h = hll_state(L.ey, R.ey, L.psi, R.psi);
@@ -582,8 +587,8 @@ template <typename T> __device__ inline T hlle_flux(T a, T b, T fl, T fr, T ul, 
 // contact solve in the flang -O2 reference build (still in the register)
 template <typename T, bool MC = false> __device__ inline T hlle_y_ery(const Side<T>& L, const Side<T>& R) {
   T ez_h = hll_state(L.ez, R.ez, L.bx, R.bx);
-  T ex_h = hll_state(L.ex, R.ex, MC ? L.bm : -L.bz, MC ? R.bm : -R.bz);
-  T bz_h = hll_state(L.bz, R.bz, MC ? L.em : -L.ex, MC ? R.em : -R.ex);
+  T ex_h = hll_state(L.ex, R.ex, MC ? L.bm : msl(L.bz), MC ? R.bm : msl(R.bz));
+  T bz_h = hll_state(L.bz, R.bz, MC ? L.em : msl(L.ex), MC ? R.em : msl(R.ex));
   T bx_h = hll_state(L.bx, R.bx, L.ez, R.ez);
   return bx_h * ez_h - bz_h * ex_h;
 }
@@ -601,33 +606,33 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
   if constexpr (DIR == 0) {
     e[0] = hll_state(L.ex, R.ex, L.psi, R.psi);
     e[1] = hll_state(L.ey, R.ey, L.bz, R.bz);
-    e[2] = hll_state(L.ez, R.ez, -L.by, -R.by);
+    e[2] = hll_state(L.ez, R.ez, msl(L.by), msl(R.by));
     b[0] = hll_state(L.bx, R.bx, L.phi, R.phi);
-    b[1] = hll_state(L.by, R.by, -L.ez, -R.ez);
+    b[1] = hll_state(L.by, R.by, msl(L.ez), msl(R.ez));
     b[2] = hll_state(L.bz, R.bz, L.ey, R.ey);
     f[FX_EXPSI] = hll_flux(L.ex, R.ex, L.psi, R.psi);
     f[FX_BXPHI] = hll_flux(L.bx, R.bx, L.phi, R.phi);
     f[FX_PHIBX] = hll_flux(L.phi, R.phi, L.bx, R.bx);
-    f[FX_EZBY] = -hll_flux(-L.ez, -R.ez, L.by, R.by);
+    f[FX_EZBY] = -hll_flux(msl(L.ez), msl(R.ez), L.by, R.by);
     f[FX_EYBZ] = hll_flux(L.ey, R.ey, L.bz, R.bz);
     f[FX_PSIEX] = hll_flux(L.psi, R.psi, L.ex, R.ex);
     f[FX_BZEY] = hll_flux(L.bz, R.bz, L.ey, R.ey);
-    f[FX_BYEZ] = -hll_flux(-L.by, -R.by, L.ez, R.ez);
+    f[FX_BYEZ] = -hll_flux(msl(L.by), msl(R.by), L.ez, R.ez);
   } else {
     e[1] = hll_state(L.ey, R.ey, L.psi, R.psi);
     e[2] = hll_state(L.ez, R.ez, L.bx, R.bx);
-    e[0] = hll_state(L.ex, R.ex, -L.bz, -R.bz);
+    e[0] = hll_state(L.ex, R.ex, msl(L.bz), msl(R.bz));
     b[1] = hll_state(L.by, R.by, L.phi, R.phi);
-    b[2] = hll_state(L.bz, R.bz, -L.ex, -R.ex);
+    b[2] = hll_state(L.bz, R.bz, msl(L.ex), msl(R.ex));
     b[0] = hll_state(L.bx, R.bx, L.ez, R.ez);
     f[FY_EYPSI] = hll_flux(L.ey, R.ey, L.psi, R.psi);
     f[FY_BYPHI] = hll_flux(L.by, R.by, L.phi, R.phi);
     f[FY_PHIBY] = hll_flux(L.phi, R.phi, L.by, R.by);
-    f[FY_EXBZ] = -hll_flux(-L.ex, -R.ex, L.bz, R.bz);
+    f[FY_EXBZ] = -hll_flux(msl(L.ex), msl(R.ex), L.bz, R.bz);
     f[FY_EZBX] = hll_flux(L.ez, R.ez, L.bx, R.bx);
     f[FY_PSIEY] = hll_flux(L.psi, R.psi, L.ey, R.ey);
     f[FY_BXEZ] = hll_flux(L.bx, R.bx, L.ez, R.ez);
-    f[FY_BZEX] = -hll_flux(-L.bz, -R.bz, L.ex, R.ex);
+    f[FY_BZEX] = -hll_flux(msl(L.bz), msl(R.bz), L.ex, R.ex);
   }
   T sa, sb;
   int code = hlle_speeds<DIR>(gam, L, R, sa, sb);
@@ -932,16 +937,23 @@ __device__ inline void ohm_source(double sig, double w, double vx, double vy, do
 struct C2PIn {
   double d, tau, sx, sy, sz, ex, ey, ez, bx, by, bz, vx0, vy0, vz0, rho_old, p_old;
 };
-struct C2POut { double vx, vy, vz, rho, p; int fix_rho, fix_p, fix_v; };
+struct C2POut {
+  double vx, vy, vz, rho, p;
+  int fix_rho, fix_p, fix_v;
+  double rec[6];  // the values each fix-up record prints: rho* rho^n, p* p^n, v*^2 v_drift^2
+};
 
 // Quartic coefficients in W (12_varprimitive.f95:245-296)
 struct Quartic { double e2b2, g1, v2old, b3, b2, b1, b0, d2, d1, d0, c1, c2; };
 
-__device__ inline Quartic c2p_quartic(double gamma, const C2PIn& in) {
+// eps1: the clamp of a superluminal previous velocity (12_varprimitive.f95:236-241),
+// whose square enters only the pressure loop
+__device__ inline Quartic c2p_quartic(double gamma, double eps1, const C2PIn& in) {
   Quartic qd;
   qd.e2b2 = 0.5 * (((((sq(in.ex) + sq(in.ey)) + sq(in.ez)) + sq(in.bx)) + sq(in.by)) + sq(in.bz));
   qd.g1 = (gamma - 1.0) / gamma;
   qd.v2old = (sq(in.vx0) + sq(in.vy0)) + sq(in.vz0);
+  if (qd.v2old > 1.0) qd.v2old = mx(-1.0 + eps1, mn(1.0 - eps1, qd.v2old));
   double C1 = (sq((in.sx - in.ey * in.bz) + in.ez * in.by) + sq((in.sy - in.ez * in.bx) + in.ex * in.bz)) +
               sq((in.sz - in.ex * in.by) + in.ey * in.bx);
   double C2 = in.tau - qd.e2b2;
@@ -1046,7 +1058,12 @@ __device__ inline int c2p_finish(double gamma, const C2PIn& in, const Quartic& q
   double W = Wn;
   o.fix_rho = o.fix_p = o.fix_v = 0;
   double rho = D / W;
-  if (rho < 0.0) { o.fix_rho = 1; rho = mx(in.rho_old * 1.0e-1, rho); }
+  if (rho < 0.0) {
+    o.fix_rho = 1;
+    o.rec[0] = rho;
+    o.rec[1] = in.rho_old;
+    rho = mx(in.rho_old * 1.0e-1, rho);
+  }
   double h = ((tau - E2B2) - g1 * rho) / (sq(W) - g1);
   double p = g1 * (h - rho);
   double pn = p;
@@ -1060,7 +1077,12 @@ __device__ inline int c2p_finish(double gamma, const C2PIn& in, const Quartic& q
     pn = pr;
     nr = nr + 1;
   }
-  if (pn < 0.0) { o.fix_p = 1; pn = mx(in.p_old * 1.0e-4, pn); }
+  if (pn < 0.0) {
+    o.fix_p = 1;
+    o.rec[2] = pn;
+    o.rec[3] = in.p_old;
+    pn = mx(in.p_old * 1.0e-4, pn);
+  }
   p = pn;
   tau = (E2B2 + h * sq(W)) - p;
   double den = (tau - E2B2) + p;
@@ -1086,15 +1108,16 @@ __device__ inline int c2p_finish(double gamma, const C2PIn& in, const Quartic& q
     vz = ((in.sz - in.ex * in.by) + in.ey * in.bx) / den;
     V2c = (sq(vx) + sq(vy)) + sq(vz);
     o.fix_v = 1;
+    o.rec[4] = V2c;
+    o.rec[5] = sq(vd);
     if (V2c >= 1.0) return 26;
   }
   o.vx = vx; o.vy = vy; o.vz = vz; o.rho = rho; o.p = p;
   return 0;
 }
 
-__device__ inline int c2p_cardano_full(double gamma, const C2PIn& in, C2POut& o, bool crm) {
-  Quartic qd = c2p_quartic(gamma, in);
-  if (qd.v2old > 1.0) return 20;
+__device__ inline int c2p_cardano_full(double gamma, double eps1, const C2PIn& in, C2POut& o, bool crm) {
+  Quartic qd = c2p_quartic(gamma, eps1, in);
   double Wn;
   int st = c2p_cardano(qd, Wn, crm);
   if (st) return st;

@@ -367,7 +367,7 @@ __device__ inline void lax_face(const Geom& g, const Planes& P, const Scheme& S,
 // of it.  YFIRST solves the y face first (the two solves are independent
 // except under HLLE, whose y solve starts from the x solve's last iterate).
 template <int ORD, int SOLVER, typename T, bool YFIRST = false, typename PutX, typename PutY>
-__device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const Scheme& S, DevStatus* st, int i,
+__device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const Scheme& S, const StatusRef& st, int i,
                                           int j, PutX put_x, PutY put_y) {
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
   const int64_t c = g.at(i, j);
@@ -501,7 +501,8 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
 }
 
 template <int ORD, int SOLVER, typename T = double>
-__global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Scheme S, DevStatus* st) {
+__global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Scheme S, StatusRef st) {
+  if (halted(st)) return;
 #if CUEVA_XCD_FACES
   int bx, by;
   if (!xcd_tile((g.imax + 2 + (int)blockDim.x - 1) / (int)blockDim.x,
@@ -566,7 +567,7 @@ struct MarchArgs {
   Geom g;
   Scheme S;
   int l, seg;
-  DevStatus* st;
+  StatusRef st;
 };
 static_assert(offsetof(MarchArgs, P) == 0, "argument block layout");
 
@@ -601,6 +602,7 @@ k_faces_march(MarchArgs args) {
   __shared__ T fyb[2][MW][FY_N][64];            // y-face rows of this and the previous tile step
   __shared__ T fyp[FY_N][64];                   // wavefront 0's row below: previous step's top row
   __shared__ T fxb[XLDS ? MW : 1][FX_N][64];    // x-face rows (HLLE)
+  if (halted(args.st)) return;  // uniform over the launch: before any barrier
   const int lane = threadIdx.x, w = threadIdx.y;
   const int i = (int)blockIdx.x * MARCH_COLS - 1 + lane;
   const int jc0 = (int)blockIdx.y * args.seg;
@@ -622,7 +624,7 @@ k_faces_march(MarchArgs args) {
     const Geom& g = A.g;
     const Scheme& S = A.S;
     const int l = A.l;
-    DevStatus* st = A.st;
+    const StatusRef& st = A.st;
     const int j = jt + w;
     const bool face_ok = i <= g.imax && j <= jend;
     T fx[FX_N];

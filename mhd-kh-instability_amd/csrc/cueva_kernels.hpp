@@ -24,7 +24,8 @@ struct PlaneList {
   int nzero;  // the last nzero planes get 0 at every ghost point instead of a copy
 };
 
-__global__ void k_fill_ghosts(Geom g, PlaneList pl) {
+__global__ void k_fill_ghosts(Geom g, PlaneList pl, StatusRef sr) {
+  if (halted(sr)) return;
   const int im = g.imax, jm = g.jmax;
   const int64_t nb = 6LL * (im + 13);  // one horizontal band
   const int64_t nv = 6LL * (jm + 1);   // one vertical band
@@ -154,19 +155,20 @@ __global__ void k_varfin(Geom g, Planes P, Scheme S, int which) {
 }
 
 // primitive recovery over the interior; reads D tau S and E B of the new step
-__device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, DevStatus* st, int i, int j,
+__device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, const StatusRef& sr, int i, int j,
                                  int64_t k) {
   C2POut o;
-  c2p_apply(P, S, st, i, j, k, c2p_load(P, k), o);
+  c2p_apply(P, S, sr, i, j, k, c2p_load(P, k), o);
 }
 
 // STAGE0: also the next step's stage-0 state (fused step)
 template <bool STAGE0 = false>
-__global__ void k_c2p(Geom g, Planes P, Scheme S, DevStatus* st) {
+__global__ void k_c2p(Geom g, Planes P, Scheme S, StatusRef sr) {
+  if (halted(sr)) return;
   INTERIOR_IJ
   const C2PIn in = c2p_load(P, k);
   C2POut o;
-  if (c2p_apply(P, S, st, i, j, k, in, o) && STAGE0) stage0_after_c2p(P, S, k, in, o);
+  if (c2p_apply(P, S, sr, i, j, k, in, o) && STAGE0) stage0_after_c2p(P, S, k, in, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -199,7 +201,8 @@ __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& 
 // differences u(i+1,j+1) - u and u - u(i-1,j-1).  The 14 results go to the
 // x-face planes (dead between steps); k_recons_copy writes them back, so
 // every read sees the old values.
-__global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P) {
+__global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P, StatusRef sr) {
+  if (halted(sr)) return;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i > g.imax || j > g.jmax) return;
   const int64_t c = g.at(i, j), dd = g.pitch + 1;
@@ -212,7 +215,8 @@ __global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P) {
     P.fx[k][c] = 0.5 * (r + l);
   }
 }
-__global__ void __launch_bounds__(256) k_recons_copy(Geom g, Planes P) {
+__global__ void __launch_bounds__(256) k_recons_copy(Geom g, Planes P, StatusRef sr) {
+  if (halted(sr)) return;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i > g.imax || j > g.jmax) return;
   const int64_t c = g.at(i, j);
@@ -222,7 +226,8 @@ __global__ void __launch_bounds__(256) k_recons_copy(Geom g, Planes P) {
 static_assert(P_N <= FX_N, "the reconstruction stages its results in the x-face planes");
 
 template <typename FT>
-__global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l) {
+__global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l, StatusRef sr) {
+  if (halted(sr)) return;
   INTERIOR_IJ
   Rhs o = flux_source<FT>(g, P, S, i, j, k);
 #pragma unroll
@@ -235,7 +240,8 @@ __global__ void k_fluxandsource(Geom g, Planes P, Scheme S, int l) {
 
 // Stage state at every point the stencils read: the point's ghost map gives
 // the interior point whose values the reference's ghost fills copy.
-__global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo) {
+__global__ void k_stage_fused(Geom g, Planes P, Scheme S, int l, int halo, StatusRef sr) {
+  if (halted(sr)) return;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - halo;
   const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - halo;
   if (i > g.imax + halo || j > g.jmax + halo) return;
@@ -260,12 +266,13 @@ __device__ inline void final_update(const Geom& g, const Planes& P, const Scheme
 // Last stage + sumfin + varfin + electricfield + varprimitivecardano
 // (C2P false: without the primitive recovery, which then runs on its own).
 template <typename FT, bool C2P = true>
-__global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
+__global__ void k_final_fused(Geom g, Planes P, Scheme S, StatusRef sr) {
+  if (halted(sr)) return;
   INTERIOR_IJ
   double nu[A_N];
   final_update<FT>(g, P, S, i, j, k, nu);
   store_update(P, k, nu);
-  if (C2P) c2p_store(g, P, S, st, i, j, k);
+  if (C2P) c2p_store(g, P, S, sr, i, j, k);
 }
 
 // The same with the learned primitive recovery.  Every lane of a wavefront
@@ -275,7 +282,8 @@ __global__ void k_final_fused(Geom g, Planes P, Scheme S, DevStatus* st) {
 // it does not occupy registers across the network.
 template <typename FT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_final_learned(Geom g, Planes P, Scheme S, const float* __restrict__ wp,
-                                                       DevStatus* st) {
+                                                       StatusRef sr) {
+  if (halted(sr)) return;
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const int j0 = blockIdx.y * blockDim.y + threadIdx.y;
   const bool valid = i0 <= g.imax && j0 <= g.jmax;
@@ -291,7 +299,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     in.bx = nu[A_BX]; in.by = nu[A_BY]; in.bz = nu[A_BZ];
     in.vx0 = in.vy0 = in.vz0 = 0.0;  // not used by the features
     in.rho_old = in.p_old = 0.0;
-    c2p_features(S.gamma, in, wp, f0, f1, f2);
+    c2p_features(S.gamma, S.eps1, in, wp, f0, f1, f2);
     if (valid) store_update(P, k, nu);
   }
   const int lane = threadIdx.x & 63;
@@ -310,12 +318,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
   in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
   C2POut out;
   bool fallback = false;
-  const int code = c2p_learned_finish(S.gamma, in, wp, y0, y1, S.crmath != 0, out, fallback);
-  if (fallback) atomicAdd(&st->fallback, 1ULL);
-  if (code) { report(st, code, i, j); return; }
-  if (out.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
-  if (out.fix_p) atomicAdd(&st->cnt[1], 1ULL);
-  if (out.fix_v) atomicAdd(&st->cnt[2], 1ULL);
+  const int code = c2p_learned_finish(S.gamma, S.eps1, in, wp, y0, y1, S.crmath != 0, out, fallback);
+  if (fallback) atomicAdd(&sr.st->fallback, 1ULL);
+  if (code) { report(sr, code, i, j); return; }
+  c2p_count(sr, i, j, out);
   P.p[P_VX][k] = out.vx; P.p[P_VY][k] = out.vy; P.p[P_VZ][k] = out.vz;
   P.p[P_RHO][k] = out.rho; P.p[P_P][k] = out.p;
   P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(out.vx) + sq(out.vy)) + sq(out.vz)));

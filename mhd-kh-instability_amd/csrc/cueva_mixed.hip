@@ -12,7 +12,7 @@ namespace cueva {
 
 template <int ORD>
 static hipError_t launch_ord(dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P, const Scheme& S,
-                             DevStatus* st) {
+                             const StatusRef& st) {
   if (S.flux_solver == 1)
     hipLaunchKernelGGL((k_faces<ORD, 1, float>), grid, blk, 0, s, g, P, S, st);
   else if (S.flux_solver == 4)
@@ -32,7 +32,7 @@ static hipError_t launch_ord(dim3 grid, dim3 blk, hipStream_t s, const Geom& g, 
 
 // one launch of the FP32 face kernel over the face pairs [-1,imax] x [-1,jmax]
 hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
-                             const Scheme& S, DevStatus* st) {
+                             const Scheme& S, const StatusRef& st) {
   if (S.flux_solver >= 1 && S.flux_solver <= 4)
     return launch_ord<5>(grid, blk, s, g, P, S, st);  // the MP order does not enter
   switch (rec_order) {

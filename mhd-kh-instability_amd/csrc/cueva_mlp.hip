@@ -183,8 +183,9 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* __restrict__ p
 // ---------------------------------------------------------------------------
 template <bool STAGE0>
 __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S, const float* __restrict__ wp,
-                                                     DevStatus* st) {
+                                                     StatusRef sr) {
   using namespace mlp;
+  if (halted(sr)) return;
   const int lane = threadIdx.x & 63;
   const int ni = g.imax + 1;
   const int64_t ncell = (int64_t)ni * (g.jmax + 1);
@@ -202,13 +203,11 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
   Operands o = load_operands(wp, lane);
   C2POut out;
   bool fallback = false;
-  const int code = c2p_learned_cell(S.gamma, in, wp, o, lane, S.crmath != 0, out, fallback);
+  const int code = c2p_learned_cell(S.gamma, S.eps1, in, wp, o, lane, S.crmath != 0, out, fallback);
   if (!valid) return;
-  if (fallback) atomicAdd(&st->fallback, 1ULL);
-  if (code) { report(st, code, i, j); return; }
-  if (out.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
-  if (out.fix_p) atomicAdd(&st->cnt[1], 1ULL);
-  if (out.fix_v) atomicAdd(&st->cnt[2], 1ULL);
+  if (fallback) atomicAdd(&sr.st->fallback, 1ULL);
+  if (code) { report(sr, code, i, j); return; }
+  c2p_count(sr, i, j, out);
   P.p[P_VX][k] = out.vx; P.p[P_VY][k] = out.vy; P.p[P_VZ][k] = out.vz;
   P.p[P_RHO][k] = out.rho; P.p[P_P][k] = out.p;
   P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(out.vx) + sq(out.vy)) + sq(out.vz)));
@@ -216,13 +215,13 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
 }
 
 hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
-                              DevStatus* st, bool stage0) {
+                              const StatusRef& sr, bool stage0) {
   const int64_t ncell = (int64_t)(g.imax + 1) * (g.jmax + 1);
   const dim3 grid((unsigned)((ncell + 255) / 256));
   if (stage0)
-    hipLaunchKernelGGL(k_c2p_learned<true>, grid, dim3(256), 0, s, g, P, S, wp, st);
+    hipLaunchKernelGGL(k_c2p_learned<true>, grid, dim3(256), 0, s, g, P, S, wp, sr);
   else
-    hipLaunchKernelGGL(k_c2p_learned<false>, grid, dim3(256), 0, s, g, P, S, wp, st);
+    hipLaunchKernelGGL(k_c2p_learned<false>, grid, dim3(256), 0, s, g, P, S, wp, sr);
   return hipGetLastError();
 }
 

@@ -178,10 +178,10 @@ __device__ inline void forward_wave(const Operands& o, int lane, float x0, float
 // Normalised network features of a conserved state: (log(D/C2),
 // |S - E x B|/C2, log C2), C2 = tau - (E^2+B^2)/2, formed as the quartic of
 // 12_varprimitive.f95:245-296 forms C1 and C2.
-__device__ inline void c2p_features(double gamma, const C2PIn& in, const float* __restrict__ wp, float& f0,
+__device__ inline void c2p_features(double gamma, double eps1, const C2PIn& in, const float* __restrict__ wp, float& f0,
                                     float& f1, float& f2) {
   using namespace mlp;
-  const Quartic qd = c2p_quartic(gamma, in);
+  const Quartic qd = c2p_quartic(gamma, eps1, in);
   const double C2 = qd.c2, sf = sqrt(qd.c1);
   // the features only need single precision (the network reads them in BF16)
   f0 = (__logf((float)(in.d / C2)) - wp[OFF_IN_MEAN + 0]) / wp[OFF_IN_STD + 0];
@@ -197,12 +197,11 @@ __device__ inline void c2p_features(double gamma, const C2PIn& in, const float* 
 // reference's Cardano root (fallback = true).  Then the reference's pressure
 // loop, fix-ups and velocities (12_varprimitive.f95:510-696).  Returns 0 or a
 // status.
-__device__ inline int c2p_learned_finish(double gamma, const C2PIn& in, const float* __restrict__ wp, float y0,
-                                         float y1, bool crm, C2POut& out, bool& fallback) {
+__device__ inline int c2p_learned_finish(double gamma, double eps1, const C2PIn& in, const float* __restrict__ wp,
+                                         float y0, float y1, bool crm, C2POut& out, bool& fallback) {
   using namespace mlp;
-  const Quartic qd = c2p_quartic(gamma, in);
+  const Quartic qd = c2p_quartic(gamma, eps1, in);
   fallback = false;
-  if (qd.v2old > 1.0) return 20;  // as the Cardano path (12_varprimitive.f95:229-238)
   const double C2 = qd.c2, sf = sqrt(qd.c1);
   // Start: W from the network's pressure through |S - E x B| = (C2 + p) v,
   // W = 1/sqrt(1 - v^2); its error is v^2 W^2 p/(C2+p) times that of p,
@@ -232,12 +231,12 @@ __device__ inline int c2p_learned_finish(double gamma, const C2PIn& in, const fl
 
 // Both together (every lane of the wavefront must call: the network runs
 // wavefront-wide).
-__device__ inline int c2p_learned_cell(double gamma, const C2PIn& in, const float* __restrict__ wp,
+__device__ inline int c2p_learned_cell(double gamma, double eps1, const C2PIn& in, const float* __restrict__ wp,
                                        const mlp::Operands& o, int lane, bool crm, C2POut& out, bool& fallback) {
   float f0, f1, f2, y0 = 0.0f, y1 = 0.0f;
-  c2p_features(gamma, in, wp, f0, f1, f2);
+  c2p_features(gamma, eps1, in, wp, f0, f1, f2);
   mlp::forward_wave(o, lane, f0, f1, f2, y0, y1);
-  return c2p_learned_finish(gamma, in, wp, y0, y1, crm, out, fallback);
+  return c2p_learned_finish(gamma, eps1, in, wp, y0, y1, crm, out, fallback);
 }
 
 }  // namespace cueva

@@ -139,13 +139,17 @@ __device__ inline void store_stage(double* const* s, int64_t k, const double* v)
 
 // primitive recovery of one point (Cardano, 12_varprimitive.f95) and the
 // store of its primitives and W; false on a failure (reported)
-__device__ inline bool c2p_apply(Planes& P, const Scheme& S, DevStatus* st, int i, int j, int64_t k,
+__device__ inline void c2p_count(const StatusRef& sr, int i, int j, const C2POut& o) {
+  if (o.fix_rho) add_fixup(sr, 0, i, j, o.rec[0], o.rec[1]);
+  if (o.fix_p) add_fixup(sr, 1, i, j, o.rec[2], o.rec[3]);
+  if (o.fix_v) add_fixup(sr, 2, i, j, o.rec[4], o.rec[5]);
+}
+
+__device__ inline bool c2p_apply(Planes& P, const Scheme& S, const StatusRef& sr, int i, int j, int64_t k,
                                  const C2PIn& in, C2POut& o) {
-  int code = c2p_cardano_full(S.gamma, in, o, S.crmath != 0);
-  if (code) { report(st, code, i, j); return false; }
-  if (o.fix_rho) atomicAdd(&st->cnt[0], 1ULL);
-  if (o.fix_p) atomicAdd(&st->cnt[1], 1ULL);
-  if (o.fix_v) atomicAdd(&st->cnt[2], 1ULL);
+  int code = c2p_cardano_full(S.gamma, S.eps1, in, o, S.crmath != 0);
+  if (code) { report(sr, code, i, j); return false; }
+  c2p_count(sr, i, j, o);
   P.p[P_VX][k] = o.vx; P.p[P_VY][k] = o.vy; P.p[P_VZ][k] = o.vz;
   P.p[P_RHO][k] = o.rho; P.p[P_P][k] = o.p;
   P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));

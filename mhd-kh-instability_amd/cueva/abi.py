@@ -36,7 +36,7 @@ CONS_NAMES = ("D", "tau", "Sx", "Sy", "Sz")
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory",
     7: "get_hllc_vel: no convergence", 8: "HLLC contact speed is NaN", 9: "HLLE wave speed is NaN",
-    20: "superluminal velocity entering Cardano", 21: "Cardano: no admissible resolvent root",
+    20: "(not returned)", 21: "Cardano: no admissible resolvent root",
     22: "Cardano: det < 0", 23: "Cardano: both roots below 1", 24: "Cardano: no root selected",
     25: "superluminal velocity, magnetisation below 10", 26: "drift velocity still superluminal",
 }
@@ -93,6 +93,9 @@ SIGNATURES = {
     "cueva_c2p": (_i, [_c, _i, _P(ctypes.c_int32), _P(ctypes.c_int32)]),
     "cueva_step": (_i, [_c, _i]),
     "cueva_step_counters": (_i, [_c, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    "cueva_last_error": (_i, [_c, _P(ctypes.c_int32)]),
+    "cueva_set_step_index": (_i, [_c, ctypes.c_int64]),
+    "cueva_fixup_records": (_i, [_c, _i, _P(ctypes.c_int32), _P(_d), ctypes.c_int64, _P(ctypes.c_int64)]),
     "cueva_reduce_globals": (_i, [_c, _P(_d)]),
     "cueva_max_speed": (_i, [_c, _P(_d)]),
     "cueva_kh_init": (_i, [_i, _i, _d, _d, _i, _d, _d, _d, ctypes.c_uint64, _P(_d)]),
@@ -354,6 +357,38 @@ class Solver:
         fb = ctypes.c_int64()
         check(self.lib.cueva_step_counters(self.ctx, c, ctypes.byref(fb)), "step_counters")
         return list(c), fb.value
+
+    def c2p(self):
+        """varprimitivecardano alone (l_swap = 1): (status, counters, fallback)
+        -- the status is returned, not raised, so that stops can be tested."""
+        cnt = (ctypes.c_int32 * 3)()
+        fb = ctypes.c_int32()
+        rc = self.lib.cueva_c2p(self.ctx, 1, cnt, ctypes.byref(fb))
+        return rc, list(cnt), fb.value
+
+    def step_status(self, n=1):
+        """Like step() but returns the status instead of raising."""
+        return int(self.lib.cueva_step(self.ctx, int(n)))
+
+    def last_error(self):
+        """(status, (h, l, i, j)) of the first failure (0 and zeros if none)."""
+        w = (ctypes.c_int32 * 4)()
+        rc = self.lib.cueva_last_error(self.ctx, w)
+        return rc, tuple(w)
+
+    def set_step_index(self, h):
+        check(self.lib.cueva_set_step_index(self.ctx, int(h)), "set_step_index")
+
+    def fixup_records(self, kind):
+        """Fix-up records of one kind (0 rho, 1 p, 2 drift) in the reference's
+        order: (hij int32 [n, 3], values [n, 2])."""
+        n = ctypes.c_int64()
+        check(self.lib.cueva_fixup_records(self.ctx, int(kind), None, None, 0, ctypes.byref(n)), "fixup_records")
+        hij = np.zeros((max(n.value, 1), 3), dtype=np.int32)
+        vals = np.zeros((max(n.value, 1), 2))
+        check(self.lib.cueva_fixup_records(self.ctx, int(kind), hij.ctypes.data_as(_P(ctypes.c_int32)), _dp(vals),
+                                           n.value, ctypes.byref(n)), "fixup_records")
+        return hij[:n.value], vals[:n.value]
 
     # -- per-subroutine entry points, in the reference's call order
     def step_by_routines(self):

@@ -14,6 +14,7 @@ module cueva_gpu
                                CUEVA_HLL = 5, CUEVA_HLLE = 6, CUEVA_HLLC = 7
   integer(c_int), parameter :: CUEVA_BND_PRIM = 0, CUEVA_BND_CONS = 1, CUEVA_BND_ELEC = 2, CUEVA_BND_FLUX = 3
   integer(c_int), parameter :: CUEVA_N_GLOBALS = 37
+  integer(c_int), parameter :: CUEVA_FIX_RHO = 0, CUEVA_FIX_P = 1, CUEVA_FIX_DRIFT = 2
 
   ! mirror of struct cueva_params
   type, bind(C) :: cueva_params
@@ -130,6 +131,25 @@ module cueva_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: l_swap
       integer(c_int32_t), intent(out) :: counters(3), fallback
+    end function
+    integer(c_int) function cueva_last_error(ctx, where) bind(C, name="cueva_last_error")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: where(4)
+    end function
+    integer(c_int) function cueva_set_step_index(ctx, h) bind(C, name="cueva_set_step_index")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: h
+    end function
+    integer(c_int) function cueva_fixup_records(ctx, kind, hij, vals, cap, n) bind(C, name="cueva_fixup_records")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      integer(c_int32_t), intent(out) :: hij(*)
+      real(c_double), intent(out) :: vals(*)
+      integer(c_int64_t), value :: cap
+      integer(c_int64_t), intent(out) :: n
     end function
     integer(c_int) function cueva_step(ctx, nsteps) bind(C, name="cueva_step")
       import :: c_int, c_ptr

@@ -7,16 +7,58 @@ module cueva_shims
   use cueva_gpu
   use parameters_kh
   implicit none
+  integer(c_int32_t) :: hij_dummy(3)
+  real(c_double) :: vals_dummy(2)
 contains
 
   subroutine check(rc, what)
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: what
+    integer(c_int32_t) :: where(4)
+    integer(c_int) :: code
     if (rc /= CUEVA_OK) then
-      write(*,*) "STOP: ", what, " returned status ", rc, " at step ", h, " stage ", l
+      if (rc >= 7) then
+        ! a stop of the reference: where it arose (step, stage, i, j)
+        code = cueva_last_error(ctx, where)
+        write(*,*) "STOP: ", what, " returned status ", rc, " (h,l,i,j) ---> ", where
+      else
+        write(*,*) "STOP: ", what, " returned status ", rc, " at step ", h, " stage ", l
+      end if
       stop 1
     end if
   end subroutine check
+
+  ! The fix-up records of this step's recovery in the reference's files
+  ! (units 301-303, opened by the driver as 01_openfiles.f95:115-117 does),
+  ! with the headers it writes at h = 1 and the record of
+  ! 12_varprimitive.f95:544,601,663: h, h*Delt, running count, i, j, l, x(i),
+  ! y(j) and the two values.
+  subroutine write_fixups
+    integer :: kind, u, e
+    integer(c_int64_t) :: n
+    integer(c_int32_t), allocatable :: hij(:)
+    real(c_double), allocatable :: vals(:)
+    if (h == 1) then
+      write(301,*) "iteration    Delt     contador   i    j     RK-step     x(i)     y(j)   rho*       rho^n"
+      write(301,*) "========================================================================================"
+      write(302,*) "iteration    Delt     contador   i    j     RK-step     x(i)     y(j)   p*          p^n "
+      write(302,*) "========================================================================================"
+      write(303,*) "iteration    Delt     contador   i    j     RK-step     x(i)     y(j)   v*^2  v_drift^2 "
+      write(303,*) "========================================================================================"
+    end if
+    do kind = 0, 2
+      u = 301 + kind
+      call check(cueva_fixup_records(ctx, int(kind, c_int), hij_dummy, vals_dummy, 0_c_int64_t, n), "fixup_records")
+      if (n == 0) cycle
+      allocate(hij(3 * n), vals(2 * n))
+      call check(cueva_fixup_records(ctx, int(kind, c_int), hij, vals, n, n), "fixup_records")
+      do e = 1, int(n)
+        write(u,*) h, h * Delt, e, hij(3 * e - 1), hij(3 * e), l, 0.5d0 * prm%longx - hij(3 * e - 1) * Delx, &
+                   hij(3 * e) * Dely, vals(2 * e - 1), vals(2 * e)
+      end do
+      deallocate(hij, vals)
+    end do
+  end subroutine write_fixups
 
   subroutine boundary                 ! 04_boundary.f95
     call check(cueva_boundary(ctx, CUEVA_BND_PRIM, 1_c_int), "boundary")
@@ -114,6 +156,7 @@ contains
   subroutine varprimitivecardano      ! 12_varprimitive.f95:6 (counters of this call)
     integer(c_int32_t) :: cnt(3), fb
     call check(cueva_c2p(ctx, int(l_swap, c_int), cnt, fb), "varprimitivecardano")
+    call write_fixups
     contador_r = contador_r + cnt(1)
     contador_p = contador_p + cnt(2)
     contador_v = contador_v + cnt(3)

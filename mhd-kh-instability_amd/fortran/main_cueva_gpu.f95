@@ -11,6 +11,10 @@
 ! (imax+13)*(jmax+13) doubles, x fastest) and, every HGLB steps (and at the
 ! first), a line of global diagnostics to OUTPREFIX_globals.dat, and the
 ! reference's data_global/2d_data_*.dat files when that directory exists.
+! The primitive recovery's fix-up records go to the reference's files
+! fix-up_densidad.dat, fix-up_presion.dat and fix-up_velocity_drift.dat in
+! the working directory (01_openfiles.f95:115-117, 12_varprimitive.f95:33-40,
+! 544, 601, 663); a failure prints its status and (h, l, i, j) and stops.
 !
 ! HLOC > 0: every HLOC steps, the reference's restart file
 !   data_restart/all_variables_NNNN.dat, NNNN = h/HLOC (stops, as lector
@@ -101,6 +105,11 @@ program main_cueva_gpu
   end if
   call check(cueva_create(prm, ctx), "create")
   call check(cueva_upload_prims(ctx, prims), "upload")
+  call check(cueva_set_step_index(ctx, int(hini, c_int64_t)), "set_step_index")
+  ! the primitive recovery's fix-up files (01_openfiles.f95:115-117)
+  open(unit=301, file="fix-up_densidad.dat")
+  open(unit=302, file="fix-up_presion.dat")
+  open(unit=303, file="fix-up_velocity_drift.dat")
   gfiles = .false.
   if (hglb > 0) then
     open(unit=70, file=trim(prefix)//"_globals.dat", status="replace", action="write")
@@ -186,6 +195,9 @@ program main_cueva_gpu
   write(90) cons
   close(90)
   if (hglb > 0) close(70)
+  close(301)
+  close(302)
+  close(303)
   if (gfiles) call close_globals(37)
   call check(cueva_destroy(ctx), "destroy")
   write(*,'(a,i0,a,3i6)') "done ", hmax, " steps; fix-ups of the last step: ", contador_r, contador_p, contador_v

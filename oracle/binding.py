@@ -19,7 +19,8 @@ class Cfg(ctypes.Structure):
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("sigma", ctypes.c_double),
                 ("kappa_psi", ctypes.c_double), ("kappa_phi", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("wb", ctypes.c_double * 8), ("wbt", ctypes.c_double * 8), ("Ab", ctypes.c_double * 64),
-                ("Abt", ctypes.c_double * 64)]
+                ("Abt", ctypes.c_double * 64), ("primitive_rec", ctypes.c_int32), ("hllc_seed", ctypes.c_int32),
+                ("libm_mode", ctypes.c_int32), ("pad1", ctypes.c_int32)]
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -43,13 +44,18 @@ def load():
         lib.oracle_get_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
         lib.oracle_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.oracle_get_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P]
+        lib.oracle_get_face.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P]
         lib.oracle_mp_line.argtypes = [ctypes.c_int, _P, ctypes.c_int, _P, _P, ctypes.c_int]
         lib.oracle_hllc_vel.restype = ctypes.c_double
         lib.oracle_hllc_vel.argtypes = [ctypes.c_double] * 5
-        lib.oracle_c2p_point.argtypes = [ctypes.c_double, _P, _P, ctypes.POINTER(ctypes.c_int)]
-        lib.oracle_set_hllc_seed_mode.argtypes = [ctypes.c_int]
-        lib.oracle_set_primitive_rec.argtypes = [ctypes.c_int]
-        lib.oracle_set_libm_mode.argtypes = [ctypes.c_int]
+        lib.oracle_c2p_point.argtypes = [ctypes.c_double, ctypes.c_double, _P, _P, ctypes.POINTER(ctypes.c_int), _P]
+        lib.oracle_set_cons.argtypes = [ctypes.c_void_p, _P]
+        lib.oracle_recover.argtypes = [ctypes.c_void_p]
+        lib.oracle_last_stop.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        lib.oracle_face_events.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+        lib.oracle_get_face_events.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), _P, ctypes.c_int]
+        lib.oracle_get_fixups.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32), _P,
+                                          ctypes.c_int]
         lib.oracle_globals.argtypes = [ctypes.c_void_p, _P]
         _lib = lib
     return _lib
@@ -75,11 +81,11 @@ class Oracle:
                                     params.bc_y, params.sigma, params.cfl)
         if rc:
             raise ValueError("oracle config rejected (%d)" % rc)
-        lib.oracle_set_hllc_seed_mode(int(params.hllc_seed))
-        lib.oracle_set_primitive_rec(int(params.primitive_rec))
+        self.c.hllc_seed = int(params.hllc_seed)
+        self.c.primitive_rec = int(params.primitive_rec)
         if libm is None:
             libm = "cr" if params.crmath else "c"
-        lib.oracle_set_libm_mode(1 if libm == "cr" else 0)
+        self.c.libm_mode = 1 if libm == "cr" else 0
         self.s = lib.oracle_create(ctypes.byref(self.c))
         self.shape = (params.jmax + 13, params.imax + 13)
 
@@ -105,6 +111,59 @@ class Oracle:
         cons = np.zeros((5,) + self.shape)
         self.lib.oracle_get_fields(self.s, _dp(prims), _dp(cons))
         return prims, cons
+
+    def upload_cons(self, cons):
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        self.lib.oracle_set_cons(self.s, _dp(cons))
+
+    def recover(self):
+        """varprimitivecardano alone (l_swap = 1); returns 0 or the stop status."""
+        return int(self.lib.oracle_recover(self.s))
+
+    def last_stop(self):
+        """(step within the last step() call, stage, i, j) of the last nonzero status."""
+        out = (ctypes.c_int32 * 4)()
+        self.lib.oracle_last_stop(self.s, out)
+        return tuple(out)
+
+    def face_events(self):
+        """(superluminal face fallbacks, HLLC face-density fixes) since creation."""
+        out = (ctypes.c_int64 * 2)()
+        self.lib.oracle_face_events(self.s, out)
+        return tuple(out)
+
+    def rhs(self, l, which):
+        """Stage right-hand side `which` (0..16) of stage l (0-based) of the last step."""
+        out = np.zeros(self.shape)
+        self.lib.oracle_get_rhs(self.s, int(l), int(which), _dp(out))
+        return out
+
+    def face(self, direction, slot):
+        """Face flux `slot` of the last face pass (direction 0: x faces, 1: y faces)."""
+        out = np.zeros(self.shape)
+        self.lib.oracle_get_face(self.s, int(direction), int(slot), _dp(out))
+        return out
+
+    def face_event_list(self):
+        """(kind, l, i, j) int32 [n, 4] and values [n] of every face event since creation."""
+        n = self.lib.oracle_get_face_events(self.s, None, None, 0)
+        k = np.zeros((max(n, 1), 4), dtype=np.int32)
+        v = np.zeros(max(n, 1))
+        self.lib.oracle_get_face_events(self.s, k.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(v), n)
+        return k[:n], v[:n]
+
+    def step_status(self, n=1):
+        """Like step() but returns the status instead of raising."""
+        return int(self.lib.oracle_step(self.s, int(n)))
+
+    def fixups(self, kind):
+        """Fix-up records of the last recovery, kind 0 rho / 1 p / 2 drift:
+        (ij int32 [n, 2], values [n, 2]) in the reference's loop order."""
+        n = self.lib.oracle_get_fixups(self.s, kind, None, None, 0)
+        ij = np.zeros((max(n, 1), 2), dtype=np.int32)
+        vals = np.zeros((max(n, 1), 2))
+        self.lib.oracle_get_fixups(self.s, kind, ij.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(vals), n)
+        return ij[:n], vals[:n]
 
     def counters(self):
         c = (ctypes.c_int64 * 3)()

@@ -32,8 +32,9 @@
  * bit for bit on the machine the fixtures were made on).  Mode 1 returns the
  * correctly rounded result, computed in quad precision (libquadmath) and
  * rounded once to double. */
+/* The mode switches below are set from the running instance's cfg at the
+ * start of every oracle_step / recovery call (install_modes). */
 static int g_libm_mode = 0;
-void oracle_set_libm_mode(int mode) { g_libm_mode = mode; }
 static double o_acos(double x) { return g_libm_mode ? (double)acosq((__float128)x) : acos(x); }
 static double o_cos(double x) { return g_libm_mode ? (double)cosq((__float128)x) : cos(x); }
 static double o_pow(double x, double y) {
@@ -82,6 +83,18 @@ struct oracle_state {
   double *fx[FX_N], *fy[FY_N];
   double *rhs[MAXL][NRHS];
   int64_t cnt[3];
+  double eps1; /* the reference's module eps1: 1e-6 once hll_flow_mpx ran (16_hll_flow_mpx.f95:1375) */
+  /* fix-up records of the last recovery, per kind: (i, j) and two values */
+  int nfix[3], capfix[3];
+  int32_t stop[4]; /* step (1-based within oracle_step), stage (1-based), i, j of the last stop */
+  int64_t face_events[2]; /* superluminal face fallbacks, HLLC face-density fixes (since create) */
+  /* the same events as the reference prints them, in its order: kind, stage, i, j, and a value */
+  int nfe, capfe;
+  int32_t *fe_kind;
+  double *fe_val;
+  int cur_l;
+  int32_t *fix_ij[3];
+  double *fix_val[3];
 };
 
 #define IX(s, i, j) (((size_t)((j) + G)) * (size_t)(s)->nx + (size_t)((i) + G))
@@ -122,6 +135,9 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   c->dt = cfl * m;
   c->sigma = sigma;
   c->gamma = 4.0 / 3.0; /* parameters.f95:148 */
+  c->primitive_rec = 0;
+  c->hllc_seed = 1;
+  c->libm_mode = 0;
   /* parameters.f95:325-334: kappa = alpha_ch**2 / cp**2, cp = sqrt(min(Delx,Dely)/alpha) */
   double cp = sqrt(fmin2(c->dx, c->dy) / 1.0);
   c->kappa_psi = 1.0 / (cp * cp);
@@ -183,6 +199,9 @@ void oracle_destroy(oracle_state *s) {
   for (int k = 0; k < FY_N; k++) free(s->fy[k]);
   for (int l = 0; l < s->c.lmax; l++)
     for (int k = 0; k < NRHS; k++) free(s->rhs[l][k]);
+  for (int k = 0; k < 3; k++) { free(s->fix_ij[k]); free(s->fix_val[k]); }
+  free(s->fe_kind);
+  free(s->fe_val);
   free(s);
 }
 
@@ -226,6 +245,11 @@ void oracle_get_fields(const oracle_state *s, double *prims14, double *cons5) {
     const double *c5[5] = {s->D, s->tau, s->Sx, s->Sy, s->Sz};
     for (int k = 0; k < 5; k++) memcpy(cons5 + (size_t)k * s->n, c5[k], s->n * sizeof(double));
   }
+}
+
+void oracle_set_cons(oracle_state *s, const double *cons5) {
+  double *c5[5] = {s->D, s->tau, s->Sx, s->Sy, s->Sz};
+  for (int k = 0; k < 5; k++) memcpy(c5[k], cons5 + (size_t)k * s->n, s->n * sizeof(double));
 }
 
 void oracle_get_counters(const oracle_state *s, int64_t counters3[3]) {
@@ -750,7 +774,6 @@ static double hllc_contact(double a, double b, double c, double v0) {
  * build: the register still holds Bzint_mp5_x_R(i,j).  For the y face it
  * holds the negated b coefficient, i.e. the b argument itself. */
 static int g_hllc_seed_mode = 1; /* 1 reference build, 0 clamped seed */
-void oracle_set_hllc_seed_mode(int mode) { g_hllc_seed_mode = mode; }
 
 static int hllc_x(const side_t *L, const side_t *R, double f[FX_N]) {
   double psi_h = hll_state(L->psi, R->psi, L->ex, R->ex);
@@ -1181,8 +1204,49 @@ static void lax_face(const oracle_state *s, int i, int j, int dir, double *f);
 static int hllc_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy);
 static int hlle_mcl_faces(const oracle_state *s, int i, int j, double *fx, double *fy);
 
+/* one printed face event (16_hll_flow_mpx.f95:1419,1453 "FIXED velocity",
+ * 16_hllc_flow_mpx.f95:2665-2691 "Negative density"): kind 0 fallback
+ * from the x faces, 1 from the y faces only, 2..5 density fix of y_L, y_R,
+ * x_L, x_R; value: the negative density (0 for a fallback) */
+static void face_event(oracle_state *s, int kind, int i, int j, double value) {
+  if (s->nfe == s->capfe) {
+    int cap = s->capfe ? 2 * s->capfe : 256;
+    int32_t *k = realloc(s->fe_kind, sizeof(int32_t) * 4 * (size_t)cap);
+    if (!k) return;
+    s->fe_kind = k;
+    double *v = realloc(s->fe_val, sizeof(double) * (size_t)cap);
+    if (!v) return;
+    s->fe_val = v;
+    s->capfe = cap;
+  }
+  int n = s->nfe++;
+  s->fe_kind[4 * n] = kind;
+  s->fe_kind[4 * n + 1] = s->cur_l;
+  s->fe_kind[4 * n + 2] = i;
+  s->fe_kind[4 * n + 3] = j;
+  s->fe_val[n] = value;
+}
+
+int oracle_get_face_events(const oracle_state *s, int32_t *kind_l_i_j, double *vals, int cap) {
+  for (int e = 0; e < s->nfe && e < cap; e++) {
+    for (int q = 0; q < 4; q++) kind_l_i_j[4 * e + q] = s->fe_kind[4 * e + q];
+    vals[e] = s->fe_val[e];
+  }
+  return s->nfe;
+}
+
+/* where the reference stops: the face (i, j) of its loop */
+static int face_stop(oracle_state *s, int i, int j, int st) {
+  s->stop[2] = i;
+  s->stop[3] = j;
+  return st;
+}
+
 static int face_loop(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, fs = s->c.flux_solver;
+  /* hll_flow_mpx sets the shared module variable eps1 (16_hll_flow_mpx.f95:1375,
+   * not in its PRIVATE list); the recovery's superluminal clamp reads it */
+  if (fs == OR_HLL) s->eps1 = 1.0e-6;
   double **P = s->prim;
   for (int j = -1; j <= jm; j++)
     for (int i = -1; i <= im; i++) {
@@ -1191,7 +1255,7 @@ static int face_loop(oracle_state *s) {
         double fx[FX_N], fy[FY_N];
         if (fs == OR_HLLC_MCL || fs == OR_HLLE_MCL) {
           int st = fs == OR_HLLC_MCL ? hllc_mcl_faces(s, i, j, fx, fy) : hlle_mcl_faces(s, i, j, fx, fy);
-          if (st) return st;
+          if (st) return face_stop(s, i, j, st);
         } else if (fs == OR_LAX) {
           lax_face(s, i, j, 0, fx);
           lax_face(s, i, j, 1, fy);
@@ -1217,6 +1281,8 @@ static int face_loop(oracle_state *s) {
       double v2yl = (sq(yl.vx) + sq(yl.vy)) + sq(yl.vz);
       double v2yr = (sq(yr.vx) + sq(yr.vy)) + sq(yr.vz);
       if (fs != OR_HLLE && (v2xl >= 1.0 || v2xr >= 1.0 || v2yl >= 1.0 || v2yr >= 1.0)) {
+        s->face_events[0]++;
+        face_event(s, (v2xl >= 1.0 || v2xr >= 1.0) ? 0 : 1, i, j, 0.0);
         size_t kc = IX(s, i, j), km = IX(s, i - 1, j), kj = IX(s, i, j - 1);
         xl.vx = P[OR_VX][kc]; xl.vy = P[OR_VY][kc]; xl.vz = P[OR_VZ][kc];
         xr.vx = P[OR_VX][km]; xr.vy = P[OR_VY][km]; xr.vz = P[OR_VZ][km];
@@ -1225,10 +1291,11 @@ static int face_loop(oracle_state *s) {
       }
       if (fs == OR_HLLC) { /* 16_hllc_flow_mpx.f95:2663-2693 */
         double rc = 0.1 * P[OR_RHO][k];
-        if (yl.rho < 0.0) yl.rho = rc;
-        else if (yr.rho < 0.0) yr.rho = rc;
-        else if (xl.rho < 0.0) xl.rho = rc;
-        else if (xr.rho < 0.0) xr.rho = rc;
+        if (yl.rho < 0.0 || yr.rho < 0.0 || xl.rho < 0.0 || xr.rho < 0.0) s->face_events[1]++;
+        if (yl.rho < 0.0) { face_event(s, 2, i, j, yl.rho); yl.rho = rc; }
+        else if (yr.rho < 0.0) { face_event(s, 3, i, j, yr.rho); yr.rho = rc; }
+        else if (xl.rho < 0.0) { face_event(s, 4, i, j, xl.rho); xl.rho = rc; }
+        else if (xr.rho < 0.0) { face_event(s, 5, i, j, xr.rho); xr.rho = rc; }
       }
       int he = fs == OR_HLLE;
       side_physics(s->c.gamma, 0, he, &xl);
@@ -1241,12 +1308,12 @@ static int face_loop(oracle_state *s) {
         hll_y(&yl, &yr, fy);
       } else if (fs == OR_HLLC) {
         int st = hllc_x(&xl, &xr, fx);
-        if (st) return st;
+        if (st) return face_stop(s, i, j, st);
         st = hllc_y(&yl, &yr, fy);
-        if (st) return st;
+        if (st) return face_stop(s, i, j, st);
       } else {
         int st = hlle_xy(s->c.gamma, &xl, &xr, &yl, &yr, fx, fy);
-        if (st) return st;
+        if (st) return face_stop(s, i, j, st);
       }
       for (int f = 0; f < FX_N; f++) s->fx[f][k] = fx[f];
       for (int f = 0; f < FY_N; f++) s->fy[f][k] = fy[f];
@@ -1279,12 +1346,10 @@ typedef struct {
 /* from the same differences, and every interior point becomes          */
 /* 0.5 (R + L): u again up to rounding.  Ghosts are left to the caller. */
 /* ------------------------------------------------------------------ */
-static int g_primitive_rec = 0;
-void oracle_set_primitive_rec(int on) { g_primitive_rec = on != 0; }
-
-static void recons_prims(oracle_state *s) {
+static int recons_prims(oracle_state *s) {
   const int64_t dd = IX(s, 1, 1) - IX(s, 0, 0);
   double *tmp = malloc(s->n * sizeof(double));
+  if (!tmp) return 90;
   for (int k = 0; k < OR_NPRIM; k++) {
     double *a = s->prim[k];
     for (int j = 0; j <= s->c.jmax; j++)
@@ -1299,6 +1364,7 @@ static void recons_prims(oracle_state *s) {
       for (int i = 0; i <= s->c.imax; i++) a[IX(s, i, j)] = tmp[IX(s, i, j)];
   }
   free(tmp);
+  return 0;
 }
 
 /* u_L = u(i) + FAC mcl(u(i+1)-u(i), u(i)-u(i-1)), u_R = u(i+1) - FAC
@@ -1846,7 +1912,7 @@ static void varfin(oracle_state *s) {
  * the old rho and p through out[3], out[4] on entry.
  * out[5] = {Vx, Vy, Vz, rho, p}; flags = fix-ups {rho, p, drift}.
  * Returns 0 or a stop status. */
-int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags[3]) {
+int oracle_c2p_point(double gamma, double eps1, const double in[14], double out[5], int flags[3], double rec[6]) {
   const double TOL = 1.0e-10;
   const int nrmax = 40;
   double D = in[0], tau = in[1], Sx = in[2], Sy = in[3], Sz = in[4];
@@ -1858,7 +1924,10 @@ int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags
   double E2B2 = 0.5 * (((((sq(Ex) + sq(Ey)) + sq(Ez)) + sq(Bx)) + sq(By)) + sq(Bz));
   double g1 = (gamma - 1.0) / gamma;
   double V2c = (sq(vx0) + sq(vy0)) + sq(vz0);
-  if (V2c > 1.0) return 20; /* superluminal old velocity: the reference clamps with an undefined eps1 */
+  /* :236-241: a superluminal previous velocity is clamped (it enters only the
+   * pressure loop below; W is taken from the quartic) */
+  if (V2c > 1.0) V2c = fmax2(-1.0 + eps1, fmin2(1.0 - eps1, V2c));
+  for (int k = 0; k < 6; k++) rec[k] = 0.0;
 
   double C1 = (sq((Sx - Ey * Bz) + Ez * By) + sq((Sy - Ez * Bx) + Ex * Bz)) + sq((Sz - Ex * By) + Ey * Bx);
   double C2 = tau - E2B2;
@@ -1924,7 +1993,7 @@ int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags
   }
   double W = Wn;
   double rho = D / W;
-  if (rho < 0.0) { flags[0] = 1; rho = fmax2(rho_old * 1.0e-1, rho); }
+  if (rho < 0.0) { flags[0] = 1; rec[0] = rho; rec[1] = rho_old; rho = fmax2(rho_old * 1.0e-1, rho); }
   double h = ((tau - E2B2) - g1 * rho) / (sq(W) - g1);
   double p = g1 * (h - rho);
 
@@ -1940,7 +2009,7 @@ int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags
     pn = pr;
     nr = nr + 1;
   }
-  if (pn < 0.0) { flags[1] = 1; pn = fmax2(p_old * 1.0e-4, pn); }
+  if (pn < 0.0) { flags[1] = 1; rec[2] = pn; rec[3] = p_old; pn = fmax2(p_old * 1.0e-4, pn); }
   p = pn;
   tau = (E2B2 + h * sq(W)) - p;
   double den = (tau - E2B2) + p;
@@ -1965,14 +2034,48 @@ int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags
     vz = ((Sz - Ex * By) + Ey * Bx) / den;
     V2c = (sq(vx) + sq(vy)) + sq(vz);
     flags[2] = 1;
+    rec[4] = V2c;
+    rec[5] = sq(vd);
     if (V2c >= 1.0) return 26;
   }
   out[0] = vx; out[1] = vy; out[2] = vz; out[3] = rho; out[4] = p;
   return 0;
 }
 
+static int add_fixup(oracle_state *s, int kind, int i, int j, double a, double b) {
+  if (s->nfix[kind] == s->capfix[kind]) {
+    int cap = s->capfix[kind] ? 2 * s->capfix[kind] : 64;
+    int32_t *ij = realloc(s->fix_ij[kind], sizeof(int32_t) * 2 * (size_t)cap);
+    if (!ij) return 90;
+    s->fix_ij[kind] = ij;
+    double *v = realloc(s->fix_val[kind], sizeof(double) * 2 * (size_t)cap);
+    if (!v) return 90;
+    s->fix_val[kind] = v;
+    s->capfix[kind] = cap;
+  }
+  int n = s->nfix[kind]++;
+  s->fix_ij[kind][2 * n] = i;
+  s->fix_ij[kind][2 * n + 1] = j;
+  s->fix_val[kind][2 * n] = a;
+  s->fix_val[kind][2 * n + 1] = b;
+  return 0;
+}
+
+int oracle_get_fixups(const oracle_state *s, int kind, int32_t *ij, double *vals, int cap) {
+  if (kind < 0 || kind > 2) return -1;
+  int n = s->nfix[kind];
+  for (int e = 0; e < n && e < cap; e++) {
+    ij[2 * e] = s->fix_ij[kind][2 * e];
+    ij[2 * e + 1] = s->fix_ij[kind][2 * e + 1];
+    vals[2 * e] = s->fix_val[kind][2 * e];
+    vals[2 * e + 1] = s->fix_val[kind][2 * e + 1];
+  }
+  return n;
+}
+
 static int c2p(oracle_state *s) {
   double **P = s->prim;
+  s->nfix[0] = s->nfix[1] = s->nfix[2] = 0;
   for (int j = 0; j <= s->c.jmax; j++)
     for (int i = 0; i <= s->c.imax; i++) {
       size_t k = IX(s, i, j);
@@ -1981,9 +2084,16 @@ static int c2p(oracle_state *s) {
                        P[OR_VX][k], P[OR_VY][k], P[OR_VZ][k]};
       double out[5] = {0, 0, 0, P[OR_RHO][k], P[OR_P][k]};
       int fl[3];
-      int st = oracle_c2p_point(s->c.gamma, in, out, fl);
-      if (st) return st;
+      double rec[6];
+      int st = oracle_c2p_point(s->c.gamma, s->eps1, in, out, fl, rec);
+      if (st) {
+        s->stop[2] = i;
+        s->stop[3] = j;
+        return st;
+      }
       s->cnt[0] += fl[0]; s->cnt[1] += fl[1]; s->cnt[2] += fl[2];
+      for (int kind = 0; kind < 3; kind++)
+        if (fl[kind] && add_fixup(s, kind, i, j, rec[2 * kind], rec[2 * kind + 1])) return 90;
       P[OR_VX][k] = out[0]; P[OR_VY][k] = out[1]; P[OR_VZ][k] = out[2];
       P[OR_RHO][k] = out[3]; P[OR_P][k] = out[4];
     }
@@ -1994,11 +2104,36 @@ static int c2p(oracle_state *s) {
 /* one time step: main_cueva_code.f95:47-332                            */
 /* ------------------------------------------------------------------ */
 
+static void install_modes(const oracle_state *s);
+
+int oracle_recover(oracle_state *s) {
+  install_modes(s);
+  s->cnt[0] = s->cnt[1] = s->cnt[2] = 0;
+  s->stop[0] = 1;
+  s->stop[1] = s->c.lmax;
+  return c2p(s);
+}
+
+void oracle_face_events(const oracle_state *s, int64_t out[2]) {
+  out[0] = s->face_events[0];
+  out[1] = s->face_events[1];
+}
+
+void oracle_last_stop(const oracle_state *s, int32_t out[4]) {
+  for (int k = 0; k < 4; k++) out[k] = s->stop[k];
+}
+
+static void install_modes(const oracle_state *s) {
+  g_libm_mode = s->c.libm_mode;
+  g_hllc_seed_mode = s->c.hllc_seed;
+}
+
 int oracle_step(oracle_state *s, int nsteps) {
+  install_modes(s);
   for (int h = 0; h < nsteps; h++) {
     for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
-    if (g_primitive_rec) { /* main_cueva_code.f95:170-175 */
-      recons_prims(s);
+    if (s->c.primitive_rec) { /* main_cueva_code.f95:170-175 */
+      if (recons_prims(s)) return 90;
       for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
     }
     startconvar(s);
@@ -2010,14 +2145,23 @@ int oracle_step(oracle_state *s, int nsteps) {
       electricvarint(s, l);
       flowinit(s);
       if (s->c.flux_solver == OR_HLL || s->c.flux_solver == OR_HLLE || s->c.flux_solver == OR_HLLC) reconstruct(s);
+      s->cur_l = l + 1;
       int st = face_loop(s);
-      if (st) return st;
+      if (st) {
+        s->stop[0] = h + 1;
+        s->stop[1] = l + 1;
+        return st;
+      }
       fluxandsource(s, l);
       sumfin(s, l);
     }
     varfin(s);
     int st = c2p(s);
-    if (st) return st;
+    if (st) {
+      s->stop[0] = h + 1;
+      s->stop[1] = s->c.lmax;
+      return st;
+    }
     for (int k = 0; k < OR_NPRIM; k++) fill_ghosts(s, s->prim[k]);
   }
   return 0;

@@ -42,6 +42,11 @@ typedef struct {
   double dt, dx, dy;
   double sigma, kappa_psi, kappa_phi, gamma;
   double wb[8], wbt[8], Ab[8][8], Abt[8][8];
+  /* modes of this instance (oracle_step installs them for its own run) */
+  int32_t primitive_rec;   /* 1: recons_var_primitive before each step (parameters.f95:26) */
+  int32_t hllc_seed;       /* 1: first contact-speed Newton point as the reference build; 0: clamped seed */
+  int32_t libm_mode;       /* 0: acos/cos/pow of the C library (as the reference build); 1: correctly rounded */
+  int32_t pad1;
 } oracle_cfg;
 
 typedef struct oracle_state oracle_state;
@@ -61,6 +66,26 @@ void oracle_set_prims(oracle_state *s, const double *prims14);
 /* Copy out the 14 planes and the 5 evolved conserved planes D tau Sx Sy Sz. */
 void oracle_get_fields(const oracle_state *s, double *prims14, double *cons5);
 void oracle_get_counters(const oracle_state *s, int64_t counters3[3]);
+/* Replace the conserved D tau Sx Sy Sz (5 planes) the recovery reads. */
+void oracle_set_cons(oracle_state *s, const double *cons5);
+/* varprimitivecardano alone with l_swap = 1 (12_varprimitive.f95:6-824) on
+ * the state's primitives (previous velocity, rho, p; E, B) and conserved
+ * planes; counters and fix-up records restart from zero.  Returns 0 or the
+ * status of the reference's first stop in its loop order. */
+int oracle_recover(oracle_state *s);
+/* Where the last nonzero status arose: {step (1-based within the
+ * oracle_step call), stage l (1-based; lmax for the recovery), i, j}. */
+void oracle_last_stop(const oracle_state *s, int32_t out[4]);
+/* How often the MPx face solves took their fallbacks since create: {the
+ * superluminal face-velocity fallback (16_hll_flow_mpx.f95:1392-1461), the
+ * HLLC face-density fix (16_hllc_flow_mpx.f95:2663-2693)}. */
+void oracle_face_events(const oracle_state *s, int64_t out[2]);
+/* The same events one by one, in the order the reference prints them:
+ * {kind, stage l, i, j} and a value per event; kind 0 = fallback from an
+ * x face ("FIXED velocity ... V2_X"), 1 = from a y face only ("V2_Y"),
+ * 2..5 = density fix of y_L, y_R, x_L, x_R ("Negative density"; value =
+ * the negative density).  Returns the count; copies at most cap. */
+int oracle_get_face_events(const oracle_state *s, int32_t *kind_l_i_j, double *vals, int cap);
 
 /* Advance nsteps steps; returns 0, or a nonzero status naming the failure
  * the reference would stop on (see cueva_oracle.c). */
@@ -75,16 +100,27 @@ void oracle_get_face(const oracle_state *s, int dir, int slot, double *out);
  * order and summation order (i outer, j inner). */
 void oracle_globals(const oracle_state *s, double out[37]);
 
+/* Fix-up records of the last step's primitive recovery, as the reference
+ * writes them to units 301 (rho), 302 (p) and 303 (drift) in
+ * 12_varprimitive.f95:544,601,663: per record i, j and the two values it
+ * prints (rho* and rho^n; p* and p^n; v*^2 and v_drift^2), in the
+ * reference's loop order (j outer, i inner).  Returns the number of
+ * records; copies at most cap of them. */
+int oracle_get_fixups(const oracle_state *s, int kind, int32_t *ij, double *vals, int cap);
+
 /* Pointwise/linewise pieces, exported for unit tests. */
 void oracle_mp_line(int order, const double *u, int n, double *up, double *um, int positivity);
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0);
-/* 1: recons_var_primitive before each step (primitive_rec = 1) */
-void oracle_set_primitive_rec(int on);
-/* 1: first contact-speed Newton point as in the reference build; 0: clamped seed */
-void oracle_set_hllc_seed_mode(int mode);
-/* 0: acos/cos/pow from the C library (as the reference build); 1: correctly rounded */
-void oracle_set_libm_mode(int mode);
-int oracle_c2p_point(double gamma, const double in[14], double out[5], int flags[3]);
+/* One cell of varprimitivecardano with l_swap = 1 (12_varprimitive.f95:48-813).
+ * in = {D, tau, Sx, Sy, Sz, Ex, Ey, Ez, Bx, By, Bz, Vx, Vy, Vz} (V: the
+ * previous velocity); out[3], out[4] hold the previous rho and p on entry;
+ * on return out = {Vx, Vy, Vz, rho, p}, flags = fix-ups {rho, p, drift},
+ * rec = the two values of each fix-up record {rho*, rho^n, p*, p^n, v*^2,
+ * v_drift^2}.  eps1: the reference's module variable eps1, which clamps a
+ * superluminal previous velocity (:238): 1e-6 once hll_flow_mpx has run in
+ * the process (16_hll_flow_mpx.f95:1375), else 0.  Returns 0 or the status
+ * of the reference's stop. */
+int oracle_c2p_point(double gamma, double eps1, const double in[14], double out[5], int flags[3], double rec[6]);
 
 #ifdef __cplusplus
 }

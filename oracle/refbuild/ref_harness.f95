@@ -11,7 +11,7 @@
 ! text read instead).
 !
 ! usage: cueva_ref INPUT NSTEPS OUTPREFIX [DUMPSTEPS...] [-debug] [-globals]
-!                  [-restart] [-lector H]
+!                  [-restart] [-lector H] [-c2p]
 !   INPUT      stream file: 14 arrays (-6:imax+6,-6:jmax+6) of real(8), x fastest,
 !              in the restart order psi phi Bx By Bz Vx Vy Vz Ex Ey Ez q rho p
 !   -restart   INPUT is instead a restart text file, read by the reference's
@@ -27,6 +27,15 @@
 !              which writes its global diagnostics (22_writer.f95:818-991) to
 !              the units 80..217 (unconnected: files fort.80 ... in the
 !              working directory)
+!   -c2p       per-function mode: INPUT also holds D tau Sx Sy Sz (5 more arrays
+!              after the 14); instead of NSTEPS steps the harness calls only
+!              varprimitivecardano once (h = 1, l = lmax, l_swap = 1, as at
+!              main_cueva_code.f95:301-303) and writes OUTPREFIX_h00001.bin; the
+!              fix-up records go to the unit 301-303 files as in a run
+!   -c2p-hll   the same after one call of hll_flow_mpx (l = 1) on the zeroed
+!              stage arrays, for the module variable eps1 it sets
+!              (16_hll_flow_mpx.f95:1375), which the recovery's clamp of a
+!              superluminal previous velocity reads (12_varprimitive.f95:238)
 !   -lector H  the same with h = H; H = 0 also writes the reference's restart
 !              file data_restart/all_variables_0000.dat (22_writer.f95:1116-1172)
 !              and its local dumps (the directories data_restart, data_print
@@ -44,7 +53,7 @@ program cueva_ref_harness
   character(len=1024) :: arg, infile, prefix, fname
   integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out, hlector
   integer, allocatable :: dumps(:)
-  logical             :: debug, globals, from_text
+  logical             :: debug, globals, from_text, c2p_only, c2p_hll
 
   nargs = command_argument_count()
   if (nargs < 3) then
@@ -60,6 +69,8 @@ program cueva_ref_harness
   debug = .false.
   globals = .false.
   from_text = .false.
+  c2p_only = .false.
+  c2p_hll = .false.
   hlector = 1
   iarg = 4
   do while (iarg <= nargs)
@@ -70,6 +81,11 @@ program cueva_ref_harness
         globals = .true.
      else if (trim(arg) == "-restart") then
         from_text = .true.
+     else if (trim(arg) == "-c2p") then
+        c2p_only = .true.
+     else if (trim(arg) == "-c2p-hll") then
+        c2p_only = .true.
+        c2p_hll = .true.
      else if (trim(arg) == "-lector") then
         globals = .true.
         iarg = iarg + 1
@@ -96,10 +112,28 @@ program cueva_ref_harness
      read(u_in) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
                 Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
                 q(:,:,1), rho(:,:,1), p(:,:,1)
+     if (c2p_only) read(u_in) D(:,:,1), tau(:,:,1), Sx(:,:,1), Sy(:,:,1), Sz(:,:,1)
      close(u_in)
   end if
 
   call boundary
+
+  if (c2p_only) then
+     h = 1
+     if (c2p_hll) then
+        l = 1
+        call hll_flow_mpx
+     end if
+     l = lmax
+     l_swap = 1
+     contador_r = 0
+     contador_p = 0
+     contador_v = 0
+     call varprimitivecardano
+     call write_dump(1)
+     nsteps = 0
+     globals = .false.
+  end if
   t_advance = (hini-1) * Delt
 
   do h = 1, nsteps
@@ -160,17 +194,7 @@ program cueva_ref_harness
      t_init = 1
 
      do kd = 1, ndump
-        if (dumps(kd) == h) then
-           write(fname,'(a,"_h",i5.5,".bin")') trim(prefix), h
-           open(newunit=u_out, file=trim(fname), access="stream", form="unformatted", &
-                status="replace", action="write")
-           write(u_out) contador_r, contador_p, contador_v
-           write(u_out) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
-                        Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
-                        q(:,:,1), rho(:,:,1), p(:,:,1), &
-                        D(:,:,1), tau(:,:,1), Sx(:,:,1), Sy(:,:,1), Sz(:,:,1)
-           close(u_out)
-        end if
+        if (dumps(kd) == h) call write_dump(h)
      end do
   end do
 
@@ -184,6 +208,20 @@ program cueva_ref_harness
   close(303)
 
 contains
+
+  ! counters (contador_r, contador_p, contador_v), then the 14 fields and D tau S
+  subroutine write_dump(hd)
+    integer, intent(in) :: hd
+    write(fname,'(a,"_h",i5.5,".bin")') trim(prefix), hd
+    open(newunit=u_out, file=trim(fname), access="stream", form="unformatted", &
+         status="replace", action="write")
+    write(u_out) contador_r, contador_p, contador_v
+    write(u_out) psi(:,:,1), phi(:,:,1), Bx(:,:,1), By(:,:,1), Bz(:,:,1), &
+                 Vx(:,:,1), Vy(:,:,1), Vz(:,:,1), Ex(:,:,1), Ey(:,:,1), Ez(:,:,1), &
+                 q(:,:,1), rho(:,:,1), p(:,:,1), &
+                 D(:,:,1), tau(:,:,1), Sx(:,:,1), Sy(:,:,1), Sz(:,:,1)
+    close(u_out)
+  end subroutine write_dump
 
   subroutine write_stage(ls)
     integer, intent(in) :: ls

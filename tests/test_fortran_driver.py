@@ -117,3 +117,50 @@ def test_driver_restart_files_are_the_reference_s(tmp_path, name):
     assert np.array_equal(vy[:, 0], table[:, 0]) and np.array_equal(vy[:, 2], table[:, 2 + k])
     assert np.array_equal(vy[:, 1], table[:, 1])
     assert len(os.listdir(os.path.join(whole, "data_global"))) == len(cueva.GLOBAL_NAMES)
+
+
+def write_restart_text(path, prims):
+    """A restart file of host planes in the reference's record order (i
+    outer, j inner, 14 values a record, 03_restart_primitive.f95:50-61),
+    shortest round-trip decimals, which list-directed input reads back exactly."""
+    nj, ni = prims.shape[1:]
+    with open(path, "w") as fh:
+        for i in range(ni):
+            for j in range(nj):
+                fh.write(" ".join(repr(float(v)) for v in prims[:, j, i]) + "\n")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["kh_hll_lowp_32x64", "kh_hllc_lowp_32x64"])
+def test_driver_fixup_files_are_the_reference_s(tmp_path, name):
+    """Crafted data whose recovery takes the pressure fix-up every step
+    (tests/crafted.py): the driver's fix-up files are the reference's own
+    byte for byte (12_varprimitive.f95:33-40,601) and its fields after the
+    fixture's steps are the reference's."""
+    import hashlib
+    import json
+    import crafted
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (needs flang)")
+    meta = json.load(open(os.path.join(ROOT, "tests", "golden", name + ".json")))
+    p, prims = crafted.problem(meta)
+    src = str(tmp_path / "initial.dat")
+    write_restart_text(src, prims)
+    assert np.array_equal(cueva.read_restart(src, p.imax, p.jmax), prims)
+    ni, nj = meta["native_grid"]
+    print("driver fix-up files", name, flush=True)
+    raw = _run_driver(str(tmp_path), ni, nj, meta["flux_solver"], meta["rec_mp"], meta["order"], meta["sigma"], 0.0,
+                      meta["nsteps"], 0, restart_in=src, hglb=0)
+    for kind, fn in (("rho", "fix-up_densidad.dat"), ("p", "fix-up_presion.dat"),
+                     ("v", "fix-up_velocity_drift.dat")):
+        got = hashlib.sha256(open(str(tmp_path / fn), "rb").read()).hexdigest()
+        assert got == meta["fixups"][kind]["sha256"], (kind, open(str(tmp_path / fn)).read()[:800])
+    cnt = np.frombuffer(raw[:12], dtype="<i4").tolist()
+    assert cnt == meta["dumps"][str(meta["nsteps"])]["counters"]
+    arr = np.frombuffer(raw[12:], dtype="<f8").reshape(19, p.jmax + 13, p.imax + 13)
+    inner = arr[:, 6:6 + p.jmax + 1, 6:6 + p.imax + 1]
+    fields = list(cueva.PRIM_NAMES) + list(cueva.CONS_NAMES)
+    ref = meta["dumps"][str(meta["nsteps"])]["sha256"]
+    bad = [f for k, f in enumerate(fields) if hashlib.sha256(np.ascontiguousarray(inner[k]).tobytes()).hexdigest()
+           != ref[f]]
+    assert not bad, bad

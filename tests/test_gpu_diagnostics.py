@@ -11,6 +11,7 @@ import sys
 import numpy as np
 import pytest
 
+import crafted
 import cueva
 from oracle.binding import Oracle
 
@@ -100,8 +101,7 @@ def test_globals_match_reference_lector(name):
         pytest.skip("fixture without the reference's global diagnostics")
     ni, nj = meta["native_grid"]
     say("globals vs reference", name)
-    p, prims = cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"], rec_mp=meta["rec_mp"],
-                                order=meta["order"], rotated=True, seed=meta["seed"], **meta.get("init", {}))
+    p, prims = crafted.problem(meta)
     with cueva.Solver(p) as s:
         s.upload(prims)
         s.step(meta["nsteps"])

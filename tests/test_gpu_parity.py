@@ -23,6 +23,7 @@ import sys
 import numpy as np
 import pytest
 
+import crafted
 import cueva
 from oracle.binding import Oracle
 from parity import FIELDS, interior, ndiff, rel_l1
@@ -130,10 +131,11 @@ def test_ssp3_mp9_stops_like_the_reference():
         s.upload(prims)
         s.step(7)
         s.step_counters()
-        s.step(1)
         with pytest.raises(cueva.CuevaError) as e:
-            s.step_counters()
+            s.step(1)
         assert e.value.code == 22
+        code, where = s.last_error()
+        assert code == 22 and where[:2] == (8, 4)  # step 8, in the recovery after stage 4
 
 
 def golden(name):
@@ -153,6 +155,8 @@ BITWISE_DUMPS = {
     "kh_hllc_mcl_32x64": (1, 10), "kh_hllc_mcl_bz_32x64": (1, 10),
     "kh_hlle_mcl_32x64": (1, 10), "kh_hlle_mcl_bz_32x64": (1, 10),
     "kh_hll_mcl_primrec_32x64": (1, 10), "kh_hllc_mp5_primrec_32x64": (1, 10),
+    # crafted data through the pressure fix-up and the HLLC face-density fix
+    "kh_hll_lowp_32x64": (1, 4), "kh_hllc_lowp_32x64": (1, 4), "kh_hllc_negrho_32x64": (1, 3),
 }
 
 
@@ -162,9 +166,7 @@ def test_against_reference_fixture(name):
     meta, arrays = golden(name)
     ni, nj = meta["native_grid"]
     say("fixture", name)
-    p, prims = cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"],
-                                rec_mp=meta["rec_mp"], order=meta["order"], rotated=True, seed=meta["seed"],
-                                **meta.get("init", {}))
+    p, prims = crafted.problem(meta)
     assert digest(prims) == meta["init_sha256"]
     with cueva.Solver(p) as s:
         s.upload(prims)
@@ -208,7 +210,9 @@ def test_gpu_stops_where_the_reference_stops():
         s.upload(prims)
         s.step(stop["stop_step"] - 1)
         s.step_counters()
-        s.step(1)
         with pytest.raises(cueva.CuevaError) as e:
-            s.step_counters()
+            s.step(1)
         assert e.value.code == 22
+        # the step, stage and cell of the reference's message
+        code, where = s.last_error()
+        assert where == (stop["stop_step"], stop["stage"], stop["cell"][0], stop["cell"][1])

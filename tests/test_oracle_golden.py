@@ -18,6 +18,7 @@ import os
 import numpy as np
 import pytest
 
+import crafted
 import cueva
 from oracle.binding import Oracle
 
@@ -34,10 +35,7 @@ def load_case(name):
 
 
 def problem(meta):
-    ni, nj = meta["native_grid"]
-    return cueva.kh_problem(ni, nj, sigma=meta["sigma"], flux_solver=meta["flux_solver"],
-                            rec_mp=meta["rec_mp"], order=meta["order"], rotated=True, seed=meta["seed"],
-                            **meta.get("init", {}))
+    return crafted.problem(meta)
 
 
 def digest(a):
@@ -49,7 +47,25 @@ def interior(prims, cons, p):
     return both[:, 6:6 + p.jmax + 1, 6:6 + p.imax + 1]
 
 
-SMALL = sorted(os.path.basename(f)[:-5] for f in glob.glob(os.path.join(GOLD, "*_32x64.json")))
+def same_values(a, b):
+    """Equal as the reference's shortest round-trip text reads back (NaN as NaN)."""
+    a, b = np.asarray(a, dtype=float).reshape(-1), np.asarray(b, dtype=float).reshape(-1)
+    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def check_face_events(o, arrays, name):
+    """The oracle's face fallbacks and density fixes are the faces the
+    reference printed, in its order: kind (fallback from an x face / a y face
+    only; the density fix of y_L, whose message says test_rho_L, or of
+    y_R / x_L / x_R, whose messages all say test_rho_R), face and value."""
+    kl, vals = o.face_event_list()
+    kind = np.where(kl[:, 0] <= 2, kl[:, 0], 3)
+    assert np.array_equal(kind, arrays["face_event_kind"]), name
+    assert np.array_equal(kl[:, 2:4], arrays["face_event_ij"]), name
+    assert same_values(vals, arrays["face_event_val"]), name
+
+
+SMALL = sorted(os.path.basename(f)[:-5] for f in glob.glob(os.path.join(GOLD, "kh_*_32x64.json")))
 LARGE = sorted(os.path.basename(f)[:-5] for f in glob.glob(os.path.join(GOLD, "*_128x256.json")))
 
 
@@ -60,9 +76,15 @@ def run_case(name):
     o = Oracle(p, libm="c")  # the C library, as the reference build
     o.upload(prims)
     done = 0
+    fixups = {k: ([], []) for k in range(3)}  # per kind: (h, i, j) and values of every step
     for d in sorted(int(k) for k in meta["dumps"]):
-        o.step(d - done)
-        done = d
+        while done < d:
+            o.step(1)
+            done += 1
+            for kind in range(3):
+                ij, vals = o.fixups(kind)
+                fixups[kind][0].extend((done, int(i), int(j)) for i, j in ij)
+                fixups[kind][1].extend(vals.tolist())
         pr, co = o.download()
         inner = interior(pr, co, p)
         ref = meta["dumps"][str(d)]
@@ -74,6 +96,15 @@ def run_case(name):
         else:
             diff = {}
         assert not bad, (name, d, bad, diff)
+    if "fixups" in meta and done == meta["nsteps"]:
+        # the records of the reference's fix-up files, all steps (12_varprimitive.f95:544,601,663)
+        for kind, fn in enumerate(("rho", "p", "v")):
+            assert len(fixups[kind][0]) == meta["fixups"][fn]["n"], (name, fn)
+            if meta["fixups"][fn]["n"]:
+                assert np.array_equal(np.array(fixups[kind][0]), arrays["fixup_%s_hij" % fn]), (name, fn)
+                assert same_values(np.array(fixups[kind][1]), arrays["fixup_%s_val" % fn]), (name, fn)
+    if meta.get("face_events"):
+        check_face_events(o, arrays, name)
     if "globals_final" in meta:
         # the reference's own lector output after the last step (22_writer.f95:818-991)
         assert done == meta["nsteps"]
@@ -113,3 +144,40 @@ def test_oracle_stops_where_the_reference_stops():
     o.step(stop["stop_step"] - 1)
     with pytest.raises(RuntimeError, match="status 22"):
         o.step(1)
+
+
+STOP_CASES = sorted(os.path.basename(f)[:-5] for f in glob.glob(os.path.join(GOLD, "stop_*.json")))
+
+
+@pytest.mark.parametrize("name", STOP_CASES)
+def test_oracle_stop_cases(name):
+    """One step of crafted data on which the reference stops
+    (tests/crafted.py: superluminal face states).  The oracle prints-equivalent
+    face events are the reference's, in order; it stops with the reference's
+    status (and, where the reference names it, at its cell and stage); the
+    right-hand sides of every stage the reference completed, and the face
+    fluxes of the last completed face pass, have the reference's SHA-256."""
+    meta, arrays = load_case(name)
+    p, prims = problem(meta)
+    assert digest(prims) == meta["init_sha256"]
+    o = Oracle(p, libm="c")
+    o.upload(prims)
+    st = o.step_status(1)
+    assert st == meta["stop"]["code"], (st, meta["stop"])
+    h, l, i, j = o.last_stop()
+    if "i" in meta["stop"]:
+        assert (h, l, i, j) == (meta["stop"]["h"], meta["stop"]["l"], meta["stop"]["i"], meta["stop"]["j"])
+    check_face_events(o, arrays, name)
+    stages = meta["stages"]
+    assert stages
+    im, jm = p.imax, p.jmax
+    for ls, ref in stages.items():
+        for q in range(17):
+            out = o.rhs(int(ls) - 1, q)
+            assert digest(out[6:6 + jm + 1, 6:6 + im + 1]) == ref["rhs_sha256"][q], (name, ls, q)
+    last_faces = str(max(int(x) for x in stages))
+    if meta["stop"]["code"] >= 20 and last_faces == str(2 if meta["order"] == 2 else 4):
+        ref = stages[last_faces]
+        for q in range(28):
+            out = o.face(q // 14, q % 14)
+            assert digest(out[5:5 + jm + 2, 5:5 + im + 2]) == ref["faces_sha256"][q], (name, q)
