@@ -224,6 +224,20 @@ int cueva_fixup_records(cueva_ctx *ctx, int kind, int32_t *hij, double *vals, in
 #define CUEVA_N_GLOBALS 37
 int cueva_reduce_globals(cueva_ctx *ctx, double out[CUEVA_N_GLOBALS]);
 
+/* cueva_reduce_globals for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): no context, no allocation and no wait.  No reference
+ * counterpart beyond cueva_reduce_globals itself.  prims14 / cons5: device
+ * memory holding the 14 primitive planes (restart order) and D tau Sx Sy Sz
+ * in the host layout of cueva_upload_prims ((imax+13)*(jmax+13) doubles a
+ * plane, x fastest); out: device memory for the 37 results; stream: the
+ * hipStream_t to queue on (NULL: the null stream).  The block partial sums
+ * live in a fixed device array of the library, 1024 x 37 doubles per GPU
+ * (every grid fits: the reduction strides over it), held until the process
+ * ends; calls that overlap on one GPU share it and race. */
+int cueva_reduce_globals_noalloc(const cueva_params *p, const double *prims14, const double *cons5, double *out,
+                                 void *stream);
+
 /* Largest relativistic acoustic signal speed |lambda| over the interior and
  * both directions.  No reference counterpart: the reference takes the fixed
  * dt = CFL*min(Delx, Dely, 1) of parameters.f95:312 (light speed 1, which

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1018,6 +1019,72 @@ extern "C" int cueva_reduce_globals(cueva_ctx* c, double out[37]) {
   LAUNCH(K_OTHER, k_globals_final, dim3(1), dim3(RED_THREADS), (const double*)c->red, res);
   HIPCHK(hipMemcpyAsync(out, res, G_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return CUEVA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The global diagnostics without a context and without allocating: the
+// block partial sums of k_globals_partial go to a fixed device array of this
+// file (one copy per GPU, held until the process ends) instead of the
+// context's reduction buffer.  Its size does not depend on the grid (the
+// partial kernel strides over any grid with RED_BLOCKS blocks), and every
+// call writes all of it before k_globals_final reads it, so nothing needs
+// clearing.  Calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[(size_t)RED_BLOCKS * G_N];
+constexpr int MAX_DEVICES = 64;
+static void* scratch_ptr[MAX_DEVICES];
+static std::once_flag scratch_once[MAX_DEVICES];
+static hipError_t scratch_err[MAX_DEVICES];
+
+// The device address of this GPU's copy of the array (the array's name is
+// only a host-side stand-in), looked up once per device with it current.
+static double* scratch_of(int dev) {
+  std::call_once(scratch_once[dev], [dev]() {
+    void* ptr = nullptr;
+    scratch_err[dev] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    scratch_ptr[dev] = scratch_err[dev] == hipSuccess ? ptr : nullptr;
+  });
+  return (double*)scratch_ptr[dev];
+}
+
+extern "C" int cueva_reduce_globals_noalloc(const cueva_params* p, const double* prims14, const double* cons5,
+                                            double* out, void* stream) {
+  if (!p || !prims14 || !cons5 || !out) return CUEVA_E_ARG;
+  int rc = check_params(p);
+  if (rc) return rc;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return CUEVA_E_ARG;
+  double* partial = scratch_of(dev);
+  if (!partial) return CUEVA_E_HIP;
+  // the caller's planes in the host layout (-6:imax+6, -6:jmax+6), x fastest:
+  // with pitch imax+13, Geom::at(i, j) - 2 is the element's index in a plane
+  Geom g;
+  memset(&g, 0, sizeof(g));
+  g.imax = p->imax;
+  g.jmax = p->jmax;
+  g.pitch = p->imax + 13;
+  g.rows = p->jmax + 13;
+  g.plane = (int64_t)g.pitch * g.rows;
+  g.bcx = p->bc_x;
+  g.bcy = p->bc_y;
+  Planes P;
+  memset(&P, 0, sizeof(P));
+  for (int k = 0; k < P_N; k++) P.p[k] = (double*)(prims14 + (size_t)k * (size_t)g.plane) - 2;
+  for (int k = 0; k < 5; k++) P.c[k] = (double*)(cons5 + (size_t)k * (size_t)g.plane) - 2;
+  Scheme S;
+  memset(&S, 0, sizeof(S));
+  double d[5];
+  cueva_derived(p, d);
+  S.dx = d[0];
+  S.dy = d[1];
+  S.gamma = p->gamma;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_globals_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, g, P, S, partial);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_globals_final, dim3(1), dim3(RED_THREADS), 0, s, (const double*)partial, out);
+  HIPCHK(hipGetLastError());
   return CUEVA_OK;
 }
 

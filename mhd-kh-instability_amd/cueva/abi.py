@@ -98,6 +98,7 @@ SIGNATURES = {
     "cueva_fixup_records": (_i, [_c, _i, _P(ctypes.c_int32), _P(_d), ctypes.c_int64, _P(ctypes.c_int64)]),
     "cueva_reduce_globals": (_i, [_c, _P(_d)]),
     "cueva_max_speed": (_i, [_c, _P(_d)]),
+    "cueva_reduce_globals_noalloc": (_i, [_P(Params), _c, _c, _c, _c]),
     "cueva_kh_init": (_i, [_i, _i, _d, _d, _i, _d, _d, _d, ctypes.c_uint64, _P(_d)]),
     "cueva_upload_cons": (_i, [_c, _P(_d)]),
     "cueva_download_plane": (_i, [_c, _i, _i, _P(_d)]),
