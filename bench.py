@@ -252,7 +252,7 @@ def main():
     if args.form:
         forms = [args.form]
     else:
-        forms = ["plain_fp64", "mixed_learned"] if packed is not None else ["plain_fp64", "mixed"]
+        forms = ["plain_fp64", "mixed", "mixed_learned"] if packed is not None else ["plain_fp64", "mixed"]
     if "mixed_learned" in forms and packed is None:
         sys.exit("no trained parameters (cueva/c2p_mlp.json) for the learned form")
     res = {f: time_form(cfg, f, args.steps, args.warmup, packed) for f in forms}

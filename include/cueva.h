@@ -190,10 +190,11 @@ int cueva_step_counters(cueva_ctx *ctx, int64_t counters[3], int64_t *fallback);
  * where = {step index h, stage l (1-based; lmax for the primitive recovery),
  * i, j} -- the reference's (h, l, i, j) of its stop message
  * (12_varprimitive.f95:374-425,482-486,684-693; funciones.f95:238-244).
- * After a failure inside cueva_step the planes hold the state the failing
- * launch worked on (a face failure: U^n of the failing step; a recovery
- * failure: U^{n+1}, with the recovered primitives at the cells that
- * succeeded).  No reference counterpart beyond the stop message itself. */
+ * After a failure inside cueva_step the primitives are those at the start
+ * of the failing step (the step's later launches change no data, and its
+ * recovery writes to scratch planes that take the primitives' place only
+ * once it has succeeded).  No reference counterpart beyond the stop message
+ * itself. */
 int cueva_last_error(cueva_ctx *ctx, int32_t where[4]);
 
 /* The step index the next step carries in failure locations and fix-up

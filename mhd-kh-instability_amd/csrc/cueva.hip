@@ -20,7 +20,7 @@ namespace cueva {
 hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, const Geom& g, const Planes& P,
                              const Scheme& S, const StatusRef& st);
 // FP32 marching face kernels (cueva_mixed.hip)
-const void* march_kernel_fp32(int rec_order, int solver, bool final_stage);
+const void* march_kernel_fp32(int rec_order, int solver, bool final_stage, bool c2p);
 // learned primitive recovery (cueva_mlp.hip)
 hipError_t launch_c2p_learned(hipStream_t s, const Geom& g, const Planes& P, const Scheme& S, const float* wp,
                               const StatusRef& st, bool stage0);
@@ -57,7 +57,8 @@ struct cueva_ctx {
   int march;         // fused step uses the marching face kernel (default; CUEVA_MARCH=0: face planes)
   int march_rounds;  // resident-block rounds a marching launch is cut into (CUEVA_MARCH_ROUNDS)
   int march_rows;    // rows per segment when set (CUEVA_MARCH_SEG; rounded up to 4m - 1)
-  int march_seg[2];  // rows per segment of the stage / last-stage kernel (0: not sized yet)
+  int c2p_in_march;  // Cardano recovery inside the last marching launch (CUEVA_C2P_IN_MARCH)
+  int march_seg[3];  // rows per segment of the stage / last-stage / last-stage + recovery kernel (0: not sized yet)
   // failures and fix-up records
   FixRec* rec;          // device: [3][FIX_CAP], after mlpw
   uint64_t seq;         // launches numbered so far
@@ -243,6 +244,11 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     c->march_rounds = e && atoi(e) > 0 ? atoi(e) : 2;
     e = getenv("CUEVA_MARCH_SEG");
     c->march_rows = e && atoi(e) > 0 ? atoi(e) : 0;
+    // default off: inside the last marching launch the recovery's registers
+    // cost the whole launch occupancy, measured 1.3 ms (FP32 faces) and 3.3
+    // ms (FP64 faces) per step at 2048x4096 against 0.58 ms for its own pass
+    e = getenv("CUEVA_C2P_IN_MARCH");
+    c->c2p_in_march = e ? atoi(e) != 0 : 0;
   }
 
   const int nplanes = P_N + C_N + 2 * S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N;
@@ -290,6 +296,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     for (int k = 0; k < R_N; k++) c->P.rhs[l][k] = take();
   for (int k = 0; k < A_N; k++) c->P.ints[k] = take();
   for (int k = 0; k < A_N; k++) c->P.fins[k] = take();
+  recover_in_place(c->P);
   // every plane pointer the kernels receive must lie inside the allocation
   {
     const char* lo = (const char*)c->base;
@@ -381,7 +388,7 @@ static int launch_ok() { return hipGetLastError() == hipSuccess ? CUEVA_OK : CUE
 // Number the next launch.  record: a launch that can fail (face solves,
 // primitive recovery), remembered with its step and stage l (1-based) and,
 // for the fused step, the plane roles it runs with.
-static StatusRef sref(cueva_ctx* c, int l, bool record = false, bool fused = false) {
+static StatusRef sref(cueva_ctx* c, int l, bool record = false, bool fused = false, const Planes* snap = nullptr) {
   StatusRef r;
   r.st = c->dst;
   r.rec = c->rec;
@@ -394,7 +401,7 @@ static StatusRef sref(cueva_ctx* c, int l, bool record = false, bool fused = fal
     lr.h = c->hstep;
     lr.l = l;
     lr.fused = fused;
-    lr.P = c->P;
+    lr.P = snap ? *snap : c->P;
     c->launches->push_back(lr);
   }
   return r;
@@ -632,12 +639,15 @@ static int read_status(cueva_ctx* c, DevStatus* h) {
     c->err_where[0] = -1;
     c->err_where[1] = -1;
     c->err_where[2] = (int32_t)((key >> 8) & 0xFFFFF) - 8;
-    c->err_where[3] = (int32_t)(key >> 28) - 8;
+    c->err_where[3] = (int32_t)((key >> 28) & 0xFFFFF) - 8;
     for (const LaunchRecord& lr : *c->launches)
       if (lr.seq == h->seq) {
         c->err_where[0] = (int32_t)lr.h;
         c->err_where[1] = lr.l;
-        if (lr.fused) c->P = lr.P;
+        if (lr.fused) {
+          c->P = lr.P;
+          recover_in_place(c->P);
+        }
         break;
       }
   }
@@ -647,8 +657,10 @@ static int read_status(cueva_ctx* c, DevStatus* h) {
 
 // Primitive recovery after the last stage (l_swap = 1).  fused: inside
 // cueva_step; stage0: also the next step's stage-0 state at the interior.
-static int launch_c2p(cueva_ctx* c, bool fused, int cls = K_OTHER, bool stage0 = false) {
-  const StatusRef sr = sref(c, c->S.lmax, true, fused);
+// snap: the plane roles a failure restores (default: those of this launch)
+static int launch_c2p(cueva_ctx* c, bool fused, int cls = K_OTHER, bool stage0 = false,
+                      const Planes* snap = nullptr) {
+  const StatusRef sr = sref(c, c->S.lmax, true, fused, snap);
   if (c->prm.c2p_mode == CUEVA_C2P_LEARNED) {
     if (!c->has_mlp) return CUEVA_E_ARG;
     prof_begin(c, cls);
@@ -729,16 +741,33 @@ static void adopt_update(cueva_ctx* c, int L) {
   }
 }
 
-static const void* march_kernel_of(cueva_ctx* c, bool fin) {
-  if (c->prm.precision == CUEVA_MIXED) return march_kernel_fp32(c->rec_order, c->prm.flux_solver, fin);
-  return march_kernel<double>(c->rec_order, c->prm.flux_solver, fin);
+static const void* march_kernel_of(cueva_ctx* c, bool fin, bool c2p) {
+  if (c->prm.precision == CUEVA_MIXED) return march_kernel_fp32(c->rec_order, c->prm.flux_solver, fin, c2p);
+  return march_kernel<double>(c->rec_order, c->prm.flux_solver, fin, c2p);
+}
+
+// The recovered Vx Vy Vz rho p, written to the x-face planes (unused by the
+// marching kernel), take the place of the primitive planes they replace.
+static void take_recovered(cueva_ctx* c) {
+  static const int idx[5] = {P_VX, P_VY, P_VZ, P_RHO, P_P};
+  for (int q = 0; q < 5; q++) {
+    double* t = c->P.p[idx[q]];
+    c->P.p[idx[q]] = c->P.fx[q];
+    c->P.fx[q] = t;
+    c->P.fxf[q] = (float*)t;
+  }
+  recover_in_place(c->P);
+}
+
+static void recover_to_scratch(cueva_ctx* c) {
+  for (int q = 0; q < 5; q++) c->P.pn[q] = c->P.fx[q];
 }
 
 // Rows per segment of a marching launch: enough segments that the blocks
 // fill the resident slots of every CU march_rounds times, seg = MW m - 1 (the
 // segment's face rows, seg + 1, fill whole tile steps of MW rows).
-static int march_seg(cueva_ctx* c, bool fin) {
-  int& seg = c->march_seg[fin ? 1 : 0];
+static int march_seg(cueva_ctx* c, bool fin, bool c2p) {
+  int& seg = c->march_seg[fin ? (c2p ? 2 : 1) : 0];
   if (seg) return seg;
   const int mw = march_waves(c);
   if (c->march_rows > 0) {
@@ -748,7 +777,7 @@ static int march_seg(cueva_ctx* c, bool fin) {
   int per_cu = 0, cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel_of(c, fin), 64 * mw, 0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel_of(c, fin, c2p), 64 * mw, 0) != hipSuccess)
     return 0;
   const int64_t slots = (int64_t)(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
   const int strips = march_strips(c->g), rows = c->g.jmax + 1;
@@ -760,9 +789,11 @@ static int march_seg(cueva_ctx* c, bool fin) {
   return seg;
 }
 
-static int launch_march(cueva_ctx* c, int l, bool fin) {
-  const void* fn = march_kernel_of(c, fin);
-  int seg = march_seg(c, fin);
+// c2p (last stage only): with the primitive recovery (to P.pn) and, with
+// stage0, the next step's stage-0 state (to P.sn)
+static int launch_march(cueva_ctx* c, int l, bool fin, bool c2p = false, bool stage0 = false) {
+  const void* fn = march_kernel_of(c, fin, c2p);
+  int seg = march_seg(c, fin, c2p);
   if (!fn || seg <= 0) return CUEVA_E_HIP;
   const dim3 grid((unsigned)march_strips(c->g), (unsigned)((c->g.jmax + 1 + seg - 1) / seg));
   MarchArgs ma;
@@ -772,6 +803,7 @@ static int launch_march(cueva_ctx* c, int l, bool fin) {
   ma.l = l;
   ma.seg = seg;
   ma.st = sref(c, l + 1, true, true);
+  ma.stage0 = stage0 ? 1 : 0;
   note_face_solver(c);
   void* args[] = {(void*)&ma};
   prof_begin(c, K_FACES);
@@ -819,8 +851,12 @@ static int fused_steps(cueva_ctx* c, int nsteps, int halo, bool learned) {
       }
       if (c->march) {
         // reconstruction + Riemann + flux difference in one pass; then the
-        // next stage's state (interior) or U^{n+1}
-        int rc = launch_march(c, l, last);
+        // next stage's state (interior) or U^{n+1}; the Cardano recovery
+        // runs inside the last launch (the learned one in its own pass)
+        const bool in_kernel = last && !learned && c->c2p_in_march;
+        const Planes pre = c->P;
+        if (last) recover_to_scratch(c);
+        int rc = launch_march(c, l, last, in_kernel, in_kernel && h + 1 < nsteps && !c->prm.primitive_rec);
         if (rc) return rc;
         if (!last) {
           for (int t = 0; t < S_N; t++) {
@@ -835,7 +871,16 @@ static int fused_steps(cueva_ctx* c, int nsteps, int halo, bool learned) {
         // the primitive recovery also forms the next step's stage-0 state
         // (not when the next step first re-forms the primitives)
         const bool fuse = h + 1 < nsteps && !c->prm.primitive_rec;
-        if ((rc = launch_c2p(c, true, K_FINAL, fuse))) return rc;
+        if (!in_kernel) {
+          if ((rc = launch_c2p(c, true, K_FINAL, fuse, &pre))) return rc;
+        } else if (fuse) {
+          for (int t = 0; t < S_N; t++) {
+            double* x = c->P.s[t];
+            c->P.s[t] = c->P.sn[t];
+            c->P.sn[t] = x;
+          }
+        }
+        take_recovered(c);
         // ghosts of the new primitives (and of the next stage-1 state)
         if ((rc = fuse ? fill_stage_ghosts(c, true) : fill_ghosts(c, c->P.p, P_N))) return rc;
         s0_ready = fuse;

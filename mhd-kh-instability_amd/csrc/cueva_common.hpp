@@ -37,7 +37,20 @@ struct Planes {
   double* rhs[MAXL][R_N];
   double* ints[A_N];
   double* fins[A_N];
+  // where a primitive recovery writes Vx Vy Vz rho p: the primitive planes
+  // themselves, or (inside the marching step, whose stencils still read the
+  // t^n values) scratch planes that then take their place
+  double* pn[5];
 };
+
+// the recovery writes in place
+inline void recover_in_place(Planes& P) {
+  P.pn[0] = P.p[P_VX];
+  P.pn[1] = P.p[P_VY];
+  P.pn[2] = P.p[P_VZ];
+  P.pn[3] = P.p[P_RHO];
+  P.pn[4] = P.p[P_P];
+}
 
 struct Scheme {
   double dx, dy, dt, sigma, kpsi, kphi, gamma;
@@ -83,17 +96,20 @@ struct StatusRef {
 };
 
 // Cells are ordered as the reference's loops visit them (j outer, i inner),
-// so the smallest key of a launch is the cell at which the reference stops.
-__device__ inline unsigned long long fail_key(int code, int i, int j) {
-  return ((unsigned long long)(j + 8) << 28) | ((unsigned long long)(i + 8) << 8) | (unsigned long long)code;
+// after the phase of the step they belong to (0: face solves, 1: primitive
+// recovery, which the reference runs after every face of the stage), so the
+// smallest key of a launch is the cell at which the reference stops.
+__device__ inline unsigned long long fail_key(int code, int i, int j, int phase) {
+  return ((unsigned long long)phase << 48) | ((unsigned long long)(j + 8) << 28) |
+         ((unsigned long long)(i + 8) << 8) | (unsigned long long)code;
 }
 
 // Record a failure.  Only the launch that failed first reports; within it the
-// first cell in loop order wins.
-__device__ inline void report(const StatusRef& r, int code, int i, int j) {
+// first cell in the reference's order wins.
+__device__ inline void report(const StatusRef& r, int code, int i, int j, int phase = 0) {
   const unsigned long long s0 = atomicCAS(&r.st->seq, 0ULL, r.seq);
   if (s0 != 0ULL && s0 != r.seq) return;
-  atomicMax(&r.st->key, ~fail_key(code, i, j));
+  atomicMax(&r.st->key, ~fail_key(code, i, j, phase));
 }
 
 // An earlier launch failed: the step stops there and later launches leave

@@ -32,21 +32,25 @@ struct dd {
 };
 
 CR_FN dd two_sum(double a, double b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   double s = a + b;
   double bb = s - a;
   double e = (a - (s - bb)) + (b - bb);
   return {s, e};
 }
 CR_FN dd quick_two_sum(double a, double b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   double s = a + b;
   double e = b - (s - a);
   return {s, e};
 }
 CR_FN dd two_prod(double a, double b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   double p = a * b;
   return {p, fma(a, b, -p)};
 }
 CR_FN dd add(dd a, dd b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   dd s = two_sum(a.hi, b.hi);
   dd t = two_sum(a.lo, b.lo);
   s.lo = s.lo + t.hi;
@@ -55,16 +59,19 @@ CR_FN dd add(dd a, dd b) {
   return quick_two_sum(s.hi, s.lo);
 }
 CR_FN dd add_d(dd a, double b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   dd s = two_sum(a.hi, b);
   s.lo = s.lo + a.lo;
   return quick_two_sum(s.hi, s.lo);
 }
 CR_FN dd mul(dd a, dd b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   dd p = two_prod(a.hi, b.hi);
   p.lo = p.lo + (a.hi * b.lo + a.lo * b.hi);
   return quick_two_sum(p.hi, p.lo);
 }
 CR_FN dd mul_d(dd a, double b) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   dd p = two_prod(a.hi, b);
   p.lo = p.lo + a.lo * b;
   return quick_two_sum(p.hi, p.lo);
@@ -73,6 +80,7 @@ CR_FN dd mul_d(dd a, double b) {
 // 1/(2n)! and 1/(2n+1)! as double-double, n = 0..16, signs (-1)^n applied
 #define CR_C(h, l) dd{h, l}
 CR_FN dd cos_coef(int n) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   switch (n) {
     case 0: return CR_C(0x1.0000000000000p+0, 0.0);
     case 1: return CR_C(-0x1.0000000000000p-1, 0.0);
@@ -94,6 +102,7 @@ CR_FN dd cos_coef(int n) {
   }
 }
 CR_FN dd sin_coef(int n) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   switch (n) {
     case 0: return CR_C(0x1.0000000000000p+0, 0.0);
     case 1: return CR_C(-0x1.5555555555555p-3, -0x1.5555555555555p-57);
@@ -118,6 +127,7 @@ CR_FN dd sin_coef(int n) {
 
 // cos(x) in double-double for |x| <= 4
 CR_FN dd cos_dd(double x) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c07p-54, P3 = -0x1.f1976b7ed8fbcp-110;
   double kf = rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
   int k = (int)kf;
@@ -140,11 +150,13 @@ CR_FN dd cos_dd(double x) {
 }
 
 CR_FN double cos_cr(double x) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   if (!(fabs(x) <= 4.0)) return cos(x);
   return cos_dd(x).hi;
 }
 
 CR_FN double acos_cr(double x) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   if (!(fabs(x) < 1.0)) return acos(x);
   double y0 = acos(x);
   dd d = add_d(cos_dd(y0), -x);  // cos(y0) - x
@@ -154,6 +166,7 @@ CR_FN double acos_cr(double x) {
 // x**THIRD with THIRD the double nearest 1/3 (parameters.f95:88), i.e.
 // x^((1 - 2^-54)/3) = cbrt(x * x^(-2^-54)), for x > 0.
 CR_FN double pow_third_cr(double x) {
+#pragma clang fp contract(off)  // exact error-free transforms: no fused products
   if (!(x > 0.0) || !(x < 1.0e300) || !(x > 1.0e-300)) return pow(x, 0x1.5555555555555p-2);
   double r0 = cbrt(x);
   double tlo = -(x * log(x)) * 0x1p-54;

@@ -949,6 +949,7 @@ struct Quartic { double e2b2, g1, v2old, b3, b2, b1, b0, d2, d1, d0, c1, c2; };
 // eps1: the clamp of a superluminal previous velocity (12_varprimitive.f95:236-241),
 // whose square enters only the pressure loop
 __device__ inline Quartic c2p_quartic(double gamma, double eps1, const C2PIn& in) {
+  #pragma clang fp contract(off)  // FP64 as the reference build, also where the FP32 face unit inlines it
   Quartic qd;
   qd.e2b2 = 0.5 * (((((sq(in.ex) + sq(in.ey)) + sq(in.ez)) + sq(in.bx)) + sq(in.by)) + sq(in.bz));
   qd.g1 = (gamma - 1.0) / gamma;
@@ -974,6 +975,7 @@ __device__ inline Quartic c2p_quartic(double gamma, double eps1, const C2PIn& in
 
 // Ferrari-Cardano root (12_varprimitive.f95:297-500); returns 0 or a status.
 __device__ inline int c2p_cardano(const Quartic& qd, double& Wn, bool crm) {
+  #pragma clang fp contract(off)  // FP64 as the reference build, also where the FP32 face unit inlines it
   const double pi = 3.14159265358979323846264338327950288;
   double d2 = qd.d2, d1 = qd.d1, d0 = qd.d0;
   double e = (20.0 * d2) / 8.0;
@@ -1027,12 +1029,14 @@ __device__ inline int c2p_cardano(const Quartic& qd, double& Wn, bool crm) {
 
 // one Newton update of W on the quartic (12_varprimitive.f95:513-514)
 __device__ inline double c2p_newton_w(const Quartic& qd, double Wn) {
+  #pragma clang fp contract(off)  // FP64 as the reference build, also where the FP32 face unit inlines it
   return Wn - ((((pw4(Wn) + qd.b3 * pw3(Wn)) + qd.b2 * sq(Wn)) + qd.b1 * Wn) + qd.b0) /
                   (((4.0 * pw3(Wn) + (3.0 * qd.b3) * sq(Wn)) + (2.0 * qd.b2) * Wn) + qd.b1);
 }
 
 // quartic residual relative to its leading term, used by the learned guess
 __device__ inline double c2p_residual(const Quartic& qd, double W) {
+  #pragma clang fp contract(off)  // FP64 as the reference build, also where the FP32 face unit inlines it
   double f = (((pw4(W) + qd.b3 * pw3(W)) + qd.b2 * sq(W)) + qd.b1 * W) + qd.b0;
   return fabs(f) / pw4(W);
 }
@@ -1041,6 +1045,7 @@ __device__ inline double c2p_residual(const Quartic& qd, double W) {
 // reference's Newton loop on W first.  Returns 0 or a status.
 __device__ inline int c2p_finish(double gamma, const C2PIn& in, const Quartic& qd, double Wn, bool newton,
                                  C2POut& o) {
+  #pragma clang fp contract(off)  // FP64 as the reference build, also where the FP32 face unit inlines it
   const double TOL = 1.0e-10;
   const int nrmax = 40;
   if (newton) {
@@ -1117,6 +1122,7 @@ __device__ inline int c2p_finish(double gamma, const C2PIn& in, const Quartic& q
 }
 
 __device__ inline int c2p_cardano_full(double gamma, double eps1, const C2PIn& in, C2POut& o, bool crm) {
+  #pragma clang fp contract(off)  // FP64 as the reference build, also where the FP32 face unit inlines it
   Quartic qd = c2p_quartic(gamma, eps1, in);
   double Wn;
   int st = c2p_cardano(qd, Wn, crm);

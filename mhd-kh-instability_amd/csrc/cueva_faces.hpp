@@ -554,7 +554,9 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
 //                and the next stage's state (sumint, varint, electricvarint,
 //                flowinit) -> sn, interior points;
 //   FINAL true:  sumfin + varfin + electricfield of the last stage -> U^{n+1}
-//                into rhs[l][0..13], for the primitive-recovery pass.
+//                into rhs[l][0..13]; with C2P also the primitive recovery
+//                of every point (-> pn, W) and, with args.stage0, the next
+//                step's stage-0 state (-> sn); else the recovery pass follows.
 // ---------------------------------------------------------------------------
 constexpr int MARCH_COLS = 63;
 
@@ -568,6 +570,7 @@ struct MarchArgs {
   Scheme S;
   int l, seg;
   StatusRef st;
+  int stage0;  // C2P: also the next step's stage-0 state
 };
 static_assert(offsetof(MarchArgs, P) == 0, "argument block layout");
 
@@ -594,7 +597,7 @@ __host__ __device__ constexpr int march_block_waves(bool fp32, int solver) {
   return solver == 6 ? 4 : (fp32 ? CUEVA_MARCH_F32_BLOCK_WAVES : CUEVA_MARCH_F64_BLOCK_WAVES);
 }
 
-template <int ORD, int SOLVER, typename T, bool FINAL>
+template <int ORD, int SOLVER, typename T, bool FINAL, bool C2P = false>
 __global__ void __launch_bounds__(64 * march_block_waves(sizeof(T) == 4, SOLVER), (march_min_waves<T, SOLVER>()))
 k_faces_march(MarchArgs args) {
   constexpr int MW = march_block_waves(sizeof(T) == 4, SOLVER);
@@ -686,6 +689,7 @@ k_faces_march(MarchArgs args) {
         final_update_from(P, S, k, r, nu);
 #pragma unroll
         for (int a = 0; a < A_N; a++) P.rhs[l][a][k] = nu[a];
+        if constexpr (C2P) recover_point(P, S, st, i, j, k, nu, A.stage0);
       }
     }
     if (w == 0) {
@@ -698,10 +702,11 @@ k_faces_march(MarchArgs args) {
 
 // kernel of (rec_order, solver, stage kind), for a launch by pointer
 template <typename T>
-inline const void* march_kernel(int rec_order, int solver, bool final_stage) {
-#define CUEVA_MK(O, SV)                                                        \
-  (final_stage ? reinterpret_cast<const void*>(&k_faces_march<O, SV, T, true>) \
-               : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false>))
+inline const void* march_kernel(int rec_order, int solver, bool final_stage, bool c2p) {
+#define CUEVA_MK(O, SV)                                                                          \
+  (final_stage ? (c2p ? reinterpret_cast<const void*>(&k_faces_march<O, SV, T, true, true>)      \
+                      : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, true, false>))    \
+               : reinterpret_cast<const void*>(&k_faces_march<O, SV, T, false, false>))
   if (solver == 1) return CUEVA_MK(5, 1);  // second order: the MP order does not enter
   if (solver == 2) return CUEVA_MK(5, 2);
   if (solver == 3) return CUEVA_MK(5, 3);

@@ -43,8 +43,8 @@ hipError_t launch_faces_fp32(int rec_order, dim3 grid, dim3 blk, hipStream_t s, 
 }
 
 // the FP32 marching face kernel of (rec_order, solver, stage kind)
-const void* march_kernel_fp32(int rec_order, int solver, bool final_stage) {
-  return march_kernel<float>(rec_order, solver, final_stage);
+const void* march_kernel_fp32(int rec_order, int solver, bool final_stage, bool c2p) {
+  return march_kernel<float>(rec_order, solver, final_stage, c2p);
 }
 
 }  // namespace cueva

@@ -206,11 +206,9 @@ __global__ void __launch_bounds__(256) k_c2p_learned(Geom g, Planes P, Scheme S,
   const int code = c2p_learned_cell(S.gamma, S.eps1, in, wp, o, lane, S.crmath != 0, out, fallback);
   if (!valid) return;
   if (fallback) atomicAdd(&sr.st->fallback, 1ULL);
-  if (code) { report(sr, code, i, j); return; }
+  if (code) { report(sr, code, i, j, 1); return; }
   c2p_count(sr, i, j, out);
-  P.p[P_VX][k] = out.vx; P.p[P_VY][k] = out.vy; P.p[P_VZ][k] = out.vz;
-  P.p[P_RHO][k] = out.rho; P.p[P_P][k] = out.p;
-  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(out.vx) + sq(out.vy)) + sq(out.vz)));
+  c2p_store_out(P, k, out);
   if (STAGE0) stage0_after_c2p(P, S, k, in, out);
 }
 

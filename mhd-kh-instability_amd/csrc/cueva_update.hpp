@@ -145,14 +145,20 @@ __device__ inline void c2p_count(const StatusRef& sr, int i, int j, const C2POut
   if (o.fix_v) add_fixup(sr, 2, i, j, o.rec[4], o.rec[5]);
 }
 
-__device__ inline bool c2p_apply(Planes& P, const Scheme& S, const StatusRef& sr, int i, int j, int64_t k,
+// the recovered primitives of one point and its W (12_varprimitive.f95:793-802)
+__device__ inline void c2p_store_out(const Planes& P, int64_t k, const C2POut& o) {
+#pragma clang fp contract(off)
+  P.pn[0][k] = o.vx; P.pn[1][k] = o.vy; P.pn[2][k] = o.vz;
+  P.pn[3][k] = o.rho; P.pn[4][k] = o.p;
+  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
+}
+
+__device__ inline bool c2p_apply(const Planes& P, const Scheme& S, const StatusRef& sr, int i, int j, int64_t k,
                                  const C2PIn& in, C2POut& o) {
   int code = c2p_cardano_full(S.gamma, S.eps1, in, o, S.crmath != 0);
-  if (code) { report(sr, code, i, j); return false; }
+  if (code) { report(sr, code, i, j, 1); return false; }
   c2p_count(sr, i, j, o);
-  P.p[P_VX][k] = o.vx; P.p[P_VY][k] = o.vy; P.p[P_VZ][k] = o.vz;
-  P.p[P_RHO][k] = o.rho; P.p[P_P][k] = o.p;
-  P.c[C_W][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
+  c2p_store_out(P, k, o);
   return true;
 }
 
@@ -165,6 +171,30 @@ __device__ inline C2PIn c2p_load(const Planes& P, int64_t k) {
   in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
   in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
   return in;
+}
+
+// Primitive recovery of one point from its U^{n+1} still in registers (the
+// marching kernel's last stage, which thereby replaces the recovery pass):
+// the t^n velocity, rho and p come from the primitive planes, which the
+// launch's stencils still read, so the results go to P.pn; with stage0 the
+// next step's stage-0 state goes to P.sn (P.s is the launch's input).
+__device__ inline void recover_point(const Planes& P, const Scheme& S, const StatusRef& sr, int i, int j, int64_t k,
+                                     const double* nu, int stage0) {
+  C2PIn in;
+  in.d = nu[A_D]; in.tau = nu[A_TAU];
+  in.sx = nu[A_SX]; in.sy = nu[A_SY]; in.sz = nu[A_SZ];
+  in.ex = nu[A_EX]; in.ey = nu[A_EY]; in.ez = nu[A_EZ];
+  in.bx = nu[A_BX]; in.by = nu[A_BY]; in.bz = nu[A_BZ];
+  in.vx0 = P.p[P_VX][k]; in.vy0 = P.p[P_VY][k]; in.vz0 = P.p[P_VZ][k];
+  in.rho_old = P.p[P_RHO][k]; in.p_old = P.p[P_P][k];
+  C2POut o;
+  if (!c2p_apply(P, S, sr, i, j, k, in, o)) return;
+  if (stage0) {
+    double sv[S_N];
+    stage_point(P, S, 0, k, false, o.vx, o.vy, o.vz, in.ex, in.ey, in.ez, in.bx, in.by, in.bz, o.rho, o.p,
+                nu[A_PSI], nu[A_PHI], nu[A_Q], sv);
+    store_stage(P.sn, k, sv);
+  }
 }
 
 // The next step's stage-0 state of an interior point from its new primitives

@@ -155,3 +155,33 @@ def test_multi_step_call_matches_face_planes(name, precision):
     say("  differing points", sum(nd.values()))
     assert out[True][1] == out[False][1]
     assert np.array_equal(out[True][0], out[False][0]), nd
+
+
+@pytest.mark.parametrize("name", ["hllc_mp5", "hlle_mp7_bz", "hll_mp9_ssp3", "hllc_mcl_bz"])
+@pytest.mark.parametrize("precision", [cueva.FP64, cueva.MIXED])
+def test_recovery_in_last_marching_launch(name, precision):
+    """The Cardano recovery (and the next step's stage-0 state) inside the last
+    marching launch (CUEVA_C2P_IN_MARCH=1, measured slower than its own pass
+    and so not the default): the same fields and counters as the face-plane
+    path with its separate recovery."""
+    fs, rec, order, sigma, kw = CASES[name]
+    p, prims = cueva.kh_problem(130, 68, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True,
+                                precision=precision, crmath=1 - precision, **kw)
+    steps = 3 if order == 4 else 6
+    say("case", name, "precision", precision, "recovery in the marching launch", steps, "steps")
+    out = {}
+    for march in (False, True):
+        os.environ["CUEVA_MARCH"] = "1" if march else "0"
+        os.environ["CUEVA_C2P_IN_MARCH"] = "1"
+        try:
+            with cueva.Solver(p) as s:
+                s.upload(prims)
+                s.step(steps - 1)
+                s.step(1)  # a call of one step: no stage-0 state formed ahead
+                c = s.step_counters()
+                out[march] = (interior(*s.download(), p), c)
+        finally:
+            os.environ.pop("CUEVA_MARCH", None)
+            os.environ.pop("CUEVA_C2P_IN_MARCH", None)
+    assert out[True][1] == out[False][1]
+    assert np.array_equal(out[True][0], out[False][0]), ndiff(out[True][0], out[False][0])
