@@ -3,10 +3,12 @@
 Workload: the 2048x4096 Kelvin-Helmholtz shear layer (periodic x, outflow
 y), sigma = 1e6, MP5 + HLLC, IMEX SSP2(2,2,2), CFL 0.4.  A step is one full
 time step of the whole grid; inputs are resident in HBM before timing
-starts.  Two forms are timed: plain FP64 (bitwise with the reference build)
-and the mixed form with the learned start of the primitive recovery; `value`
-is the latter, the plain one and the accuracy after 100 steps (relative L1
-of every field against plain FP64) are reported beside it.  One JSON line.
+starts.  Three forms are timed in the same run: plain FP64 (bitwise with the
+reference build; `value`), the mixed form with the Cardano recovery, and the
+mixed form with the learned start of the recovery; the mixed forms' speedup
+over plain FP64, their accuracy after 100 steps (relative L1 of every field
+against plain FP64, with the equal-L1 bar of 1e-4 on rho checked) and the
+KH growth rates are reported beside it.  One JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--form F] [--no-cpu]
   python bench.py --copy-calibration   # plain copy of known size, for rocprof counters
@@ -26,7 +28,11 @@ for _p in (PKG, ROOT):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector
+# VALU issue roof: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every
+# 4 cycles (16 lanes a cycle, FP64 and unpacked FP32 alike) at 2.4 GHz
+VALU_PEAK_INSTR = 1024 * 2.4e9 / 4
+EQUAL_L1_BAR = 1.0e-4      # relative L1 on rho after 100 steps, the mixed forms' accuracy bar
 METRIC = "zone-updates/s, 2048x4096 KH: plain FP64 vs mixed+learned-c2p at equal L1"
 
 # compulsory bytes of one SSP2(2,2,2) step per zone-update (stage k reads U^n,
@@ -51,10 +57,21 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
 
 
-def cpu_baseline(workload_cfg, budget_s=20.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(workload_cfg, budget_s=12.0):
     """Time the reference binary (oracle/_ref) on the host cores, on a bounded
     128x256 sample of the same problem (same scheme and sigma), by
-    differential wall clock; fall back to the oracle restatement."""
+    differential wall clock, with every core this job may use and with one
+    thread; fall back to the oracle restatement (one thread)."""
     import numpy as np
     import cueva
     from cueva import problems
@@ -66,7 +83,10 @@ def cpu_baseline(workload_cfg, budget_s=20.0):
     p, prims = cueva.kh_problem(sample_ni, sample_nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
                                 rotated=True, v0=v0, bx0=bx, bz0=bz)
     ncpu = len(os.sched_getaffinity(0))
-    threads = max(1, min(16, ncpu))
+    # every core this job may use: its affinity mask, within the host's
+    # per-GPU share that OMP_NUM_THREADS states where it is set
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(ncpu, share) if share > 0 else ncpu)
     if name and os.path.exists(exe):
         with tempfile.TemporaryDirectory() as tmp:
             inp = os.path.join(tmp, "in.bin")
@@ -79,16 +99,23 @@ def cpu_baseline(workload_cfg, budget_s=20.0):
                                capture_output=True)
                 return time.perf_counter() - t0
 
-            t1 = run(1, threads)
-            t2 = run(2, threads)
-            per = max(t2 - t1, 1e-3)
-            n = int(max(2, min(40, budget_s / per)))
-            tn = run(n + 2, threads)
-            rate = zones * n / max(tn - t2, 1e-9)
+            def rate_at(th):
+                t1 = run(1, th)
+                t2 = run(2, th)
+                per = max(t2 - t1, 1e-3)
+                n = int(max(2, min(60, budget_s / per)))
+                tn = run(n + 2, th)
+                return zones * n / max(tn - t2, 1e-9), n
+
+            rate, n = rate_at(threads)
+            rate1, n1 = rate_at(1)
             return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
+                    "cpu_model": cpu_model(), "nproc": os.cpu_count(), "affinity_cores": ncpu,
+                    "one_thread": {"value": rate1, "cores": 1, "steps": n1},
                     "sample": "reference (flang -O2 -fopenmp, %s) on the %dx%d version of the same KH problem "
-                              "(rotated frame), %d steps by differential wall clock T(%d)-T(2), "
-                              "OMP_NUM_THREADS=%d" % (name, sample_ni, sample_nj, n, n + 2, threads)}
+                              "(rotated frame), %d steps by differential wall clock T(%d)-T(2) with "
+                              "OMP_NUM_THREADS=%d (every core of this job), and %d steps with one thread"
+                              % (name, sample_ni, sample_nj, n, n + 2, threads, n1)}
     from oracle.binding import Oracle
     o = Oracle(p)
     o.upload(prims)
@@ -97,6 +124,7 @@ def cpu_baseline(workload_cfg, budget_s=20.0):
     o.step(2)
     dt = time.perf_counter() - t0
     return {"value": zones * 2 / dt, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": "oracle restatement, %dx%d, 2 steps, 1 thread" % (sample_ni, sample_nj)}
 
 
@@ -256,49 +284,81 @@ def main():
     if "mixed_learned" in forms and packed is None:
         sys.exit("no trained parameters (cueva/c2p_mlp.json) for the learned form")
     res = {f: time_form(cfg, f, args.steps, args.warmup, packed) for f in forms}
-    head = forms[-1]
+    head = forms[0]  # plain FP64 unless one form was asked for
     r = res[head]
-    p = r.pop("p")
-    for f in forms[:-1]:
+    p = r["p"]
+    for f in forms:
         res[f].pop("p")
-    kt = r["kernel_ms_per_step"]
-    dom = max(kt, key=lambda k: kt[k])
-    dom_avg_ms = kt[dom] / (r["kernel_launches"][dom] / args.steps)
-    cells = (p.imax + 1) * (p.jmax + 1)
+    zones = p.imax * p.jmax
     lmax = 2 if order == 2 else 4
-    # the face launches of one step: lmax - 1 stage launches and the last one
-    alg_bytes = cells * (MARCH_STAGE_BYTES * (lmax - 1) + march_last_bytes(lmax)) / lmax if dom == "faces" else None
-    achieved = alg_bytes / (dom_avg_ms / 1e3) / 1e9 if alg_bytes else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (dom, cfg, head))
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
-    form_name = {"plain_fp64": "plain FP64", "mixed": "mixed FP32/FP64, Cardano c2p",
-                 "mixed_learned": "mixed FP32/FP64 + learned c2p (BF16 MLP on MFMA)"}[head]
+
+    def rooflines(form):
+        """HBM and VALU rooflines of the form's face launches (its dominant
+        kernel class): compulsory bytes per launch = the step's 728 B per
+        zone-update spread over the lmax face launches of a step; VALU
+        instructions per launch from the committed rocprofv3 counter summary."""
+        fr = res[form]
+        kt = fr["kernel_ms_per_step"]
+        avg_ms = kt["faces"] / (fr["kernel_launches"]["faces"] / args.steps)
+        alg = STEP_BYTES_PER_ZONE * zones / lmax
+        achieved = alg / (avg_ms / 1e3) / 1e9
+        prof = os.path.join(ROOT, "profiles", "faces_%s_%s.json" % (cfg, form))
+        prof = json.load(open(prof)) if os.path.exists(prof) else {}
+        hbm = {"bound": "hbm", "kernel": "faces", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("hbm_bytes_per_launch"),
+               "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms}
+        valu = None
+        if prof.get("valu_instr_per_launch"):
+            rate = prof["valu_instr_per_launch"] / (avg_ms / 1e3)
+            valu = {"bound": "valu", "kernel": "faces", "achieved": rate, "peak": VALU_PEAK_INSTR,
+                    "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK_INSTR,
+                    "instr_per_launch": prof["valu_instr_per_launch"]}
+            if prof.get("fp64_flop_per_launch"):
+                tf = prof["fp64_flop_per_launch"] / (avg_ms / 1e3) / 1e12
+                valu["fp64_tflops"] = tf
+                valu["fp64_frac"] = tf / FP64_PEAK_TFLOPS
+        return hbm, valu
+
+    hbm, valu = rooflines(head)
+    wl = "%s: KH %dx%d, sigma=%g, MP%d + %s, IMEX %s, CFL 0.4" % (
+        cfg, ni, nj, sigma, 5 + 2 * rec, {5: "HLL", 6: "HLLE", 7: "HLLC"}[fs],
+        "SSP2(2,2,2)" if order == 2 else "SSP3(4,3,3)")
+    form_name = {"plain_fp64": "plain FP64", "mixed": "mixed FP32/FP64, Cardano recovery",
+                 "mixed_learned": "mixed FP32/FP64 + learned recovery start (BF16 MLP on MFMA)"}
     out = {
         "metric": METRIC, "value": r["value"], "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if head == "plain_fp64" else "f32+f64 (bf16 MLP)",
+        "vs_baseline": None, "dtype": "f64" if head == "plain_fp64" else "f32+f64",
         "data": "synthetic (seeded KH initial data)",
-        "config": {"workload": "%s: KH %dx%d, sigma=%g, MP%d + %s, IMEX %s, CFL 0.4" % (
-            cfg, ni, nj, sigma, 5 + 2 * rec, {5: "HLL", 6: "HLLE", 7: "HLLC"}[fs],
-            "SSP2(2,2,2)" if order == 2 else "SSP3(4,3,3)"), "form": form_name},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_avg_ms},
+        "config": {"workload": wl, "form": form_name[head]},
+        "roofline": hbm,
+        "valu_roofline": valu,
         "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_ZONE,
                           "achieved_GBs": r["value"] * STEP_BYTES_PER_ZONE / 1e9,
                           "frac": r["value"] * STEP_BYTES_PER_ZONE / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms_per_step": kt, "fixups": r["fixups"], "c2p_fallbacks": r["c2p_fallbacks"],
-        "forms": {f: {k: v for k, v in res[f].items() if k != "kernel_launches"} for f in forms},
+        "kernel_ms_per_step": r["kernel_ms_per_step"], "fixups": r["fixups"], "c2p_fallbacks": r["c2p_fallbacks"],
+        "forms": {},
     }
-    if "plain_fp64" in res and head != "plain_fp64":
-        out["speedup_vs_plain_fp64"] = r["value"] / res["plain_fp64"]["value"]
-        if not args.no_accuracy:
-            log("accuracy: %d steps of %s against plain FP64" % (ACCURACY_STEPS, head))
-            out["rel_l1_vs_plain_fp64_after_%d_steps" % ACCURACY_STEPS] = accuracy(cfg, head, packed)
-            log("growth: %d steps of plain FP64 and %s" % (GROWTH_STEPS, head))
-            out["kh_vy_growth"] = growth(cfg, head, packed)
+    for f in forms:
+        e = {k: v for k, v in res[f].items() if k != "kernel_launches"}
+        e["form"] = form_name[f]
+        if f != head:
+            e["speedup_vs_plain_fp64"] = res[f]["value"] / r["value"]
+            # all forms keep every plane in HBM in FP64 (the FP32 arithmetic
+            # stays on chip), so their compulsory bytes are the same 728 B
+            e["roofline"], e["valu_roofline"] = rooflines(f)
+        out["forms"][f] = e
+    if not args.no_accuracy and len(forms) > 1:
+        for f in forms[1:]:
+            log("accuracy: %d steps of %s against plain FP64" % (ACCURACY_STEPS, f))
+            acc = accuracy(cfg, f, packed)
+            over = sorted(k for k, v in acc.items() if v > EQUAL_L1_BAR)
+            out["forms"][f]["rel_l1_vs_plain_fp64_after_%d_steps" % ACCURACY_STEPS] = acc
+            out["forms"][f]["equal_l1"] = {"bar": EQUAL_L1_BAR, "field": "rho", "rho": acc["rho"],
+                                           "passed": acc["rho"] <= EQUAL_L1_BAR, "fields_over_bar": over}
+        last = forms[-1]
+        log("growth: %d steps of plain FP64 and %s" % (GROWTH_STEPS, last))
+        out["forms"][last]["kh_vy_growth"] = growth(cfg, last, packed)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(cfg)
