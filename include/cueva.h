@@ -102,7 +102,11 @@ typedef struct cueva_params {
   int32_t crmath;         /* 1: correctly rounded acos/cos/pow in the Cardano
                              step (double-double); 0: device math library      */
   int32_t primitive_rec;  /* 1: recons_var_primitive before each step (:26)    */
-  int32_t reserved[4];
+  int32_t slop;           /* slope limiter of the second-order face states
+                             (:17): 1 MinMod, 2 monotonised central, 3 Superbee;
+                             0 = 2                                               */
+  int32_t limit;          /* the same for recons_var_primitive (:18); 0 = 2    */
+  int32_t reserved[2];
   double longx, longy;    /* domain lengths Lx, Ly (:174-175)                   */
   double cfl;             /* CFL number (:309)                                  */
   double sigma;           /* conductivity sigma_0 (:196)                        */

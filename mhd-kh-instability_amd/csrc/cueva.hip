@@ -132,6 +132,8 @@ extern "C" int cueva_params_default(cueva_params* p) {
   p->c2p_mode = CUEVA_C2P_CARDANO;
   p->hllc_seed = 1;
   p->crmath = 1;
+  p->slop = 2;
+  p->limit = 2;
   p->longx = 1.0;
   p->longy = 2.0;
   p->cfl = 0.4;
@@ -199,6 +201,7 @@ static int check_params(const cueva_params* p) {
     return CUEVA_E_ARG;
   if (p->rec_mp < 0 || p->rec_mp > 2) return CUEVA_E_ARG;
   if (p->primitive_rec != 0 && p->primitive_rec != 1) return CUEVA_E_ARG;
+  if (p->slop < 0 || p->slop > 3 || p->limit < 0 || p->limit > 3) return CUEVA_E_ARG;
   if ((p->bc_x != 0 && p->bc_x != 1) || (p->bc_y != 0 && p->bc_y != 1)) return CUEVA_E_ARG;
   if (p->precision != CUEVA_FP64 && p->precision != CUEVA_MIXED) return CUEVA_E_ARG;
   if (p->c2p_mode != CUEVA_C2P_CARDANO && p->c2p_mode != CUEVA_C2P_LEARNED) return CUEVA_E_ARG;
@@ -235,6 +238,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   S.order = p->order;
   S.hllc_seed_ref = p->hllc_seed;
   S.crmath = p->crmath;
+  S.slop = p->slop ? p->slop : 2;
   imex_tables(p->order, S);
   c->rec_order = 5 + 2 * p->rec_mp;
   {
@@ -479,7 +483,8 @@ extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
 }
 
 static int recons_prims(cueva_ctx* c) {
-  LAUNCH(K_OTHER, k_recons_prim, interior_grid(c->g, BLK), BLK, c->g, c->P, sref(c, 0));
+  LAUNCH(K_OTHER, k_recons_prim, interior_grid(c->g, BLK), BLK, c->g, c->P, sref(c, 0),
+         c->prm.limit ? c->prm.limit : 2);
   if (launch_ok()) return CUEVA_E_HIP;
   LAUNCH(K_OTHER, k_recons_copy, interior_grid(c->g, BLK), BLK, c->g, c->P, sref(c, 0));
   return launch_ok();

@@ -60,6 +60,7 @@ struct Scheme {
   double wb[MAXL], wbt[MAXL], Ab[MAXL][MAXL], Abt[MAXL][MAXL];
   int hllc_seed_ref;
   int crmath;
+  int slop;   // slope limiter of the second-order reconstructions (parameters.f95:17)
   // the reference's module variable eps1 that clamps a superluminal previous
   // velocity in the recovery (12_varprimitive.f95:236-241): 1e-6 once
   // hll_flow_mpx has run in the process (16_hll_flow_mpx.f95:1375), else 0

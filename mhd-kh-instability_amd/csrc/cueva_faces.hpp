@@ -91,20 +91,26 @@ __device__ inline void face_fields(const Planes& P, int64_t c, int64_t st, Side<
 // parameters.f95:274).  The x face uses only x-reconstructed states and the
 // y face only y-reconstructed ones.
 // ---------------------------------------------------------------------------
-// mcl(fa, fb) with slop = 2 (funciones.f95:57-95, parameters.f95:17)
-template <typename T> __device__ inline T mcl_slope(T fa, T fb) {
-  return (T(0.5) * (csign(fa) + csign(fb))) * mn(mn(T(2) * fab(fa), T(2) * fab(fb)), T(0.5) * fab(fa + fb));
+// mcl(fa, fb) of funciones.f95:57-95 (kind = slop, parameters.f95:17) and
+// limiter(ga, gb) of funciones.f95:100-142 (kind = limit, :18): 1 MinMod,
+// 2 monotonised central, 3 Superbee; the three-argument min as flang
+// evaluates it, min(min(a, b), c)
+template <typename T> __device__ inline T mcl_slope(T fa, T fb, int kind = 2) {
+  const T s = T(0.5) * (csign(fa) + csign(fb));
+  if (kind == 1) return s * mn(fab(fa), fab(fb));
+  if (kind == 3) return s * mn(mn(T(2) * fab(fa), T(2) * fab(fb)), mx(fab(fa), fab(fb)));
+  return s * mn(mn(T(2) * fab(fa), T(2) * fab(fb)), T(0.5) * fab(fa + fb));
 }
 
 // left and right face values between the cells at c and c+st
 // (u_L = u_i + FAC mcl(.), u_R = u_{i+1} - FAC mcl(.), FAC = 0.5); the
 // half-slopes are returned too, for the "_m" states
 template <typename T>
-__device__ inline void mcl_rec(const double* __restrict__ a, int64_t c, int64_t st, T& l, T& r, T* u01 = nullptr,
+__device__ inline void mcl_rec(int slop, const double* __restrict__ a, int64_t c, int64_t st, T& l, T& r, T* u01 = nullptr,
                                T* hs = nullptr) {
   const T um = (T)a[c - st], u0 = (T)a[c], u1 = (T)a[c + st], u2 = (T)a[c + 2 * st];
-  const T sl = T(0.5) * mcl_slope(u1 - u0, u0 - um);
-  const T sr = T(0.5) * mcl_slope(u2 - u1, u1 - u0);
+  const T sl = T(0.5) * mcl_slope(u1 - u0, u0 - um, slop);
+  const T sr = T(0.5) * mcl_slope(u2 - u1, u1 - u0, slop);
   l = u0 + sl;
   r = u1 - sr;
   if (u01) { u01[0] = u0; u01[1] = u1; hs[0] = sl; hs[1] = sr; }
@@ -150,25 +156,25 @@ template <int DIR, typename T>
 __device__ inline void mcl_face(const Planes& P, const Scheme& S, int64_t c, int64_t st, T* f) {
   McSide<T> L, R;
   T e0[2], es[2], b0[2], bs[2];  // cell values and half-slopes of the "_m" pair
-  mcl_rec(P.s[S_PSI], c, st, L.psi, R.psi);
-  mcl_rec(P.s[S_PHI], c, st, L.phi, R.phi);
-  mcl_rec(P.s[S_EX], c, st, L.ex, R.ex, DIR == 1 ? e0 : (T*)nullptr, DIR == 1 ? es : (T*)nullptr);
-  mcl_rec(P.s[S_EY], c, st, L.ey, R.ey);
-  mcl_rec(P.s[S_EZ], c, st, L.ez, R.ez, DIR == 0 ? e0 : (T*)nullptr, DIR == 0 ? es : (T*)nullptr);
-  mcl_rec(P.s[S_BX], c, st, L.bx, R.bx);
-  mcl_rec(P.s[S_BY], c, st, L.by, R.by, DIR == 0 ? b0 : (T*)nullptr, DIR == 0 ? bs : (T*)nullptr);
-  mcl_rec(P.s[S_BZ], c, st, L.bz, R.bz, DIR == 1 ? b0 : (T*)nullptr, DIR == 1 ? bs : (T*)nullptr);
-  mcl_rec(P.s[S_Q], c, st, L.q, R.q);
-  mcl_rec(P.s[S_D], c, st, L.d, R.d);
-  mcl_rec(P.s[S_TAU], c, st, L.tau, R.tau);
-  mcl_rec(P.s[S_SX], c, st, L.sx, R.sx);
-  mcl_rec(P.s[S_SY], c, st, L.sy, R.sy);
-  mcl_rec(P.s[S_SZ], c, st, L.sz, R.sz);
-  mcl_rec(P.p[P_VX], c, st, L.vx, R.vx);
-  mcl_rec(P.p[P_VY], c, st, L.vy, R.vy);
-  mcl_rec(P.p[P_VZ], c, st, L.vz, R.vz);
-  mcl_rec(P.p[P_P], c, st, L.p, R.p);
-  mcl_rec(P.p[P_RHO], c, st, L.rho, R.rho);
+  mcl_rec(S.slop, P.s[S_PSI], c, st, L.psi, R.psi);
+  mcl_rec(S.slop, P.s[S_PHI], c, st, L.phi, R.phi);
+  mcl_rec(S.slop, P.s[S_EX], c, st, L.ex, R.ex, DIR == 1 ? e0 : (T*)nullptr, DIR == 1 ? es : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_EY], c, st, L.ey, R.ey);
+  mcl_rec(S.slop, P.s[S_EZ], c, st, L.ez, R.ez, DIR == 0 ? e0 : (T*)nullptr, DIR == 0 ? es : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_BX], c, st, L.bx, R.bx);
+  mcl_rec(S.slop, P.s[S_BY], c, st, L.by, R.by, DIR == 0 ? b0 : (T*)nullptr, DIR == 0 ? bs : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_BZ], c, st, L.bz, R.bz, DIR == 1 ? b0 : (T*)nullptr, DIR == 1 ? bs : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_Q], c, st, L.q, R.q);
+  mcl_rec(S.slop, P.s[S_D], c, st, L.d, R.d);
+  mcl_rec(S.slop, P.s[S_TAU], c, st, L.tau, R.tau);
+  mcl_rec(S.slop, P.s[S_SX], c, st, L.sx, R.sx);
+  mcl_rec(S.slop, P.s[S_SY], c, st, L.sy, R.sy);
+  mcl_rec(S.slop, P.s[S_SZ], c, st, L.sz, R.sz);
+  mcl_rec(S.slop, P.p[P_VX], c, st, L.vx, R.vx);
+  mcl_rec(S.slop, P.p[P_VY], c, st, L.vy, R.vy);
+  mcl_rec(S.slop, P.p[P_VZ], c, st, L.vz, R.vz);
+  mcl_rec(S.slop, P.p[P_P], c, st, L.p, R.p);
+  mcl_rec(S.slop, P.p[P_RHO], c, st, L.rho, R.rho);
   // "_m" states: the cell value negated, the slope kept
   // (16_hll_flow_mcl.f95:634-655, 708-737)
   const T em_l = (-e0[0]) + es[0], em_r = (-e0[1]) - es[1];
@@ -216,25 +222,25 @@ template <int DIR, typename T>
 __device__ inline void mcl_sides(const Planes& P, const Scheme& S, int64_t c, int64_t st, Side<T>& L, Side<T>& R) {
   McSide<T> l, r;
   T e0[2], es[2], b0[2], bs[2];
-  mcl_rec(P.s[S_PSI], c, st, l.psi, r.psi);
-  mcl_rec(P.s[S_PHI], c, st, l.phi, r.phi);
-  mcl_rec(P.s[S_EX], c, st, l.ex, r.ex, DIR == 1 ? e0 : (T*)nullptr, DIR == 1 ? es : (T*)nullptr);
-  mcl_rec(P.s[S_EY], c, st, l.ey, r.ey);
-  mcl_rec(P.s[S_EZ], c, st, l.ez, r.ez, DIR == 0 ? e0 : (T*)nullptr, DIR == 0 ? es : (T*)nullptr);
-  mcl_rec(P.s[S_BX], c, st, l.bx, r.bx);
-  mcl_rec(P.s[S_BY], c, st, l.by, r.by, DIR == 0 ? b0 : (T*)nullptr, DIR == 0 ? bs : (T*)nullptr);
-  mcl_rec(P.s[S_BZ], c, st, l.bz, r.bz, DIR == 1 ? b0 : (T*)nullptr, DIR == 1 ? bs : (T*)nullptr);
-  mcl_rec(P.s[S_Q], c, st, l.q, r.q);
-  mcl_rec(P.s[S_D], c, st, l.d, r.d);
-  mcl_rec(P.s[S_TAU], c, st, l.tau, r.tau);
-  mcl_rec(P.s[S_SX], c, st, l.sx, r.sx);
-  mcl_rec(P.s[S_SY], c, st, l.sy, r.sy);
-  mcl_rec(P.s[S_SZ], c, st, l.sz, r.sz);
-  mcl_rec(P.p[P_VX], c, st, l.vx, r.vx);
-  mcl_rec(P.p[P_VY], c, st, l.vy, r.vy);
-  mcl_rec(P.p[P_VZ], c, st, l.vz, r.vz);
-  mcl_rec(P.p[P_P], c, st, l.p, r.p);
-  mcl_rec(P.p[P_RHO], c, st, l.rho, r.rho);
+  mcl_rec(S.slop, P.s[S_PSI], c, st, l.psi, r.psi);
+  mcl_rec(S.slop, P.s[S_PHI], c, st, l.phi, r.phi);
+  mcl_rec(S.slop, P.s[S_EX], c, st, l.ex, r.ex, DIR == 1 ? e0 : (T*)nullptr, DIR == 1 ? es : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_EY], c, st, l.ey, r.ey);
+  mcl_rec(S.slop, P.s[S_EZ], c, st, l.ez, r.ez, DIR == 0 ? e0 : (T*)nullptr, DIR == 0 ? es : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_BX], c, st, l.bx, r.bx);
+  mcl_rec(S.slop, P.s[S_BY], c, st, l.by, r.by, DIR == 0 ? b0 : (T*)nullptr, DIR == 0 ? bs : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_BZ], c, st, l.bz, r.bz, DIR == 1 ? b0 : (T*)nullptr, DIR == 1 ? bs : (T*)nullptr);
+  mcl_rec(S.slop, P.s[S_Q], c, st, l.q, r.q);
+  mcl_rec(S.slop, P.s[S_D], c, st, l.d, r.d);
+  mcl_rec(S.slop, P.s[S_TAU], c, st, l.tau, r.tau);
+  mcl_rec(S.slop, P.s[S_SX], c, st, l.sx, r.sx);
+  mcl_rec(S.slop, P.s[S_SY], c, st, l.sy, r.sy);
+  mcl_rec(S.slop, P.s[S_SZ], c, st, l.sz, r.sz);
+  mcl_rec(S.slop, P.p[P_VX], c, st, l.vx, r.vx);
+  mcl_rec(S.slop, P.p[P_VY], c, st, l.vy, r.vy);
+  mcl_rec(S.slop, P.p[P_VZ], c, st, l.vz, r.vz);
+  mcl_rec(S.slop, P.p[P_P], c, st, l.p, r.p);
+  mcl_rec(S.slop, P.p[P_RHO], c, st, l.rho, r.rho);
   Side<T>* sd[2] = {&L, &R};
   const McSide<T>* ms[2] = {&l, &r};
 #pragma unroll
@@ -301,11 +307,11 @@ __device__ inline void lax_cell_fluxes(const Geom& g, const Planes& P, const Sch
 // the face value from the split flux of cells i-1 .. i+2 (a: the flux, u:
 // the field it is split with, sa = -1 for the fluxes the reference splits
 // as -X +- U and negates afterwards)
-template <typename T> __device__ inline T lax_face_value(const T* a, const T* u, T sa = T(1)) {
+template <typename T> __device__ inline T lax_face_value(int slop, const T* a, const T* u, T sa = T(1)) {
   const T p0 = sa * a[0] + u[0], p1 = sa * a[1] + u[1], p2 = sa * a[2] + u[2], p3 = sa * a[3] + u[3];
   const T m2 = sa * a[2] - u[2];
-  const T l = p1 + T(0.5) * mcl_slope(p2 - p1, p1 - p0);
-  const T r = m2 - T(0.5) * mcl_slope(p3 - p2, p2 - p1);
+  const T l = p1 + T(0.5) * mcl_slope(p2 - p1, p1 - p0, slop);
+  const T r = m2 - T(0.5) * mcl_slope(p3 - p2, p2 - p1, slop);
   return T(0.5) * (l + r);
 }
 
@@ -331,23 +337,23 @@ __device__ inline void lax_face(const Geom& g, const Planes& P, const Scheme& S,
   lax_ld(P.s[S_EX], k, ex); lax_ld(P.s[S_EY], k, ey); lax_ld(P.s[S_EZ], k, ez);
   lax_ld(P.s[S_BX], k, bx); lax_ld(P.s[S_BY], k, by); lax_ld(P.s[S_BZ], k, bz);
   if constexpr (DIR == 0) {  // 15_lax_flow.f95:930-1180
-    f[FX_EXPSI] = lax_face_value(ex, psi);
-    f[FX_BXPHI] = lax_face_value(bx, phi);
-    f[FX_PHIBX] = lax_face_value(phi, bx);
-    f[FX_EZBY] = -lax_face_value(ez, by, T(-1));
-    f[FX_EYBZ] = lax_face_value(ey, bz);
-    f[FX_PSIEX] = lax_face_value(psi, ex);
-    f[FX_BZEY] = lax_face_value(bz, ey);
-    f[FX_BYEZ] = -lax_face_value(by, ez, T(-1));
+    f[FX_EXPSI] = lax_face_value(S.slop, ex, psi);
+    f[FX_BXPHI] = lax_face_value(S.slop, bx, phi);
+    f[FX_PHIBX] = lax_face_value(S.slop, phi, bx);
+    f[FX_EZBY] = -lax_face_value(S.slop, ez, by, T(-1));
+    f[FX_EYBZ] = lax_face_value(S.slop, ey, bz);
+    f[FX_PSIEX] = lax_face_value(S.slop, psi, ex);
+    f[FX_BZEY] = lax_face_value(S.slop, bz, ey);
+    f[FX_BYEZ] = -lax_face_value(S.slop, by, ez, T(-1));
   } else {
-    f[FY_EYPSI] = lax_face_value(ey, psi);
-    f[FY_BYPHI] = lax_face_value(by, phi);
-    f[FY_EZBX] = lax_face_value(ez, bx);
-    f[FY_PHIBY] = lax_face_value(phi, by);
-    f[FY_EXBZ] = -lax_face_value(ex, bz, T(-1));
-    f[FY_BZEX] = -lax_face_value(bz, ex, T(-1));
-    f[FY_PSIEY] = lax_face_value(psi, ey);
-    f[FY_BXEZ] = lax_face_value(bx, ez);
+    f[FY_EYPSI] = lax_face_value(S.slop, ey, psi);
+    f[FY_BYPHI] = lax_face_value(S.slop, by, phi);
+    f[FY_EZBX] = lax_face_value(S.slop, ez, bx);
+    f[FY_PHIBY] = lax_face_value(S.slop, phi, by);
+    f[FY_EXBZ] = -lax_face_value(S.slop, ex, bz, T(-1));
+    f[FY_BZEX] = -lax_face_value(S.slop, bz, ex, T(-1));
+    f[FY_PSIEY] = lax_face_value(S.slop, psi, ey);
+    f[FY_BXEZ] = lax_face_value(S.slop, bx, ez);
   }
   static_assert((int)FX_J == (int)FY_J && (int)FX_FSZ == (int)FY_FSZ, "fluid slots at the same place");
   const int us[6] = {S_Q, S_D, S_TAU, S_SX, S_SY, S_SZ};
@@ -357,7 +363,7 @@ __device__ inline void lax_face(const Geom& g, const Planes& P, const Scheme& S,
 #pragma unroll
     for (int c = 0; c < 4; c++) a[c] = fl[c][t];
     lax_ld(P.s[us[t]], k, u);
-    f[FX_J + t] = lax_face_value(a, u);
+    f[FX_J + t] = lax_face_value(S.slop, a, u);
   }
 }
 

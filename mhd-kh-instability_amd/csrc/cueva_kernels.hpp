@@ -201,7 +201,7 @@ __device__ inline Rhs flux_source(const Geom& g, const Planes& P, const Scheme& 
 // differences u(i+1,j+1) - u and u - u(i-1,j-1).  The 14 results go to the
 // x-face planes (dead between steps); k_recons_copy writes them back, so
 // every read sees the old values.
-__global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P, StatusRef sr) {
+__global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P, StatusRef sr, int limit) {
   if (halted(sr)) return;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i > g.imax || j > g.jmax) return;
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) k_recons_prim(Geom g, Planes P, StatusRef
   for (int k = 0; k < P_N; k++) {
     const double* a = P.p[k];
     const double u = a[c];
-    const double lim = mcl_slope(a[c + dd] - u, u - a[c - dd]);
+    const double lim = mcl_slope(a[c + dd] - u, u - a[c - dd], limit);
     const double r = u - 0.5 * lim, l = u + 0.5 * lim;
     P.fx[k][c] = 0.5 * (r + l);
   }

@@ -54,7 +54,8 @@ class Params(ctypes.Structure):
         ("flux_solver", ctypes.c_int32), ("rec_mp", ctypes.c_int32),
         ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("c2p_mode", ctypes.c_int32), ("hllc_seed", ctypes.c_int32), ("crmath", ctypes.c_int32),
-        ("primitive_rec", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4),
+        ("primitive_rec", ctypes.c_int32), ("slop", ctypes.c_int32), ("limit", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
         ("longx", ctypes.c_double), ("longy", ctypes.c_double), ("cfl", ctypes.c_double),
         ("sigma", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
         ("reserved_d", ctypes.c_double * 6),

@@ -21,7 +21,8 @@ module cueva_gpu
     integer(c_int32_t) :: imax, jmax, order, flux_solver, rec_mp, bc_x, bc_y
     integer(c_int32_t) :: precision, c2p_mode, hllc_seed, crmath
     integer(c_int32_t) :: primitive_rec
-    integer(c_int32_t) :: reserved(4)
+    integer(c_int32_t) :: slop, limit
+    integer(c_int32_t) :: reserved(2)
     real(c_double) :: longx, longy, cfl, sigma, gamma, dt
     real(c_double) :: reserved_d(6)
   end type cueva_params

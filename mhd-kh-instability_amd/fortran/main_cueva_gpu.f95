@@ -2,7 +2,8 @@
 ! driving the gfx950 library through the reference-named shims.
 !
 !   main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED
-!                  V0 BX0 BZ0 SEED HGLB OUTPREFIX [HLOC [RESTART_IN [HINI]]]
+!                  V0 BX0 BZ0 SEED HGLB OUTPREFIX [HLOC [RESTART_IN [HINI
+!                  [PRIMITIVE_REC [SLOP [LIMIT]]]]]]
 !
 ! IMAX x JMAX is the native KH grid (x periodic, y outflow); ROTATED = 1 runs
 ! it in the frame x' = y, y' = -x (the reference's BOUND 6).  Writes
@@ -24,6 +25,8 @@
 !   V0 BX0 BZ0 SEED are then unused.
 ! HINI: number of the first step, as the reference's hini (default 1): steps
 !   HINI .. HINI+NSTEPS-1 run, from t = (HINI-1) Delt.
+! PRIMITIVE_REC, SLOP, LIMIT: the reference's primitive_rec, slop and limit
+!   (parameters.f95:17-18,26; defaults 0, 2, 2).
 program main_cueva_gpu
   use iso_c_binding
   use cueva_gpu
@@ -31,7 +34,7 @@ program main_cueva_gpu
   use cueva_shims
   use cueva_io
   implicit none
-  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini, primrec
+  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini, primrec, slop, limit
   integer(c_int64_t) :: seed
   real(c_double) :: sigma, v0, bx0, bz0, dv(5), g(37)
   character(len=512) :: arg, prefix, restart_in
@@ -40,9 +43,9 @@ program main_cueva_gpu
   real(c_double), allocatable :: prims(:), cons(:)
 
   nargs = command_argument_count()
-  if (nargs < 14 .or. nargs > 18) then
+  if (nargs < 14 .or. nargs > 20) then
     write(*,*) "usage: main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED V0 BX0 BZ0 SEED HGLB ", &
-               "OUTPREFIX [HLOC [RESTART_IN [HINI [PRIMITIVE_REC]]]]"
+               "OUTPREFIX [HLOC [RESTART_IN [HINI [PRIMITIVE_REC [SLOP [LIMIT]]]]]]"
     stop 2
   end if
   call get_command_argument(1, arg); read(arg, *) ni
@@ -73,6 +76,14 @@ program main_cueva_gpu
   if (nargs >= 18) then
     call get_command_argument(18, arg); read(arg, *) primrec
   end if
+  slop = 2
+  limit = 2
+  if (nargs >= 19) then
+    call get_command_argument(19, arg); read(arg, *) slop
+  end if
+  if (nargs >= 20) then
+    call get_command_argument(20, arg); read(arg, *) limit
+  end if
 
   call check(cueva_params_default(prm), "params_default")
   if (rotated == 1) then
@@ -85,6 +96,7 @@ program main_cueva_gpu
     prm%longx = 1.d0; prm%longy = 2.d0
   end if
   prm%flux_solver = fs; prm%rec_mp = rec; prm%order = order; prm%primitive_rec = primrec
+  prm%slop = slop; prm%limit = limit
   prm%sigma = sigma; prm%cfl = 0.4d0
   lmax = merge(2, 4, order == 2)
   hmax = nsteps

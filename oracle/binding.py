@@ -20,7 +20,8 @@ class Cfg(ctypes.Structure):
                 ("kappa_psi", ctypes.c_double), ("kappa_phi", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("wb", ctypes.c_double * 8), ("wbt", ctypes.c_double * 8), ("Ab", ctypes.c_double * 64),
                 ("Abt", ctypes.c_double * 64), ("primitive_rec", ctypes.c_int32), ("hllc_seed", ctypes.c_int32),
-                ("libm_mode", ctypes.c_int32), ("pad1", ctypes.c_int32)]
+                ("libm_mode", ctypes.c_int32), ("slop", ctypes.c_int32), ("limit", ctypes.c_int32),
+                ("pad1", ctypes.c_int32)]
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -83,6 +84,8 @@ class Oracle:
             raise ValueError("oracle config rejected (%d)" % rc)
         self.c.hllc_seed = int(params.hllc_seed)
         self.c.primitive_rec = int(params.primitive_rec)
+        self.c.slop = int(getattr(params, "slop", 2) or 2)
+        self.c.limit = int(getattr(params, "limit", 2) or 2)
         if libm is None:
             libm = "cr" if params.crmath else "c"
         self.c.libm_mode = 1 if libm == "cr" else 0

@@ -138,6 +138,8 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   c->primitive_rec = 0;
   c->hllc_seed = 1;
   c->libm_mode = 0;
+  c->slop = 2;
+  c->limit = 2;
   /* parameters.f95:325-334: kappa = alpha_ch**2 / cp**2, cp = sqrt(min(Delx,Dely)/alpha) */
   double cp = sqrt(fmin2(c->dx, c->dy) / 1.0);
   c->kappa_psi = 1.0 / (cp * cp);
@@ -1327,11 +1329,17 @@ static int face_loop(oracle_state *s) {
 /* Face states of the stage fields and of the t^n velocity, pressure    */
 /* and density; Sp = 1 (const_sp = 0, parameters.f95:274).               */
 /* ------------------------------------------------------------------ */
-/* funciones.f95:57-95 with slop = 2 (parameters.f95:17) */
-static inline double mcl(double fa, double fb) {
-  return (0.5 * (copysign(1.0, fa) + copysign(1.0, fb))) *
-         fmin2(fmin2(2.0 * fabs(fa), 2.0 * fabs(fb)), 0.5 * fabs(fa + fb));
+/* mcl(fa, fb), funciones.f95:57-95, and limiter(ga, gb), :100-142: kind 1
+ * MinMod, 2 monotonised central, 3 Superbee (slop / limit,
+ * parameters.f95:17-18); min(a, b, c) evaluated as min(min(a, b), c) */
+static int g_slop = 2, g_limit = 2;
+static inline double limiter_of(int kind, double fa, double fb) {
+  double s = 0.5 * (copysign(1.0, fa) + copysign(1.0, fb));
+  if (kind == 1) return s * fmin2(fabs(fa), fabs(fb));
+  if (kind == 3) return s * fmin2(fmin2(2.0 * fabs(fa), 2.0 * fabs(fb)), fmax2(fabs(fa), fabs(fb)));
+  return s * fmin2(fmin2(2.0 * fabs(fa), 2.0 * fabs(fb)), 0.5 * fabs(fa + fb));
 }
+static inline double mcl(double fa, double fb) { return limiter_of(g_slop, fa, fb); }
 
 typedef struct {
   double psi, phi, ex, ey, ez, bx, by, bz, q, d, tau, sx, sy, sz, vx, vy, vz, p, rho;
@@ -1356,7 +1364,7 @@ static int recons_prims(oracle_state *s) {
       for (int i = 0; i <= s->c.imax; i++) {
         size_t c = IX(s, i, j);
         double u = a[c];
-        double lim = mcl(a[c + dd] - u, u - a[c - dd]);
+        double lim = limiter_of(g_limit, a[c + dd] - u, u - a[c - dd]);
         double r = u - 0.5 * lim, l = u + 0.5 * lim;
         tmp[c] = 0.5 * (r + l);
       }
@@ -2126,6 +2134,8 @@ void oracle_last_stop(const oracle_state *s, int32_t out[4]) {
 static void install_modes(const oracle_state *s) {
   g_libm_mode = s->c.libm_mode;
   g_hllc_seed_mode = s->c.hllc_seed;
+  g_slop = s->c.slop ? s->c.slop : 2;
+  g_limit = s->c.limit ? s->c.limit : 2;
 }
 
 int oracle_step(oracle_state *s, int nsteps) {

@@ -46,6 +46,8 @@ typedef struct {
   int32_t primitive_rec;   /* 1: recons_var_primitive before each step (parameters.f95:26) */
   int32_t hllc_seed;       /* 1: first contact-speed Newton point as the reference build; 0: clamped seed */
   int32_t libm_mode;       /* 0: acos/cos/pow of the C library (as the reference build); 1: correctly rounded */
+  int32_t slop;            /* second-order slope limiter (parameters.f95:17): 1 MinMod, 2 MC, 3 Superbee */
+  int32_t limit;           /* the same for recons_var_primitive (parameters.f95:18) */
   int32_t pad1;
 } oracle_cfg;
 

@@ -53,6 +53,8 @@ sed -E \
   -e "s/^( *doubleprecision,parameter +:: +sigma_0 +=) *[0-9.d]+/\1 $SIGMA/" \
   -e "s/^( *doubleprecision,parameter +:: +CFL +=) *[0-9.d]+/\1 $CFL/" \
   -e "s/^( *integer,parameter +:: +primitive_rec +=) *[0-9]+/\1 ${PRIMITIVE_REC:-0}/" \
+  -e "s/^( *integer,parameter +:: +slop +=) *[0-9]+/\1 ${SLOP:-2}/" \
+  -e "s/^( *integer,parameter +:: +limit +=) *[0-9]+/\1 ${LIMIT:-2}/" \
   "$REF/parameters.f95" > "$WORK/parameters.f95"
 
 # Every substitution must have hit exactly its line.
@@ -68,6 +70,8 @@ check "Ly +=  *$LY( |$)"
 check "sigma_0 +=  *$SIGMA( |$)"
 check "CFL +=  *$CFL( |$)"
 check "primitive_rec +=  *${PRIMITIVE_REC:-0}( |$)"
+check "slop +=  *${SLOP:-2}( |$)"
+check "limit +=  *${LIMIT:-2}( |$)"
 
 cd "$WORK"
 UNITS="scalar threevectors fourvectors funciones imex_scheme mp5 mp7 mp9

@@ -64,6 +64,15 @@ CASES = {
     "kh_hll_lowp_32x64": (32, 64, 5, 0, 2, 1.0e6, 4, [1, 4], [1, 4], None, {"crafted": "lowp"}),
     "kh_hllc_lowp_32x64": (32, 64, 7, 0, 2, 1.0e6, 4, [1, 4], [1, 4], None, {"crafted": "lowp"}),
     "kh_hllc_negrho_32x64": (32, 64, 7, 0, 2, 1.0e6, 3, [1, 3], [1, 3], None, {"crafted": "negrho"}),
+    # the MinMod and Superbee slope limiters (slop, limit = 1, 3; funciones.f95:57-142)
+    "kh_hll_mcl_minmod_32x64": (32, 64, 2, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None, {"slop": 1}),
+    "kh_hllc_mcl_superbee_32x64": (32, 64, 4, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"slop": 3}),
+    "kh_lax_superbee_bz_32x64": (32, 64, 1, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"slop": 3, "bz0": 0.1}),
+    "kh_hlle_mcl_minmod_bz_32x64": (32, 64, 3, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"slop": 1, "bz0": 0.1}),
+    "kh_hll_mcl_primrec_minmod_32x64": (32, 64, 2, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None,
+                                        {"primitive_rec": 1, "limit": 1}),
+    "kh_hllc_mp5_primrec_superbee_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None,
+                                           {"primitive_rec": 1, "limit": 3}),
 }
 SEED = 20261202
 
@@ -87,7 +96,8 @@ def main(only=None):
             subprocess.run([os.path.join(ROOT, "oracle", "refbuild", "build_ref.sh"), name, str(imax), str(jmax),
                             str(order), str(fs), str(rec), "6", "%.1fd0" % sigma if sigma < 1e4 else "1.d6",
                             "0.4d0", "2.d0", "1.d0"], check=True,
-                           env=dict(os.environ, PRIMITIVE_REC=str(init_kw.get("primitive_rec", 0))))
+                           env=dict(os.environ, PRIMITIVE_REC=str(init_kw.get("primitive_rec", 0)),
+                                    SLOP=str(init_kw.get("slop", 2)), LIMIT=str(init_kw.get("limit", 2))))
         p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
                                     rotated=True, seed=SEED, **init_kw)
         if mod:

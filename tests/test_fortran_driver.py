@@ -28,9 +28,11 @@ def test_driver_built_and_checks_arguments():
 @pytest.mark.parametrize("fs,rec,order,kw", [(cueva.HLLC, 0, 2, {}), (cueva.HLLE, 1, 2, {"bz0": 0.1}),
                                              (cueva.HLL, 2, 4, {}), (cueva.HLL_MCL, 0, 2, {"bz0": 0.1}),
                                              (cueva.LAX, 0, 2, {"bz0": 0.1}), (cueva.HLLC_MCL, 0, 2, {"bz0": 0.1}),
-                                             (cueva.HLLE_MCL, 0, 2, {"bz0": 0.1}), (cueva.HLLC, 0, 2, {"primitive_rec": 1})],
+                                             (cueva.HLLE_MCL, 0, 2, {"bz0": 0.1}), (cueva.HLLC, 0, 2, {"primitive_rec": 1}),
+                                             (cueva.HLLC_MCL, 0, 2, {"slop": 3}),
+                                             (cueva.HLL_MCL, 0, 2, {"primitive_rec": 1, "limit": 1, "slop": 1})],
                          ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl", "hlle_mcl",
-                              "hllc_mp5_primrec"])
+                              "hllc_mp5_primrec", "hllc_mcl_superbee", "hll_mcl_minmod_primrec_minmod"])
 def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built (needs flang)")
@@ -40,8 +42,9 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     print("driver", fs, rec, order, flush=True)
     args = [DRIVER, str(ni), str(nj), str(steps), str(fs), str(rec), str(order), repr(sigma), "1", "0.5", "0.01",
             repr(bz0), "20261202", "1", prefix]
-    if kw.get("primitive_rec"):
-        args += ["0", "-", "1", "1"]  # HLOC RESTART_IN HINI PRIMITIVE_REC
+    if kw.get("primitive_rec") or kw.get("slop") or kw.get("limit"):
+        # HLOC RESTART_IN HINI PRIMITIVE_REC SLOP LIMIT
+        args += ["0", "-", "1", str(kw.get("primitive_rec", 0)), str(kw.get("slop", 2)), str(kw.get("limit", 2))]
     res = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True, **kw)

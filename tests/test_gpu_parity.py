@@ -157,6 +157,10 @@ BITWISE_DUMPS = {
     "kh_hll_mcl_primrec_32x64": (1, 10), "kh_hllc_mp5_primrec_32x64": (1, 10),
     # crafted data through the pressure fix-up and the HLLC face-density fix
     "kh_hll_lowp_32x64": (1, 4), "kh_hllc_lowp_32x64": (1, 4), "kh_hllc_negrho_32x64": (1, 3),
+    # the MinMod and Superbee slope limiters
+    "kh_hll_mcl_minmod_32x64": (1, 10), "kh_hllc_mcl_superbee_32x64": (1, 10), "kh_lax_superbee_bz_32x64": (1, 10),
+    "kh_hlle_mcl_minmod_bz_32x64": (1, 10), "kh_hll_mcl_primrec_minmod_32x64": (1, 10),
+    "kh_hllc_mp5_primrec_superbee_32x64": (1, 10),
 }
 
 
