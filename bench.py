@@ -290,7 +290,7 @@ def main():
     for f in forms:
         res[f].pop("p")
     zones = p.imax * p.jmax
-    lmax = 2 if order == 2 else 4
+    lmax = order  # stages of the IMEX scheme
 
     def rooflines(form):
         """HBM and VALU rooflines of the form's face launches (its dominant
@@ -322,7 +322,7 @@ def main():
     hbm, valu = rooflines(head)
     wl = "%s: KH %dx%d, sigma=%g, MP%d + %s, IMEX %s, CFL 0.4" % (
         cfg, ni, nj, sigma, 5 + 2 * rec, {5: "HLL", 6: "HLLE", 7: "HLLC"}[fs],
-        "SSP2(2,2,2)" if order == 2 else "SSP3(4,3,3)")
+        {2: "SSP2(2,2,2)", 3: "SSP2(3,3,2)-LUM", 4: "SSP3(4,3,3)"}[order])
     form_name = {"plain_fp64": "plain FP64", "mixed": "mixed FP32/FP64, Cardano recovery",
                  "mixed_learned": "mixed FP32/FP64 + learned recovery start (BF16 MLP on MFMA)"}
     out = {

@@ -91,7 +91,8 @@ enum { CUEVA_BND_PRIM = 0, CUEVA_BND_CONS = 1, CUEVA_BND_ELEC = 2, CUEVA_BND_FLU
  * (parameters.f95:10-334).  Fill with cueva_params_default, then override. */
 typedef struct cueva_params {
   int32_t imax, jmax;     /* cells per axis (parameters.f95:110-111)           */
-  int32_t order;          /* 2 = IMEX SSP2(2,2,2), 4 = SSP3(4,3,3) (:10)        */
+  int32_t order;          /* 2 = IMEX SSP2(2,2,2), 3 = SSP2(3,3,2)-LUM,
+                             4 = SSP3(4,3,3) (:10); stages lmax = order        */
   int32_t flux_solver;    /* CUEVA_HLL/_HLLE/_HLLC/_HLL_MCL/_HLLE_MCL/_HLLC_MCL/_LAX (:21) */
   int32_t rec_mp;         /* 0 MP5, 1 MP7, 2 MP9 (:33)                          */
   int32_t bc_x, bc_y;     /* CUEVA_BC_* per axis                                */

@@ -162,7 +162,8 @@ extern "C" int cueva_derived(const cueva_params* p, double out[5]) {
   return CUEVA_OK;
 }
 
-// Butcher tables of imex_scheme.f95:40-49 (SSP2(2,2,2)) and :280-291 (SSP3(4,3,3))
+// Butcher tables of imex_scheme.f95:40-49 (SSP2(2,2,2)), :77-86 (SSP2(3,3,2)-LUM)
+// and :280-291 (SSP3(4,3,3))
 static void imex_tables(int order, Scheme& S) {
   memset(S.wb, 0, sizeof(S.wb));
   memset(S.wbt, 0, sizeof(S.wbt));
@@ -177,6 +178,14 @@ static void imex_tables(int order, Scheme& S) {
     S.Abt[0][0] = g;
     S.Abt[1][0] = 1.0 - 2.0 * g;
     S.Abt[1][1] = g;
+  } else if (order == 3) {
+    S.lmax = 3;
+    for (int l = 0; l < 3; l++) S.wb[l] = S.wbt[l] = 1.0 / 3.0;
+    S.Ab[1][0] = 0.5;
+    S.Ab[2][0] = 0.5; S.Ab[2][1] = 0.5;
+    S.Abt[0][0] = 1.0 / 5.0;
+    S.Abt[1][0] = 1.0 / 10.0; S.Abt[1][1] = 1.0 / 5.0;
+    S.Abt[2][0] = 1.0 / 3.0; S.Abt[2][1] = 1.0 / 3.0; S.Abt[2][2] = 1.0 / 3.0;
   } else {
     const double a1 = 0.24169426078821, b1 = 0.06042356519705, e1 = 0.12915286960590;  // :242-244
     S.lmax = 4;
@@ -194,7 +203,7 @@ static void imex_tables(int order, Scheme& S) {
 static int check_params(const cueva_params* p) {
   if (!p) return CUEVA_E_ARG;
   if (p->imax < 8 || p->jmax < 8 || p->imax > 65536 || p->jmax > 65536) return CUEVA_E_ARG;
-  if (p->order != 2 && p->order != 4) return CUEVA_E_ARG;
+  if (p->order < 2 || p->order > 4) return CUEVA_E_ARG;
   if (p->flux_solver != CUEVA_HLL && p->flux_solver != CUEVA_HLLC && p->flux_solver != CUEVA_HLLE &&
       p->flux_solver != CUEVA_HLL_MCL && p->flux_solver != CUEVA_LAX && p->flux_solver != CUEVA_HLLC_MCL &&
       p->flux_solver != CUEVA_HLLE_MCL)

@@ -398,7 +398,7 @@ class Solver:
         returns the fix-up counters of the primitive recovery."""
         L = self.lib
         x = self.ctx
-        lmax = 2 if self.p.order == 2 else 4
+        lmax = int(self.p.order)  # stages of SSP2(2,2,2), SSP2(3,3,2)-LUM, SSP3(4,3,3)
         check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
         if self.p.primitive_rec:  # main_cueva_code.f95:170-175
             check(L.cueva_recons_var_primitive(x), "recons_var_primitive")

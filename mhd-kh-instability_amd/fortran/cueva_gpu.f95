@@ -8,7 +8,7 @@ module cueva_gpu
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: CUEVA_OK = 0
+  integer(c_int), parameter :: CUEVA_OK = 0, CUEVA_E_ARG = 1
   integer(c_int), parameter :: CUEVA_BC_COPY = 0, CUEVA_BC_PERIODIC = 1
   integer(c_int), parameter :: CUEVA_LAX = 1, CUEVA_HLL_MCL = 2, CUEVA_HLLE_MCL = 3, CUEVA_HLLC_MCL = 4, &
                                CUEVA_HLL = 5, CUEVA_HLLE = 6, CUEVA_HLLC = 7

@@ -98,7 +98,7 @@ program main_cueva_gpu
   prm%flux_solver = fs; prm%rec_mp = rec; prm%order = order; prm%primitive_rec = primrec
   prm%slop = slop; prm%limit = limit
   prm%sigma = sigma; prm%cfl = 0.4d0
-  lmax = merge(2, 4, order == 2)
+  lmax = order   ! SSP2(2,2,2), SSP2(3,3,2)-LUM, SSP3(4,3,3)
   hmax = nsteps
   call check(cueva_derived(prm, dv), "derived")
   Delx = dv(1); Dely = dv(2); Delt = dv(3)

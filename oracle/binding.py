@@ -78,7 +78,7 @@ class Oracle:
         self.c = Cfg()
         order = {0: 5, 1: 7, 2: 9}[params.rec_mp]
         rc = lib.oracle_default_cfg(ctypes.byref(self.c), params.imax, params.jmax, params.longx, params.longy,
-                                    2 if params.order == 2 else 4, params.flux_solver, order, params.bc_x,
+                                    int(params.order), params.flux_solver, order, params.bc_x,
                                     params.bc_y, params.sigma, params.cfl)
         if rc:
             raise ValueError("oracle config rejected (%d)" % rc)

@@ -116,7 +116,7 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
                        double cfl) {
   memset(c, 0, sizeof(*c));
   if (imax < 8 || jmax < 8) return 1;
-  if (lmax != 2 && lmax != 4) return 2;
+  if (lmax < 2 || lmax > 4) return 2;
   if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL &&
       flux_solver != OR_LAX && flux_solver != OR_HLLC_MCL && flux_solver != OR_HLLE_MCL)
     return 3;
@@ -152,6 +152,13 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
     c->Abt[0][0] = g;
     c->Abt[1][0] = 1.0 - 2.0 * g;
     c->Abt[1][1] = g;
+  } else if (lmax == 3) { /* SSP2(3,3,2)-LUM, imex_scheme.f95:77-86 */
+    for (int l = 0; l < 3; l++) c->wb[l] = c->wbt[l] = 1.0 / 3.0;
+    c->Ab[1][0] = 0.5;
+    c->Ab[2][0] = 0.5; c->Ab[2][1] = 0.5;
+    c->Abt[0][0] = 1.0 / 5.0;
+    c->Abt[1][0] = 1.0 / 10.0; c->Abt[1][1] = 1.0 / 5.0;
+    c->Abt[2][0] = 1.0 / 3.0; c->Abt[2][1] = 1.0 / 3.0; c->Abt[2][2] = 1.0 / 3.0;
   } else { /* imex_scheme.f95:280-291 with parameters.f95:242-244 */
     double a1 = 0.24169426078821, b1 = 0.06042356519705, e1 = 0.12915286960590;
     c->wb[1] = 1.0 / 6.0; c->wb[2] = 1.0 / 6.0; c->wb[3] = 2.0 / 3.0;

@@ -34,7 +34,7 @@ enum { OR_LAX = 1, OR_HLL_MCL = 2, OR_HLLE_MCL = 3, OR_HLLC_MCL = 4, OR_HLL = 5,
 
 typedef struct {
   int32_t imax, jmax;
-  int32_t lmax;            /* IMEX stages: 2 = SSP2(2,2,2), 4 = SSP3(4,3,3) */
+  int32_t lmax;            /* IMEX stages: 2 = SSP2(2,2,2), 3 = SSP2(3,3,2)-LUM, 4 = SSP3(4,3,3) */
   int32_t flux_solver;     /* 5 HLL, 6 HLLE, 7 HLLC */
   int32_t rec_order;       /* 5, 7 or 9 (MP5/MP7/MP9) */
   int32_t bc_x, bc_y;      /* OR_BC_COPY / OR_BC_PERIODIC */

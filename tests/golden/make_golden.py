@@ -73,6 +73,9 @@ CASES = {
                                         {"primitive_rec": 1, "limit": 1}),
     "kh_hllc_mp5_primrec_superbee_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None,
                                            {"primitive_rec": 1, "limit": 3}),
+    # IMEX SSP2(3,3,2)-LUM (order = 3), the scheme of most of the reference's parameters_*.f95
+    "kh_hllc_mp9_lum_32x64": (32, 64, 7, 2, 3, 1.0e6, 10, [1, 10], [1, 10], None, {"slop": 1, "limit": 1}),
+    "kh_hll_mcl_lum_32x64": (32, 64, 2, 0, 3, 1.0e3, 10, [1, 10], [1, 10], None, {"slop": 1}),
 }
 SEED = 20261202
 

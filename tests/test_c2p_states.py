@@ -46,8 +46,8 @@ class _Grid:
 
     def __init__(self, imax, jmax):
         import cueva
-        self.p = cueva.Params(imax=imax, jmax=jmax, flux_solver=cueva.HLLC, bc_x=cueva.BC_COPY,
-                              bc_y=cueva.BC_PERIODIC, sigma=1.0e6)
+        self.p = cueva.default_params(imax=imax, jmax=jmax, flux_solver=cueva.HLLC, bc_x=cueva.BC_COPY,
+                                      bc_y=cueva.BC_PERIODIC, sigma=1.0e6)
 
 
 def set_cell(prims, cons, i, j, st):

@@ -43,6 +43,7 @@ CASES = {
     "hlle_mcl_bz": (cueva.HLLE_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hllc_mp5_primrec": (cueva.HLLC, 0, 2, 1.0e6, {"primitive_rec": 1}),
     "hllc_mcl_superbee": (cueva.HLLC_MCL, 0, 2, 1.0e6, {"slop": 3}),
+    "hllc_mp7_lum": (cueva.HLLC, 1, 3, 1.0e6, {}),
     "lax_minmod_primrec_superbee": (cueva.LAX, 0, 2, 1.0e3, {"slop": 1, "primitive_rec": 1, "limit": 3}),
 }
 

@@ -161,6 +161,8 @@ BITWISE_DUMPS = {
     "kh_hll_mcl_minmod_32x64": (1, 10), "kh_hllc_mcl_superbee_32x64": (1, 10), "kh_lax_superbee_bz_32x64": (1, 10),
     "kh_hlle_mcl_minmod_bz_32x64": (1, 10), "kh_hll_mcl_primrec_minmod_32x64": (1, 10),
     "kh_hllc_mp5_primrec_superbee_32x64": (1, 10),
+    # IMEX SSP2(3,3,2)-LUM
+    "kh_hllc_mp9_lum_32x64": (1, 10), "kh_hll_mcl_lum_32x64": (1, 10),
 }
 
 
