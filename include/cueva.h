@@ -53,9 +53,12 @@ enum cueva_status {
   CUEVA_E_C2P_IMAG = 21,     /* no admissible resolvent root                 */
   CUEVA_E_C2P_DET = 22,      /* det < 0 in the Cardano quadratic             */
   CUEVA_E_C2P_NOROOT = 23,   /* both quartic roots below 1                   */
-  CUEVA_E_C2P_NAN = 24,      /* no root selected (NaN quartic roots); the
-                                reference does not stop here but goes on with
-                                the previous cell's W (12_varprimitive.f95:463-500) */
+  CUEVA_E_C2P_NAN = 24,      /* no root selected (NaN quartic roots) with
+                                finite quartic coefficients; the reference does
+                                not stop here but goes on with the previous
+                                cell's W (12_varprimitive.f95:463-500).  With a
+                                NaN coefficient the point goes on to NaN
+                                results, as in the reference                  */
   CUEVA_E_C2P_SUPERLUM = 25, /* superluminal v and magnetisation < 10        */
   CUEVA_E_C2P_DRIFT = 26     /* drift-velocity fix still superluminal        */
 };
@@ -91,8 +94,9 @@ enum { CUEVA_BND_PRIM = 0, CUEVA_BND_CONS = 1, CUEVA_BND_ELEC = 2, CUEVA_BND_FLU
  * (parameters.f95:10-334).  Fill with cueva_params_default, then override. */
 typedef struct cueva_params {
   int32_t imax, jmax;     /* cells per axis (parameters.f95:110-111)           */
-  int32_t order;          /* 2 = IMEX SSP2(2,2,2), 3 = SSP2(3,3,2)-LUM,
-                             4 = SSP3(4,3,3) (:10); stages lmax = order        */
+  int32_t order;          /* 1 = first-order scheme (MIRK1), 2 = IMEX SSP2(2,2,2)
+                             or MIRK2, 3 = SSP2(3,3,2)-LUM, 4 = SSP3(4,3,3)
+                             (:10); stages lmax = order                        */
   int32_t flux_solver;    /* CUEVA_HLL/_HLLE/_HLLC/_HLL_MCL/_HLLE_MCL/_HLLC_MCL/_LAX (:21) */
   int32_t rec_mp;         /* 0 MP5, 1 MP7, 2 MP9 (:33)                          */
   int32_t bc_x, bc_y;     /* CUEVA_BC_* per axis                                */
@@ -107,7 +111,17 @@ typedef struct cueva_params {
                              (:17): 1 MinMod, 2 monotonised central, 3 Superbee;
                              0 = 2                                               */
   int32_t limit;          /* the same for recons_var_primitive (:18); 0 = 2    */
-  int32_t reserved[2];
+  int32_t mirk;           /* electric field integration (:11): 0 IMEX implicit
+                             stage solve, 1 MIRK first order, 2 MIRK second
+                             order (11_elecvarint.f95, 20_electricfield.f95);
+                             order 2 with MIRK 1 is refused, as the reference
+                             stops on it (imex_scheme.f95:51-56)              */
+  int32_t rec_prim;       /* 1: primitive recovery of every stage state and the
+                             stage's fluxes from those primitives (:23,
+                             main_cueva_code.f95:220-228, 13_intercon_var.f95);
+                             refused with the second-order HLL/HLLE/HLLC,
+                             whose REC_PRIM branch in the reference reads an
+                             unset z-velocity (16_hll_flow_mcl.f95:911-958)  */
   double longx, longy;    /* domain lengths Lx, Ly (:174-175)                   */
   double cfl;             /* CFL number (:309)                                  */
   double sigma;           /* conductivity sigma_0 (:196)                        */
@@ -159,6 +173,16 @@ int cueva_swap1(cueva_ctx *ctx);                   /* 08_swap1.f95:2           *
 int cueva_sumint(cueva_ctx *ctx, int l, int m);    /* 09_sumint.f95:5          */
 int cueva_varint(cueva_ctx *ctx, int l);           /* 10_varint.f95:5          */
 int cueva_electricvarint(cueva_ctx *ctx, int l);   /* 11_elecvarint.f95:3      */
+/* REC_PRIM = 1 only (main_cueva_code.f95:220-228): varprimitivecardano with
+ * l_swap = 0 on stage l's state (12_varprimitive.f95:111-135,792-806),
+ * started from that stage's primitives as they stand (the previous step's;
+ * zero before the first step); its fix-ups count towards the step's
+ * counters.  Does not wait: a failure surfaces at the next call that waits. */
+int cueva_c2p_stage(cueva_ctx *ctx, int l);
+/* REC_PRIM = 1 only: interconvar (13_intercon_var.f95:6), stage l's
+ * conserved variables from its recovered primitives, then the ghosts of the
+ * stage's conserved and E planes. */
+int cueva_interconvar(cueva_ctx *ctx, int l);
 int cueva_flowinit(cueva_ctx *ctx, int l);         /* 14_flowinit.f95          */
 /* hll_flow_mpx / hlle_flow_mpx / hllc_flow_mpx / hll_flow / hlle_flow /
  * hllc_flow / lax_flow per params.flux_solver (16_hll_flow_mpx.f95,
@@ -169,9 +193,12 @@ int cueva_fluxandsource(cueva_ctx *ctx, int l);    /* 17_fluxandsource.f95:6   *
 int cueva_sumfin(cueva_ctx *ctx, int l);           /* 18_sumfin.f95:5          */
 int cueva_varfin(cueva_ctx *ctx);                  /* 19_varfin.f95:5          */
 int cueva_electricfield(cueva_ctx *ctx);           /* 20_electricfield.f95     */
-/* varprimitivecardano with l_swap = 1 (12_varprimitive.f95:6).  Waits for the
- * device; counters = {contador_r, contador_p, contador_v} of this call, whose
- * fix-up records cueva_fixup_records returns; fallback = cells that fell back
+/* varprimitivecardano with l_swap = 1, or with REC_PRIM = 1 l_swap = 2 (from
+ * the last stage's primitives; 12_varprimitive.f95:6, main_cueva_code.f95:
+ * 301-309).  Waits for the device; counters = {contador_r, contador_p,
+ * contador_v} of this call (with REC_PRIM = 1: of the step since
+ * cueva_cleanvarint, stage recoveries included), whose fix-up records
+ * cueva_fixup_records returns; fallback = cells that fell back
  * to Cardano in the learned mode (else 0).  A superluminal previous velocity
  * is clamped with the reference's eps1 (1e-6 once an HLL+MPx face solve ran
  * in this ctx, else 0; 12_varprimitive.f95:236-241, 16_hll_flow_mpx.f95:1375).

@@ -162,14 +162,22 @@ extern "C" int cueva_derived(const cueva_params* p, double out[5]) {
   return CUEVA_OK;
 }
 
-// Butcher tables of imex_scheme.f95:40-49 (SSP2(2,2,2)), :77-86 (SSP2(3,3,2)-LUM)
-// and :280-291 (SSP3(4,3,3))
-static void imex_tables(int order, Scheme& S) {
+// Butcher tables of imex_scheme.f95:13-20 (first order), :24-36 (MIRK2),
+// :40-49 (SSP2(2,2,2)), :77-86 (SSP2(3,3,2)-LUM) and :280-291 (SSP3(4,3,3))
+static void imex_tables(int order, int mirk, Scheme& S) {
   memset(S.wb, 0, sizeof(S.wb));
   memset(S.wbt, 0, sizeof(S.wbt));
   memset(S.Ab, 0, sizeof(S.Ab));
   memset(S.Abt, 0, sizeof(S.Abt));
-  if (order == 2) {
+  if (order == 1) {  // imex_scheme.f95:13-20, the first-order scheme of MIRK1
+    S.lmax = 1;
+    S.wb[0] = 1.0;
+    S.Ab[0][0] = 1.0;
+  } else if (order == 2 && mirk == 2) {  // MIRK2, imex_scheme.f95:24-36
+    S.lmax = 2;
+    S.wb[0] = 0.5; S.wb[1] = 0.5;
+    S.Ab[1][0] = 1.0;
+  } else if (order == 2) {
     const double g = 1.0 - 1.0 / sqrt(2.0);  // parameters.f95:240
     S.lmax = 2;
     S.wb[0] = 0.5; S.wb[1] = 0.5;
@@ -203,7 +211,17 @@ static void imex_tables(int order, Scheme& S) {
 static int check_params(const cueva_params* p) {
   if (!p) return CUEVA_E_ARG;
   if (p->imax < 8 || p->jmax < 8 || p->imax > 65536 || p->jmax > 65536) return CUEVA_E_ARG;
-  if (p->order < 2 || p->order > 4) return CUEVA_E_ARG;
+  if (p->order < 1 || p->order > 4) return CUEVA_E_ARG;
+  if (p->mirk < 0 || p->mirk > 2 || (p->order == 2 && p->mirk == 1)) return CUEVA_E_ARG;
+  if (p->rec_prim != 0 && p->rec_prim != 1) return CUEVA_E_ARG;
+  // REC_PRIM = 1 with the second-order HLL / HLLE / HLLC: the reference's
+  // REC_PRIM branch of those solvers (16_hll_flow_mcl.f95:911-958,
+  // 16_hllc_flow_mcl.f95:2106-2153, 16_hlle_flow_mcl.f95:1993-2040) stores
+  // the z-velocity's right states in Vy_x_R / Vy_y_R and never sets Vz_x_R /
+  // Vz_y_R, OpenMP-private scalars it then reads: its results are undefined
+  if (p->rec_prim && (p->flux_solver == CUEVA_HLL_MCL || p->flux_solver == CUEVA_HLLE_MCL ||
+                      p->flux_solver == CUEVA_HLLC_MCL))
+    return CUEVA_E_ARG;
   if (p->flux_solver != CUEVA_HLL && p->flux_solver != CUEVA_HLLC && p->flux_solver != CUEVA_HLLE &&
       p->flux_solver != CUEVA_HLL_MCL && p->flux_solver != CUEVA_LAX && p->flux_solver != CUEVA_HLLC_MCL &&
       p->flux_solver != CUEVA_HLLE_MCL)
@@ -248,7 +266,12 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   S.hllc_seed_ref = p->hllc_seed;
   S.crmath = p->crmath;
   S.slop = p->slop ? p->slop : 2;
-  imex_tables(p->order, S);
+  S.mirk = p->mirk;
+  S.rec_prim = p->rec_prim;
+  S.cm1 = -0.05;  // parameters.f95:233-235
+  S.cm2 = 1.0;
+  S.cm4 = (0.5 * ((1.0 - S.cm1) * (1.0 - S.cm1))) / S.cm1;
+  imex_tables(p->order, p->mirk, S);
   c->rec_order = 5 + 2 * p->rec_mp;
   {
     const char* e = getenv("CUEVA_MARCH");
@@ -264,7 +287,8 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     c->c2p_in_march = e ? atoi(e) != 0 : 0;
   }
 
-  const int nplanes = P_N + C_N + 2 * S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N;
+  const int nstage = p->rec_prim ? S.lmax * 7 : 0;  // stage primitives of REC_PRIM = 1
+  const int nplanes = P_N + C_N + 2 * S_N + FX_N + FY_N + MAXL * R_N + 2 * A_N + nstage;
   c->bytes = (int64_t)nplanes * g.plane * (int64_t)sizeof(double);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return CUEVA_E_HIP; }
   if (hipMalloc(&c->base, (size_t)c->bytes) != hipSuccess) {
@@ -309,6 +333,14 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     for (int k = 0; k < R_N; k++) c->P.rhs[l][k] = take();
   for (int k = 0; k < A_N; k++) c->P.ints[k] = take();
   for (int k = 0; k < A_N; k++) c->P.fins[k] = take();
+  if (p->rec_prim)
+    for (int l = 0; l < S.lmax; l++) {
+      for (int d = 0; d < 3; d++) c->P.vint[l][d] = take();
+      c->P.wint[l] = take();
+      c->P.rhoint[l] = take();
+      c->P.pint[l] = take();
+      c->P.hint[l] = take();
+    }
   recover_in_place(c->P);
   // every plane pointer the kernels receive must lie inside the allocation
   {
@@ -318,6 +350,7 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     const int nptr = (int)(sizeof(Planes) / sizeof(void*));
     for (int k = 0; k < nptr; k++) {
       const char* a = (const char*)ptrs[k];
+      if (!a) continue;  // stage primitives without REC_PRIM
       if (a < lo || a + g.plane * (int64_t)sizeof(double) > hi) {
         delete c->launches;
         hipFree(c->dst);
@@ -408,6 +441,7 @@ static StatusRef sref(cueva_ctx* c, int l, bool record = false, bool fused = fal
   r.seq = ++c->seq;
   r.h = (int)c->hstep;
   r.cap = FIX_CAP;
+  r.l = 0;
   if (record && c->launches->size() < MAX_LAUNCH_RECORDS) {
     LaunchRecord lr;
     lr.seq = r.seq;
@@ -465,7 +499,6 @@ static const dim3 BLK(64, 4);
 
 extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
   if (!c) return CUEVA_E_ARG;
-  (void)l;
   double* list[20];
   int n = 0;
   switch (kind) {
@@ -484,6 +517,12 @@ extern "C" int cueva_boundary(cueva_ctx* c, int kind, int l) {
       // guarded by flux_solver == 5 .or. 7 (24_boundary_flux_conserved.f95:13)
       if (!j_ghosts_filled(c->prm.flux_solver)) return CUEVA_OK;
       list[n++] = c->P.s[S_JX]; list[n++] = c->P.s[S_JY]; list[n++] = c->P.s[S_JZ];
+      // and the stage velocity (:1426-1560 and the like); the stage rho, p and
+      // W keep zero ghosts: no routine of the reference fills them
+      if (c->prm.rec_prim) {
+        if (l < 1 || l > c->S.lmax) return CUEVA_E_ARG;
+        for (int d = 0; d < 3; d++) list[n++] = c->P.vint[l - 1][d];
+      }
       break;
     default:
       return CUEVA_E_ARG;
@@ -524,9 +563,14 @@ extern "C" int cueva_cleanvarint(cueva_ctx* c) {
   return clear_counters(c);  // contador_* = 0 (main_cueva_code.f95:189-193)
 }
 
-// swap1 copies the primitives into stage-1 arrays that only REC_PRIM=1 and
-// l_swap=2 read (08_swap1.f95); with the flags of this path it has no effect.
-extern "C" int cueva_swap1(cueva_ctx* c) { return c ? CUEVA_OK : CUEVA_E_ARG; }
+// swap1 copies the primitives into the stage-1 primitives (08_swap1.f95),
+// which only REC_PRIM = 1 reads; without it there is nothing to do.
+extern "C" int cueva_swap1(cueva_ctx* c) {
+  if (!c) return CUEVA_E_ARG;
+  if (!c->prm.rec_prim) return CUEVA_OK;
+  LAUNCH(K_OTHER, k_swap1, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S);
+  return launch_ok();
+}
 
 static int stage_ok(const cueva_ctx* c, int l) { return c && l >= 1 && l <= c->S.lmax; }
 
@@ -552,28 +596,46 @@ extern "C" int cueva_electricvarint(cueva_ctx* c, int l) {
 
 extern "C" int cueva_flowinit(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
-  LAUNCH(K_OTHER, k_flowinit, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S);
+  LAUNCH(K_OTHER, k_flowinit, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
   return launch_ok();
 }
 
 template <int ORD>
-static int launch_faces_ord(cueva_ctx* c, const StatusRef& sr) {
+static int launch_faces_ord(cueva_ctx* c, const StatusRef& sr, const Planes& P) {
   const dim3 grid = faces_grid(c->g, BLK);
   if (c->prm.flux_solver == CUEVA_LAX)
-    LAUNCH(K_FACES, (k_faces<ORD, 1>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 1>), grid, BLK, c->g, P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLC_MCL)
-    LAUNCH(K_FACES, (k_faces<ORD, 4>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 4>), grid, BLK, c->g, P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLE_MCL)
-    LAUNCH(K_FACES, (k_faces<ORD, 3>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 3>), grid, BLK, c->g, P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLL_MCL)
-    LAUNCH(K_FACES, (k_faces<ORD, 2>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 2>), grid, BLK, c->g, P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLC)
-    LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 7>), grid, BLK, c->g, P, c->S, sr);
   else if (c->prm.flux_solver == CUEVA_HLLE)
-    LAUNCH(K_FACES, (k_faces<ORD, 6>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 6>), grid, BLK, c->g, P, c->S, sr);
   else
-    LAUNCH(K_FACES, (k_faces<ORD, 5>), grid, BLK, c->g, c->P, c->S, sr);
+    LAUNCH(K_FACES, (k_faces<ORD, 5>), grid, BLK, c->g, P, c->S, sr);
   return launch_ok();
+}
+
+// The plane set of stage l's face solves and right-hand side: with REC_PRIM
+// = 1 they read the stage's velocity, pressure and density (16_*_flow_*.f95
+// REC_PRIM branches; Lax-Friedrichs through flowinit's cell fluxes,
+// 14_flowinit.f95:376-489; 17_fluxandsource.f95:460-474).  l is 1-based.
+static Planes stage_planes(const cueva_ctx* c, int l) {
+  Planes P = c->P;
+  for (int q = 0; q < K_N; q++) P.kin[q] = nullptr;
+  if (c->prm.rec_prim && l >= 1 && l <= c->S.lmax) {
+    const int s = l - 1;
+    P.kin[K_VX] = c->P.vint[s][0];
+    P.kin[K_VY] = c->P.vint[s][1];
+    P.kin[K_VZ] = c->P.vint[s][2];
+    P.kin[K_P] = c->P.pint[s];
+    P.kin[K_RHO] = c->P.rhoint[s];
+  }
+  return P;
 }
 
 // hll_flow_mpx sets the module variable eps1 that the recovery's clamp of a
@@ -586,20 +648,21 @@ static void note_face_solver(cueva_ctx* c) {
 static int launch_faces(cueva_ctx* c, int l, bool fused) {
   const StatusRef sr = sref(c, l, true, fused);
   note_face_solver(c);
+  const Planes P = stage_planes(c, l);
   if (c->prm.precision == CUEVA_MIXED) {
     const dim3 grid = faces_grid(c->g, BLK);
     prof_begin(c, K_FACES);
-    hipError_t e = launch_faces_fp32(c->rec_order, grid, BLK, c->stream, c->g, c->P, c->S, sr);
+    hipError_t e = launch_faces_fp32(c->rec_order, grid, BLK, c->stream, c->g, P, c->S, sr);
     prof_end(c);
     return e == hipSuccess ? CUEVA_OK : CUEVA_E_HIP;
   }
   if (c->prm.flux_solver == CUEVA_HLL_MCL || c->prm.flux_solver == CUEVA_LAX ||
       c->prm.flux_solver == CUEVA_HLLC_MCL || c->prm.flux_solver == CUEVA_HLLE_MCL)
-    return launch_faces_ord<5>(c, sr);  // the MP order does not enter
+    return launch_faces_ord<5>(c, sr, P);  // the MP order does not enter
   switch (c->rec_order) {
-    case 5: return launch_faces_ord<5>(c, sr);
-    case 7: return launch_faces_ord<7>(c, sr);
-    default: return launch_faces_ord<9>(c, sr);
+    case 5: return launch_faces_ord<5>(c, sr, P);
+    case 7: return launch_faces_ord<7>(c, sr, P);
+    default: return launch_faces_ord<9>(c, sr, P);
   }
 }
 
@@ -615,11 +678,34 @@ extern "C" int cueva_flux(cueva_ctx* c, int l) {
 
 extern "C" int cueva_fluxandsource(cueva_ctx* c, int l) {
   if (!stage_ok(c, l)) return CUEVA_E_ARG;
+  const Planes P = stage_planes(c, l);
   if (c->prm.precision == CUEVA_MIXED)
-    LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1, sref(c, l));
+    LAUNCH(K_FLUXSRC, k_fluxandsource<float>, interior_grid(c->g, BLK), BLK, c->g, P, c->S, l - 1, sref(c, l));
   else
-    LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1, sref(c, l));
+    LAUNCH(K_FLUXSRC, k_fluxandsource<double>, interior_grid(c->g, BLK), BLK, c->g, P, c->S, l - 1, sref(c, l));
   return launch_ok();
+}
+
+// varprimitivecardano with l_swap = 0 on stage l (main_cueva_code.f95:220-224):
+// REC_PRIM = 1 only; its fix-ups count towards the step's counters
+extern "C" int cueva_c2p_stage(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l) || !c->prm.rec_prim) return CUEVA_E_ARG;
+  if (c->err) return c->err;
+  StatusRef sr = sref(c, l, true, false);
+  sr.l = l;
+  LAUNCH(K_OTHER, k_c2p_stage, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1, sr);
+  return launch_ok();
+}
+
+// interconvar, 13_intercon_var.f95: stage l's conserved variables from its
+// recovered primitives, then the ghosts of the stage's conserved and E
+// planes (:74-75)
+extern "C" int cueva_interconvar(cueva_ctx* c, int l) {
+  if (!stage_ok(c, l) || !c->prm.rec_prim) return CUEVA_E_ARG;
+  LAUNCH(K_OTHER, k_interconvar, interior_grid(c->g, BLK), BLK, c->g, c->P, c->S, l - 1);
+  if (launch_ok()) return CUEVA_E_HIP;
+  int rc = cueva_boundary(c, CUEVA_BND_CONS, l);
+  return rc ? rc : cueva_boundary(c, CUEVA_BND_ELEC, l);
 }
 
 extern "C" int cueva_sumfin(cueva_ctx* c, int l) {
@@ -698,11 +784,14 @@ extern "C" int cueva_set_c2p_params(cueva_ctx* c, const float* packed) {
 }
 
 extern "C" int cueva_c2p(cueva_ctx* c, int l_swap, int32_t counters[3], int32_t* fallback) {
-  if (!c || l_swap != 1) return CUEVA_E_ARG;
+  // l_swap = 2: from the last stage's primitives, as main_cueva_code.f95:301-303
+  // selects with REC_PRIM = 1
+  if (!c || (l_swap != 1 && l_swap != 2) || (l_swap == 2) != (c->prm.rec_prim != 0)) return CUEVA_E_ARG;
   if (c->err) return c->err;
   // the step's counters and records start here (the reference zeroes the
-  // counters after cleanvarint and nothing else counts before the recovery)
-  int rc = clear_counters(c);
+  // counters after cleanvarint and nothing else counts before the recovery),
+  // except with REC_PRIM = 1, whose stage recoveries counted since cleanvarint
+  int rc = c->prm.rec_prim ? CUEVA_OK : clear_counters(c);
   if (rc) return rc;
   if ((rc = launch_c2p(c, false))) return rc;
   DevStatus h;
@@ -827,6 +916,7 @@ static int launch_march(cueva_ctx* c, int l, bool fin, bool c2p = false, bool st
 }
 
 static int fused_steps(cueva_ctx* c, int nsteps, int halo, bool learned);
+static int routine_steps(cueva_ctx* c, int nsteps);
 
 extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   if (!c || nsteps < 0) return CUEVA_E_ARG;
@@ -834,7 +924,7 @@ extern "C" int cueva_step(cueva_ctx* c, int nsteps) {
   const bool learned = c->prm.c2p_mode == CUEVA_C2P_LEARNED;
   if (learned && !c->has_mlp) return CUEVA_E_ARG;
   if (c->err) return c->err;
-  int rc0 = fused_steps(c, nsteps, halo, learned);
+  int rc0 = c->prm.rec_prim ? routine_steps(c, nsteps) : fused_steps(c, nsteps, halo, learned);
   // one wait per call: a failure stops the call's later launches on the
   // device, and its status, step, stage and cell are known here
   DevStatus h;
@@ -935,6 +1025,55 @@ static int fused_steps(cueva_ctx* c, int nsteps, int halo, bool learned) {
   return CUEVA_OK;
 }
 
+// Steps with REC_PRIM = 1 as the reference sequences them (main_cueva_code.f95:
+// 53-328), one launch per subroutine: every stage recovers its primitives,
+// which feed that stage's fluxes, so no stage can be formed ahead of its
+// recovery.  The device is waited on after every launch that can fail, and
+// a failure ends the call there (the state is then that of the failing
+// launch, as where the reference stops).
+static int routine_steps(cueva_ctx* c, int nsteps) {
+  const int lmax = c->S.lmax;
+  DevStatus st;
+  int rc;
+#define RS(x) do { if ((rc = (x))) return rc; } while (0)
+#define RS_WAIT() do { RS(read_status(c, &st)); if (c->err) return c->err; } while (0)
+  for (int h = 0; h < nsteps; h++) {
+    RS(cueva_boundary(c, CUEVA_BND_PRIM, 1));
+    if (c->prm.primitive_rec) {
+      RS(recons_prims(c));
+      RS(cueva_boundary(c, CUEVA_BND_PRIM, 1));
+    }
+    RS(cueva_startconvar(c));
+    RS(cueva_cleanvarint(c));
+    RS(cueva_swap1(c));
+    for (int l = 1; l <= lmax; l++) {
+      for (int m = 1; m < l; m++) RS(cueva_sumint(c, l, m));
+      RS(cueva_varint(c, l));
+      RS(cueva_electricvarint(c, l));
+      RS(cueva_c2p_stage(c, l));
+      RS_WAIT();
+      RS(cueva_interconvar(c, l));
+      RS(cueva_boundary(c, CUEVA_BND_CONS, l));
+      RS(cueva_boundary(c, CUEVA_BND_ELEC, l));
+      RS(cueva_flowinit(c, l));
+      RS(cueva_boundary(c, CUEVA_BND_FLUX, l));
+      RS(cueva_flux(c, l));
+      RS_WAIT();
+      RS(cueva_fluxandsource(c, l));
+      RS(cueva_sumfin(c, l));
+    }
+    RS(cueva_varfin(c));
+    RS(cueva_electricfield(c));
+    RS(launch_c2p(c, false));
+    RS_WAIT();
+    c->hstep++;
+    RS(cueva_boundary(c, CUEVA_BND_PRIM, 1));
+  }
+#undef RS_WAIT
+#undef RS
+  return CUEVA_OK;
+}
+
 extern "C" int cueva_step_counters(cueva_ctx* c, int64_t counters[3], int64_t* fallback) {
   if (!c) return CUEVA_E_ARG;
   DevStatus h;
@@ -977,9 +1116,14 @@ extern "C" int cueva_fixup_records(cueva_ctx* c, int kind, int32_t* hij, double*
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  // the reference's order: steps in turn, then its loop order (j outer, i inner)
-  std::sort(r.begin(), r.end(), [](const FixRec& a, const FixRec& b) {
+  // the reference's order: steps in turn; within a step the stage recoveries
+  // (REC_PRIM = 1) in stage order, then the one after varfin; within a
+  // recovery its loop order (j outer, i inner)
+  const int after = c->S.lmax + 1;
+  std::sort(r.begin(), r.end(), [after](const FixRec& a, const FixRec& b) {
     if (a.h != b.h) return a.h < b.h;
+    const int la = a.l ? a.l : after, lb = b.l ? b.l : after;
+    if (la != lb) return la < lb;
     if (a.j != b.j) return a.j < b.j;
     return a.i < b.i;
   });

@@ -41,7 +41,29 @@ struct Planes {
   // themselves, or (inside the marching step, whose stencils still read the
   // t^n values) scratch planes that then take their place
   double* pn[5];
+  // stage primitives of REC_PRIM = 1, per stage (fourvectors.f95:12-13:
+  // Vxint..Vzint, Wint, rhoint, pint, enthpyint), kept from step to step;
+  // null without REC_PRIM
+  double* vint[MAXL][3];
+  double* wint[MAXL];
+  double* rhoint[MAXL];
+  double* pint[MAXL];
+  double* hint[MAXL];
+  // where the face solves and the stage right-hand side read the velocity,
+  // pressure and density they reconstruct (K_* order): null = the primitive
+  // planes (REC_PRIM = 0); the stage primitives with REC_PRIM = 1, set only in
+  // the launch's own copy of the plane set
+  const double* kin[5];
 };
+
+enum { K_VX, K_VY, K_VZ, K_P, K_RHO, K_N };
+
+// the plane a face solve or the stage right-hand side reads for kinematic
+// quantity q (K_*)
+__device__ inline const double* kin_plane(const Planes& P, int q) {
+  const int pq = q == K_VX ? P_VX : q == K_VY ? P_VY : q == K_VZ ? P_VZ : q == K_P ? P_P : P_RHO;
+  return P.kin[q] ? P.kin[q] : P.p[pq];
+}
 
 // the recovery writes in place
 inline void recover_in_place(Planes& P) {
@@ -65,6 +87,11 @@ struct Scheme {
   // velocity in the recovery (12_varprimitive.f95:236-241): 1e-6 once
   // hll_flow_mpx has run in the process (16_hll_flow_mpx.f95:1375), else 0
   double eps1;
+  // electric field integration (parameters.f95:11): 0 IMEX, 1 MIRK1, 2 MIRK2,
+  // with the MIRK coefficients cm_1, cm_2, cm_4 (:233-235)
+  int mirk;
+  int rec_prim;  // stage primitive recovery (parameters.f95:23)
+  double cm1, cm2, cm4;
 };
 
 // status words in device memory
@@ -82,7 +109,8 @@ constexpr size_t STATUS_COUNTERS_OFFSET = 2 * sizeof(unsigned long long);
 
 // one record of the reference's fix-up files 301-303 (12_varprimitive.f95:544,601,663)
 struct FixRec {
-  int h, i, j, pad;
+  int h, i, j;
+  int l;  // the recovery that wrote it: stage l (REC_PRIM = 1), or 0 for the one after varfin
   double a, b;  // rho* and rho^n / p* and p^n / v*^2 and v_drift^2
 };
 
@@ -94,6 +122,7 @@ struct StatusRef {
   unsigned long long seq;
   int h;
   int cap;
+  int l;  // stage of a stage recovery (REC_PRIM = 1), else 0
 };
 
 // Cells are ordered as the reference's loops visit them (j outer, i inner),
@@ -131,7 +160,7 @@ __device__ inline void add_fixup(const StatusRef& r, int kind, int i, int j, dou
     e.h = r.h;
     e.i = i;
     e.j = j;
-    e.pad = 0;
+    e.l = r.l;
     e.a = a;
     e.b = b;
   }

@@ -48,8 +48,12 @@ template <typename T> __device__ inline T pw4(T x) { return x * (x * (x * x)); }
 // FP32 form uses the IEEE-754 2019 minimum/maximum (one v_minimum3_f32 /
 // v_maximum3_f32 instead of a compare, a hazard wait and a select; they
 // differ only in the sign of a zero result and propagate every NaN)
-template <typename T> __device__ inline T mn(T a, T b) { return b < a ? b : a; }
-template <typename T> __device__ inline T mx(T a, T b) { return b > a ? b : a; }
+// Fortran min / max as the reference's compiler (flang) evaluates them: the
+// first argument when it compares strictly less (greater), else the second,
+// so min(NaN, x) = x, min(x, NaN) = NaN, min(-0, +0) = +0; three arguments
+// fold from the left
+template <typename T> __device__ inline T mn(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ inline T mx(T a, T b) { return a > b ? a : b; }
 template <> __device__ inline float mn<float>(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 template <> __device__ inline float mx<float>(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 // max(lo, min(hi, x)) as the reference build evaluates it (a NaN passes)
@@ -381,7 +385,7 @@ __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed
   T c = sx_h - erx;
   int fail = 0;
   T v = MC ? hllc_contact(a, b, c, seed_ref, mcref, fail, T(1.0e-6), vlast) : hllc_contact(a, b, c, seed_ref, R.bz, fail);
-  if (fail || v != v) return 7;
+  if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
   T pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
   T psiR = (S2 * psi_h - v * ex_f) / (S2 - v);
   T psiL = (-(S1 * psi_h) + v * ex_f) / (v - S1);
@@ -484,7 +488,7 @@ __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed
   T c = sy_h - ery;
   int fail = 0;
   T v = MC ? hllc_contact(a, b, c, seed_ref, mcref, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, -b, fail);
-  if (fail || v != v) return 7;
+  if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
   T pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
   T psiR = (S2 * psi_h - v * ey_f) / (S2 - v);
   T psiL = (-(S1 * psi_h) + v * ey_f) / (v - S1);
@@ -673,7 +677,7 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
     else if (fab(v) > T(1))
       v = clampf(v, T(-1) + T(1.0e-6), T(1) - T(1.0e-6));
   }
-  if (fail || v != v) return 7;
+  if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
   T pstr = (s_f[n] + (sq(e[n]) + sq(b[n]))) - (tau_f - er[n]) * v;
   T sR[3], sL[3];
   sR[n] = (sb * s_h[n] - tau_f * v) / (sb - v);
@@ -796,7 +800,7 @@ __device__ inline int hlle_mcl_face(T gam, Side<T>& L, Side<T>& R, T* f, bool se
     else if (fab(v) > T(1))
       v = clampf(v, T(-1) + T(1.0e-6), T(1) - T(1.0e-6));
   }
-  if (fail || v != v) return 7;
+  if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
   // EM face fluxes: the HLL states (their HLL fluxes when v* = 0)
   if (v == T(0)) {
     if constexpr (DIR == 0) {
@@ -920,6 +924,33 @@ __device__ inline void e_solve(double abt, double sig, double dt, double w, doub
               kk * ((((aw2 * vx) * vz) * rx + ((aw2 * vy) * vz) * ry) + tzz * rz));
 }
 
+// MIRK2 stage E of stages l >= 2 (11_elecvarint.f95:249-265): the linearly
+// implicit update from E^n, v^n, B^n, W^n and the stage sum `acc` of the E
+// equation; cm1 = cm_1 (parameters.f95:233)
+__device__ inline double mirk2_stage_e(double sb, double cm1, double dt, double e, double v, double edv, double r,
+                                       double acc) {
+#pragma clang fp contract(off)
+  return (1.0 / (1.0 + sb * (1.0 - cm1))) * ((((e * (1.0 - sb * cm1)) + (sb * v) * edv) - sb * r) + dt * acc);
+}
+
+// MIRK1 E^{n+1} (20_electricfield.f95:36-46): e = E^n, v = v^n, rn / r = the
+// components of v^n x B^n and v^n x B^{n+1}, acc = the final sum
+__device__ inline double mirk1_final_e(double sb, double cm2, double dt, double e, double v, double edv, double rn,
+                                       double r, double acc) {
+#pragma clang fp contract(off)
+  return (1.0 / (1.0 + sb)) * (((e + (sb * v) * edv) - sb * (cm2 * rn + (1.0 - cm2) * r)) + dt * acc);
+}
+
+// MIRK2 E^{n+1} (20_electricfield.f95:117-141): ei = the last stage's E, en =
+// E^n, v = the last stage's velocity, r = its v x B, acc = the final sum
+__device__ inline double mirk2_final_e(double sdb, double cm1, double cm4, double dt, double ei, double en, double v,
+                                       double edv, double r, double acc) {
+#pragma clang fp contract(off)
+  const double a = -1.0 + sdb * (1.0 - cm1);
+  const double den = 1.0 + (0.5 * cm1 - cm4) * sdb;
+  return (ei + ((0.5 * a) / den) * (ei - en)) + (0.5 / den) * ((((-(sdb * ei)) + (sdb * edv) * v) + dt * acc) - sdb * r);
+}
+
 // sigma W (E + v x B - (E.v) v) (14_flowinit.f95:281-301, 17_fluxandsource.f95:527-537)
 __device__ inline void ohm_source(double sig, double w, double vx, double vy, double vz, double ex, double ey,
                                   double ez, double bx, double by, double bz, double& sx, double& sy, double& sz) {
@@ -1022,7 +1053,12 @@ __device__ inline int c2p_cardano(const Quartic& qd, double& Wn, bool crm) {
   if (y1 <= 1.0 && y2 <= 1.0) return 23;
   if (y1 >= 1.0 && y2 <= 1.0) W = y1;
   if (y1 <= 1.0 && y2 >= 1.0) W = y2;
-  if (W != W) return 24;
+  // No root selected (NaN roots): the reference keeps the previous cell's W
+  // (12_varprimitive.f95:463-500 assign nothing).  With a NaN or infinite
+  // quartic coefficient every Newton update is NaN whatever the start, so the
+  // point goes on to NaN results as there; with finite coefficients the
+  // previous cell's W would matter, and the point stops instead (status 24).
+  if (W != W && isfinite(qd.b3) && isfinite(qd.b2) && isfinite(qd.b1) && isfinite(qd.b0)) return 24;
   Wn = W;
   return 0;
 }

@@ -170,11 +170,11 @@ __device__ inline void mcl_face(const Planes& P, const Scheme& S, int64_t c, int
   mcl_rec(S.slop, P.s[S_SX], c, st, L.sx, R.sx);
   mcl_rec(S.slop, P.s[S_SY], c, st, L.sy, R.sy);
   mcl_rec(S.slop, P.s[S_SZ], c, st, L.sz, R.sz);
-  mcl_rec(S.slop, P.p[P_VX], c, st, L.vx, R.vx);
-  mcl_rec(S.slop, P.p[P_VY], c, st, L.vy, R.vy);
-  mcl_rec(S.slop, P.p[P_VZ], c, st, L.vz, R.vz);
-  mcl_rec(S.slop, P.p[P_P], c, st, L.p, R.p);
-  mcl_rec(S.slop, P.p[P_RHO], c, st, L.rho, R.rho);
+  mcl_rec(S.slop, kin_plane(P, K_VX), c, st, L.vx, R.vx);
+  mcl_rec(S.slop, kin_plane(P, K_VY), c, st, L.vy, R.vy);
+  mcl_rec(S.slop, kin_plane(P, K_VZ), c, st, L.vz, R.vz);
+  mcl_rec(S.slop, kin_plane(P, K_P), c, st, L.p, R.p);
+  mcl_rec(S.slop, kin_plane(P, K_RHO), c, st, L.rho, R.rho);
   // "_m" states: the cell value negated, the slope kept
   // (16_hll_flow_mcl.f95:634-655, 708-737)
   const T em_l = (-e0[0]) + es[0], em_r = (-e0[1]) - es[1];
@@ -236,11 +236,11 @@ __device__ inline void mcl_sides(const Planes& P, const Scheme& S, int64_t c, in
   mcl_rec(S.slop, P.s[S_SX], c, st, l.sx, r.sx);
   mcl_rec(S.slop, P.s[S_SY], c, st, l.sy, r.sy);
   mcl_rec(S.slop, P.s[S_SZ], c, st, l.sz, r.sz);
-  mcl_rec(S.slop, P.p[P_VX], c, st, l.vx, r.vx);
-  mcl_rec(S.slop, P.p[P_VY], c, st, l.vy, r.vy);
-  mcl_rec(S.slop, P.p[P_VZ], c, st, l.vz, r.vz);
-  mcl_rec(S.slop, P.p[P_P], c, st, l.p, r.p);
-  mcl_rec(S.slop, P.p[P_RHO], c, st, l.rho, r.rho);
+  mcl_rec(S.slop, kin_plane(P, K_VX), c, st, l.vx, r.vx);
+  mcl_rec(S.slop, kin_plane(P, K_VY), c, st, l.vy, r.vy);
+  mcl_rec(S.slop, kin_plane(P, K_VZ), c, st, l.vz, r.vz);
+  mcl_rec(S.slop, kin_plane(P, K_P), c, st, l.p, r.p);
+  mcl_rec(S.slop, kin_plane(P, K_RHO), c, st, l.rho, r.rho);
   Side<T>* sd[2] = {&L, &R};
   const McSide<T>* ms[2] = {&l, &r};
 #pragma unroll
@@ -279,8 +279,10 @@ __device__ inline void lax_cell_fluxes(const Geom& g, const Planes& P, const Sch
     for (int t = 0; t < 6; t++) out[t] = T(0);
     return;
   }
-  const T vx = (T)P.p[P_VX][k], vy = (T)P.p[P_VY][k], vz = (T)P.p[P_VZ][k];
-  const T p = (T)P.p[P_P][k], rho = (T)P.p[P_RHO][k];
+  // flowinit's cell fluxes: of v^n, p^n, rho^n, or with REC_PRIM = 1 of the
+  // stage's primitives (14_flowinit.f95:376-489)
+  const T vx = (T)kin_plane(P, K_VX)[k], vy = (T)kin_plane(P, K_VY)[k], vz = (T)kin_plane(P, K_VZ)[k];
+  const T p = (T)kin_plane(P, K_P)[k], rho = (T)kin_plane(P, K_RHO)[k];
   const T ex = (T)P.s[S_EX][k], ey = (T)P.s[S_EY][k], ez = (T)P.s[S_EZ][k];
   const T bx = (T)P.s[S_BX][k], by = (T)P.s[S_BY][k], bz = (T)P.s[S_BZ][k];
   const T w = T(1) / sqrt(T(1) - ((sq(vx) + sq(vy)) + sq(vz)));
@@ -421,16 +423,16 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   }
   Side<T> xl, xr, yl, yr;
   // kinematics of both faces first: the fallbacks couple them
-  face_rec<ORD>(P.p[P_VX], c, sx, xl.vx, xr.vx);
-  face_rec<ORD>(P.p[P_VY], c, sx, xl.vy, xr.vy);
-  face_rec<ORD>(P.p[P_VZ], c, sx, xl.vz, xr.vz);
-  face_rec<ORD>(P.p[P_VX], c, sy, yl.vx, yr.vx);
-  face_rec<ORD>(P.p[P_VY], c, sy, yl.vy, yr.vy);
-  face_rec<ORD>(P.p[P_VZ], c, sy, yl.vz, yr.vz);
-  face_rec_pos<ORD>(P.p[P_P], c, sx, xl.p, xr.p);
-  face_rec_pos<ORD>(P.p[P_RHO], c, sx, xl.rho, xr.rho);
-  face_rec_pos<ORD>(P.p[P_P], c, sy, yl.p, yr.p);
-  face_rec_pos<ORD>(P.p[P_RHO], c, sy, yl.rho, yr.rho);
+  face_rec<ORD>(kin_plane(P, K_VX), c, sx, xl.vx, xr.vx);
+  face_rec<ORD>(kin_plane(P, K_VY), c, sx, xl.vy, xr.vy);
+  face_rec<ORD>(kin_plane(P, K_VZ), c, sx, xl.vz, xr.vz);
+  face_rec<ORD>(kin_plane(P, K_VX), c, sy, yl.vx, yr.vx);
+  face_rec<ORD>(kin_plane(P, K_VY), c, sy, yl.vy, yr.vy);
+  face_rec<ORD>(kin_plane(P, K_VZ), c, sy, yl.vz, yr.vz);
+  face_rec_pos<ORD>(kin_plane(P, K_P), c, sx, xl.p, xr.p);
+  face_rec_pos<ORD>(kin_plane(P, K_RHO), c, sx, xl.rho, xr.rho);
+  face_rec_pos<ORD>(kin_plane(P, K_P), c, sy, yl.p, yr.p);
+  face_rec_pos<ORD>(kin_plane(P, K_RHO), c, sy, yl.rho, yr.rho);
   // superluminal fallback, 16_hll_flow_mpx.f95:1392-1461 (same in hllc;
   // hlle only prints, 16_hlle_flow_mpx.f95:2270-2300)
   if (SOLVER != 6) {

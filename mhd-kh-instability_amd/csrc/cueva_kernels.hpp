@@ -88,22 +88,42 @@ __global__ void k_varint(Geom g, Planes P, Scheme S) {
   for (int a = 0; a < A_N; a++) P.s[dst[a]][k] = un_value(P, a, k) + S.dt * P.ints[a][k];
 }
 
-// electricvarint (IMEX), 11_elecvarint.f95:279-328
+// electricvarint, 11_elecvarint.f95:210-328 (DIM = 2): the IMEX implicit
+// solve (:279-328), or the MIRK stage E (:210-277) from E^n, v^n, B^n, W^n
+// and the stage sums of the E equation
 __global__ void k_electricvarint(Geom g, Planes P, Scheme S, int l) {
   INTERIOR_IJ
   double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
   double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
+  if (S.rec_prim && !S.mirk) {
+    // REC_PRIM = 1 (:331-378): this stage's primitives as they stand before
+    // its own recovery -- from the previous step, zero in the first
+    vx = P.vint[l][0][k]; vy = P.vint[l][1][k]; vz = P.vint[l][2][k];
+    w = P.wint[l][k];
+  }
   double ex, ey, ez;
-  e_solve(S.Abt[l][l], S.sigma, S.dt, w, vx, vy, vz, P.s[S_BX][k], P.s[S_BY][k], P.s[S_BZ][k],
-          P.s[S_EXA][k], P.s[S_EYA][k], P.s[S_EZA][k], ex, ey, ez);
+  if (S.mirk) {
+    const double ea[3] = {P.s[S_EXA][k], P.s[S_EYA][k], P.s[S_EZA][k]};
+    const double acc[3] = {P.ints[A_EX][k], P.ints[A_EY][k], P.ints[A_EZ][k]};
+    mirk_stage_e(S, l, w, vx, vy, vz, P.p[P_EX][k], P.p[P_EY][k], P.p[P_EZ][k], P.p[P_BX][k], P.p[P_BY][k],
+                 P.p[P_BZ][k], ea, acc, ex, ey, ez);
+  } else {
+    e_solve(S.Abt[l][l], S.sigma, S.dt, w, vx, vy, vz, P.s[S_BX][k], P.s[S_BY][k], P.s[S_BZ][k],
+            P.s[S_EXA][k], P.s[S_EYA][k], P.s[S_EZA][k], ex, ey, ez);
+  }
   P.s[S_EX][k] = ex; P.s[S_EY][k] = ey; P.s[S_EZ][k] = ez;
 }
 
-// flowinit (current density), 14_flowinit.f95:258-301
-__global__ void k_flowinit(Geom g, Planes P, Scheme S) {
+// flowinit (current density), 14_flowinit.f95:258-301; with REC_PRIM = 1
+// from the stage primitives (:376-420)
+__global__ void k_flowinit(Geom g, Planes P, Scheme S, int l) {
   INTERIOR_IJ
   double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
   double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
+  if (S.rec_prim) {
+    vx = P.vint[l][0][k]; vy = P.vint[l][1][k]; vz = P.vint[l][2][k];
+    w = P.wint[l][k];
+  }
   double sx, sy, sz;
   ohm_source(S.sigma, w, vx, vy, vz, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k], P.s[S_BY][k],
              P.s[S_BZ][k], sx, sy, sz);
@@ -126,10 +146,33 @@ __global__ void k_sumfin(Geom g, Planes P, Scheme S, int l) {
   }
 }
 
-// varfin (19_varfin.f95:35-50) and electricfield (20_electricfield.f95:157-163);
-// which = 0: the 11 non-E fields, 1: E
+// varfin (19_varfin.f95:20-50) and electricfield (20_electricfield.f95);
+// which = 0: the 11 non-E fields, 1: E.  With MIRK, varfin keeps B^n and
+// E^n (Bxn..Ezn, 19_varfin.f95:24-30) in the stage-sum planes of B and E,
+// which no later part of the step reads, and electricfield applies the MIRK
+// update (:13-146) instead of E^n + dt finsum.
 __global__ void k_varfin(Geom g, Planes P, Scheme S, int which) {
   INTERIOR_IJ
+  if (S.mirk && which == 0)
+    for (int d = 0; d < 3; d++) {
+      P.ints[A_BX + d][k] = P.p[P_BX + d][k];
+      P.ints[A_EX + d][k] = P.p[P_EX + d][k];
+    }
+  if (S.mirk && which == 1) {
+    const int L = S.lmax - 1;
+    const double v[3] = {P.p[P_VX][k], P.p[P_VY][k], P.p[P_VZ][k]};
+    const double vl[3] = {S.rec_prim ? P.vint[L][0][k] : 0.0, S.rec_prim ? P.vint[L][1][k] : 0.0,
+                          S.rec_prim ? P.vint[L][2][k] : 0.0};
+    const double en[3] = {P.ints[A_EX][k], P.ints[A_EY][k], P.ints[A_EZ][k]};
+    const double bn[3] = {P.ints[A_BX][k], P.ints[A_BY][k], P.ints[A_BZ][k]};
+    const double b[3] = {P.p[P_BX][k], P.p[P_BY][k], P.p[P_BZ][k]};
+    const double acc[3] = {P.fins[A_EX][k], P.fins[A_EY][k], P.fins[A_EZ][k]};
+    double e[3];
+    mirk_final_e(S, P.c[C_W][k], v, vl, en, bn, b, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k],
+                 P.s[S_BY][k], P.s[S_BZ][k], acc, e, S.rec_prim ? P.wint[L] + k : nullptr);
+    P.p[P_EX][k] = e[0]; P.p[P_EY][k] = e[1]; P.p[P_EZ][k] = e[2];
+    return;
+  }
   for (int a = 0; a < A_N; a++) {
     bool e = (a >= A_EX && a <= A_EZ);
     if (e != (which == 1)) continue;
@@ -154,6 +197,59 @@ __global__ void k_varfin(Geom g, Planes P, Scheme S, int which) {
   }
 }
 
+// swap1, 08_swap1.f95:13-33: the step's primitives into the stage-1
+// primitives (interior), with W and the enthalpy as startconvar forms them
+__global__ void k_swap1(Geom g, Planes P, Scheme S) {
+#pragma clang fp contract(off)
+  INTERIOR_IJ
+  const double rho = P.p[P_RHO][k], p = P.p[P_P][k];
+  const double eps = p / ((S.gamma - 1.0) * rho);
+  P.rhoint[0][k] = rho;
+  P.pint[0][k] = p;
+  P.vint[0][0][k] = P.p[P_VX][k]; P.vint[0][1][k] = P.p[P_VY][k]; P.vint[0][2][k] = P.p[P_VZ][k];
+  P.wint[0][k] = P.c[C_W][k];
+  P.hint[0][k] = rho * (1.0 + eps) + p;
+}
+
+// varprimitivecardano with l_swap = 0 (12_varprimitive.f95:111-135, 792-806):
+// the recovery of stage l's state, started from that stage's primitives as
+// they stand (the previous step's, zero in the first); E is kept
+__global__ void k_c2p_stage(Geom g, Planes P, Scheme S, int l, StatusRef sr) {
+  if (halted(sr)) return;
+  INTERIOR_IJ
+  C2PIn in;
+  in.d = P.s[S_D][k]; in.tau = P.s[S_TAU][k];
+  in.sx = P.s[S_SX][k]; in.sy = P.s[S_SY][k]; in.sz = P.s[S_SZ][k];
+  in.ex = P.s[S_EX][k]; in.ey = P.s[S_EY][k]; in.ez = P.s[S_EZ][k];
+  in.bx = P.s[S_BX][k]; in.by = P.s[S_BY][k]; in.bz = P.s[S_BZ][k];
+  in.vx0 = P.vint[l][0][k]; in.vy0 = P.vint[l][1][k]; in.vz0 = P.vint[l][2][k];
+  in.rho_old = P.rhoint[l][k]; in.p_old = P.pint[l][k];
+  C2POut o;
+  const int code = c2p_cardano_full(S.gamma, S.eps1, in, o, S.crmath != 0);
+  if (code) { report(sr, code, i, j, 1); return; }
+  c2p_count(sr, i, j, o);
+  P.vint[l][0][k] = o.vx; P.vint[l][1][k] = o.vy; P.vint[l][2][k] = o.vz;
+  P.rhoint[l][k] = o.rho;
+  P.pint[l][k] = o.p;
+  P.wint[l][k] = 1.0 / sqrt(1.0 - ((sq(o.vx) + sq(o.vy)) + sq(o.vz)));
+}
+
+// interconvar, 13_intercon_var.f95:13-72: stage l's W, enthalpy and
+// conserved D tau S from its recovered primitives, E and B (interior)
+__global__ void k_interconvar(Geom g, Planes P, Scheme S, int l) {
+#pragma clang fp contract(off)
+  INTERIOR_IJ
+  const double vx = P.vint[l][0][k], vy = P.vint[l][1][k], vz = P.vint[l][2][k];
+  const double rho = P.rhoint[l][k], p = P.pint[l][k];
+  Cons cs = prim_to_cons(S.gamma, vx, vy, vz, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k], P.s[S_BX][k],
+                         P.s[S_BY][k], P.s[S_BZ][k], rho, p);
+  const double eps = p / ((S.gamma - 1.0) * rho);
+  P.wint[l][k] = cs.w;
+  P.hint[l][k] = rho * (1.0 + eps) + p;
+  P.s[S_D][k] = cs.d; P.s[S_TAU][k] = cs.tau;
+  P.s[S_SX][k] = cs.sx; P.s[S_SY][k] = cs.sy; P.s[S_SZ][k] = cs.sz;
+}
+
 // primitive recovery over the interior; reads D tau S and E B of the new step
 __device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, const StatusRef& sr, int i, int j,
                                  int64_t k) {
@@ -161,12 +257,19 @@ __device__ inline void c2p_store(const Geom& g, Planes& P, const Scheme& S, cons
   c2p_apply(P, S, sr, i, j, k, c2p_load(P, k), o);
 }
 
-// STAGE0: also the next step's stage-0 state (fused step)
+// STAGE0: also the next step's stage-0 state (fused step).  With REC_PRIM = 1
+// the recovery starts from the last stage's primitives (l_swap = 2,
+// 12_varprimitive.f95:59-83) instead of the step's.
 template <bool STAGE0 = false>
 __global__ void k_c2p(Geom g, Planes P, Scheme S, StatusRef sr) {
   if (halted(sr)) return;
   INTERIOR_IJ
-  const C2PIn in = c2p_load(P, k);
+  C2PIn in = c2p_load(P, k);
+  if (S.rec_prim) {
+    const int L = S.lmax - 1;
+    in.vx0 = P.vint[L][0][k]; in.vy0 = P.vint[L][1][k]; in.vz0 = P.vint[L][2][k];
+    in.rho_old = P.rhoint[L][k]; in.p_old = P.pint[L][k];
+  }
   C2POut o;
   if (c2p_apply(P, S, sr, i, j, k, in, o) && STAGE0) stage0_after_c2p(P, S, k, in, o);
 }

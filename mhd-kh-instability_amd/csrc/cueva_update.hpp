@@ -26,7 +26,9 @@ __device__ inline Rhs rhs_of(const Planes& P, const Scheme& S, int64_t k, const 
 #pragma clang fp contract(off)  // FP64: as the reference build, also inside the FP32 face unit
 #define DX(a) ((double)dfx[a])
 #define DY(a) ((double)dfy[a])
-  double vx = P.p[P_VX][k], vy = P.p[P_VY][k], vz = P.p[P_VZ][k];
+  // v^n, or with REC_PRIM = 1 the stage velocity (17_fluxandsource.f95:460-474);
+  // W as 06_conserved_var.f95 / 13_intercon_var.f95:27 form it from that velocity
+  double vx = kin_plane(P, K_VX)[k], vy = kin_plane(P, K_VY)[k], vz = kin_plane(P, K_VZ)[k];
   double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
   double q = P.s[S_Q][k];
   Rhs o;
@@ -51,6 +53,32 @@ __device__ inline Rhs rhs_of(const Planes& P, const Scheme& S, int64_t k, const 
   return o;
 }
 
+// The MIRK electric field of one interior point (20_electricfield.f95:13-146):
+// w = W^n, v = v^n, en / bn = E^n / B^n, b = B^{n+1}, ei* / bi* = the last
+// stage's E and B, vl = its velocity, acc = the three final sums; the new E
+// goes to e.  MIRK2 also stores the last stage's W (:71) to wl when given.
+__device__ inline void mirk_final_e(const Scheme& S, double w, const double* v, const double* vl, const double* en,
+                                    const double* bn, const double* b, double eix, double eiy, double eiz, double bix,
+                                    double biy, double biz, const double* acc, double* e, double* wl) {
+#pragma clang fp contract(off)
+  if (S.mirk == 1) {
+    const double sb = (S.dt * S.sigma) * w;
+    const double edv = (en[0] * v[0] + en[1] * v[1]) + en[2] * v[2];
+    const double rn[3] = {bn[2] * v[1] - bn[1] * v[2], bn[0] * v[2] - bn[2] * v[0], bn[1] * v[0] - bn[0] * v[1]};
+    const double r[3] = {b[2] * v[1] - b[1] * v[2], b[0] * v[2] - b[2] * v[0], b[1] * v[0] - b[0] * v[1]};
+    for (int d = 0; d < 3; d++) e[d] = mirk1_final_e(sb, S.cm2, S.dt, en[d], v[d], edv, rn[d], r[d], acc[d]);
+    return;
+  }
+  const double v2 = (sq(vl[0]) + sq(vl[1])) + sq(vl[2]);
+  const double wi = 1.0 / sqrt(1.0 - v2);
+  if (wl) *wl = wi;
+  const double sdb = (S.dt * S.sigma) * wi;
+  const double edv = (eix * vl[0] + eiy * vl[1]) + eiz * vl[2];
+  const double r[3] = {biz * vl[1] - biy * vl[2], bix * vl[2] - biz * vl[0], biy * vl[0] - bix * vl[1]};
+  const double ei[3] = {eix, eiy, eiz};
+  for (int d = 0; d < 3; d++) e[d] = mirk2_final_e(sdb, S.cm1, S.cm4, S.dt, ei[d], en[d], vl[d], edv, r[d], acc[d]);
+}
+
 // sumfin + varfin + electricfield of one interior point (18_sumfin.f95:20-60,
 // 19_varfin.f95:35-50, 20_electricfield.f95:157-163) with the last stage's
 // right-hand side in registers: U^{n+1} in A_* order.
@@ -64,6 +92,7 @@ __device__ inline void final_update_from(const Planes& P, const Scheme& S, int64
   Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, P.p[P_RHO][k], P.p[P_P][k]);
   double un[A_N] = {P.p[P_PSI][k], P.p[P_PHI][k], bx0, by0, bz0, ex0, ey0, ez0, P.p[P_Q][k],
                     cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
+  double acc_e[3];
 #pragma unroll
   for (int a = 0; a < A_N; a++) {
     double acc = 0.0;
@@ -77,6 +106,16 @@ __device__ inline void final_update_from(const Planes& P, const Scheme& S, int64
       }
     }
     nu[a] = un[a] + S.dt * acc;
+    if (a >= A_EX && a <= A_EZ) acc_e[a - A_EX] = acc;
+  }
+  if (S.mirk) {
+    const double ve[3] = {vx, vy, vz};
+    // the last stage's velocity: the stage primitives with REC_PRIM = 1, else
+    // the never-written stage array (zero)
+    const double vl[3] = {S.rec_prim ? P.vint[L][0][k] : 0.0, S.rec_prim ? P.vint[L][1][k] : 0.0,
+                          S.rec_prim ? P.vint[L][2][k] : 0.0};
+    mirk_final_e(S, cs.w, ve, vl, un + A_EX, un + A_BX, nu + A_BX, P.s[S_EX][k], P.s[S_EY][k], P.s[S_EZ][k],
+                 P.s[S_BX][k], P.s[S_BY][k], P.s[S_BZ][k], acc_e, nu + A_EX, S.rec_prim ? P.wint[L] + k : nullptr);
   }
 }
 
@@ -87,6 +126,29 @@ __device__ inline void store_update(Planes& P, int64_t k, const double* nu) {
   P.p[P_Q][k] = nu[A_Q];
   P.c[C_D][k] = nu[A_D]; P.c[C_TAU][k] = nu[A_TAU];
   P.c[C_SX][k] = nu[A_SX]; P.c[C_SY][k] = nu[A_SY]; P.c[C_SZ][k] = nu[A_SZ];
+}
+
+// The MIRK stage E (11_elecvarint.f95:210-277, DIM = 2): MIRK1 E^n; MIRK2
+// the explicit E* in the first stage (ea), later the linearly implicit
+// update from E^n, v^n, B^n, W^n and the stage sums acc.
+__device__ inline void mirk_stage_e(const Scheme& S, int l, double w, double vx, double vy, double vz, double ex0,
+                                    double ey0, double ez0, double bx0, double by0, double bz0, const double* ea,
+                                    const double* acc, double& ex, double& ey, double& ez) {
+#pragma clang fp contract(off)
+  if (S.mirk == 1) {
+    ex = ex0; ey = ey0; ez = ez0;
+    return;
+  }
+  if (l == 0) {
+    ex = ea[0]; ey = ea[1]; ez = ea[2];
+    return;
+  }
+  const double sb = (S.dt * S.sigma) * w;
+  const double edv = (ex0 * vx + ey0 * vy) + ez0 * vz;
+  const double rx = bz0 * vy - by0 * vz, ry = bx0 * vz - bz0 * vx, rz = by0 * vx - bx0 * vy;
+  ex = mirk2_stage_e(sb, S.cm1, S.dt, ex0, vx, edv, rx, acc[0]);
+  ey = mirk2_stage_e(sb, S.cm1, S.dt, ey0, vy, edv, ry, acc[1]);
+  ez = mirk2_stage_e(sb, S.cm1, S.dt, ez0, vz, edv, rz, acc[2]);
 }
 
 // Stage-l state of one point from its U^n primitives (values) and the
@@ -100,7 +162,7 @@ __device__ inline void stage_point(const Planes& P, const Scheme& S, int l, int6
 #pragma clang fp contract(off)  // FP64: as the reference build, also inside the FP32 face unit
   Cons cs = prim_to_cons(S.gamma, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, rho, p);
   double un[A_N] = {psi, phi, bx0, by0, bz0, ex0, ey0, ez0, q, cs.d, cs.tau, cs.sx, cs.sy, cs.sz};
-  double stv[A_N];
+  double stv[A_N], acc_e[3];
 #pragma unroll
   for (int a = 0; a < A_N; a++) {
     double acc = 0.0;
@@ -113,10 +175,15 @@ __device__ inline void stage_point(const Planes& P, const Scheme& S, int l, int6
           acc = acc + S.Ab[lp][m] * f;
       }
     stv[a] = un[a] + S.dt * acc;
+    if (a >= A_EX && a <= A_EZ) acc_e[a - A_EX] = acc;
   }
   double ex, ey, ez;
-  e_solve(S.Abt[l][l], S.sigma, S.dt, cs.w, vx, vy, vz, stv[A_BX], stv[A_BY], stv[A_BZ], stv[A_EX], stv[A_EY],
-          stv[A_EZ], ex, ey, ez);
+  if (S.mirk == 0) {
+    e_solve(S.Abt[l][l], S.sigma, S.dt, cs.w, vx, vy, vz, stv[A_BX], stv[A_BY], stv[A_BZ], stv[A_EX], stv[A_EY],
+            stv[A_EZ], ex, ey, ez);
+  } else {
+    mirk_stage_e(S, l, cs.w, vx, vy, vz, ex0, ey0, ez0, bx0, by0, bz0, stv + A_EX, acc_e, ex, ey, ez);
+  }
   double jx, jy, jz;
   ohm_source(S.sigma, cs.w, vx, vy, vz, ex, ey, ez, stv[A_BX], stv[A_BY], stv[A_BZ], jx, jy, jz);
   jx = jx + stv[A_Q] * vx;

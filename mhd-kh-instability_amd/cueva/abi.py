@@ -55,7 +55,7 @@ class Params(ctypes.Structure):
         ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("c2p_mode", ctypes.c_int32), ("hllc_seed", ctypes.c_int32), ("crmath", ctypes.c_int32),
         ("primitive_rec", ctypes.c_int32), ("slop", ctypes.c_int32), ("limit", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("mirk", ctypes.c_int32), ("rec_prim", ctypes.c_int32),
         ("longx", ctypes.c_double), ("longy", ctypes.c_double), ("cfl", ctypes.c_double),
         ("sigma", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
         ("reserved_d", ctypes.c_double * 6),
@@ -82,6 +82,8 @@ SIGNATURES = {
     "cueva_startconvar": (_i, [_c]),
     "cueva_cleanvarint": (_i, [_c]),
     "cueva_swap1": (_i, [_c]),
+    "cueva_c2p_stage": (_i, [_c, _i]),
+    "cueva_interconvar": (_i, [_c, _i]),
     "cueva_sumint": (_i, [_c, _i, _i]),
     "cueva_varint": (_i, [_c, _i]),
     "cueva_electricvarint": (_i, [_c, _i]),
@@ -398,7 +400,7 @@ class Solver:
         returns the fix-up counters of the primitive recovery."""
         L = self.lib
         x = self.ctx
-        lmax = int(self.p.order)  # stages of SSP2(2,2,2), SSP2(3,3,2)-LUM, SSP3(4,3,3)
+        lmax = int(self.p.order)  # stages = order
         check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
         if self.p.primitive_rec:  # main_cueva_code.f95:170-175
             check(L.cueva_recons_var_primitive(x), "recons_var_primitive")
@@ -411,6 +413,9 @@ class Solver:
                 check(L.cueva_sumint(x, l, m), "sumint")
             check(L.cueva_varint(x, l), "varint")
             check(L.cueva_electricvarint(x, l), "electricvarint")
+            if self.p.rec_prim:  # main_cueva_code.f95:220-228
+                check(L.cueva_c2p_stage(x, l), "varprimitivecardano (l_swap = 0)")
+                check(L.cueva_interconvar(x, l), "interconvar")
             check(L.cueva_boundary(x, BND_CONS, l), "boundary_conserved")
             check(L.cueva_boundary(x, BND_ELEC, l), "boundary_electric")
             check(L.cueva_flowinit(x, l), "flowinit")
@@ -422,6 +427,6 @@ class Solver:
         check(L.cueva_electricfield(x), "electricfield")
         cnt = (ctypes.c_int32 * 3)()
         fb = ctypes.c_int32()
-        check(L.cueva_c2p(x, 1, cnt, ctypes.byref(fb)), "varprimitivecardano")
+        check(L.cueva_c2p(x, 2 if self.p.rec_prim else 1, cnt, ctypes.byref(fb)), "varprimitivecardano")
         check(L.cueva_boundary(x, BND_PRIM, 1), "boundary")
         return list(cnt)

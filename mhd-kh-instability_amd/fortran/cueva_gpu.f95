@@ -22,7 +22,7 @@ module cueva_gpu
     integer(c_int32_t) :: precision, c2p_mode, hllc_seed, crmath
     integer(c_int32_t) :: primitive_rec
     integer(c_int32_t) :: slop, limit
-    integer(c_int32_t) :: reserved(2)
+    integer(c_int32_t) :: mirk, rec_prim
     real(c_double) :: longx, longy, cfl, sigma, gamma, dt
     real(c_double) :: reserved_d(6)
   end type cueva_params
@@ -126,6 +126,16 @@ module cueva_gpu
     integer(c_int) function cueva_electricfield(ctx) bind(C, name="cueva_electricfield")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function cueva_c2p_stage(ctx, l) bind(C, name="cueva_c2p_stage")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: l
+    end function
+    integer(c_int) function cueva_interconvar(ctx, l) bind(C, name="cueva_interconvar")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: l
     end function
     integer(c_int) function cueva_c2p(ctx, l_swap, counters, fallback) bind(C, name="cueva_c2p")
       import :: c_int, c_ptr, c_int32_t

@@ -28,6 +28,8 @@ contains
     prm%primitive_rec = primitive_rec
     prm%slop = slop
     prm%limit = limit
+    prm%mirk = MIRK
+    prm%rec_prim = REC_PRIM
     prm%longx = Longx
     prm%longy = Longy
     prm%cfl = CFL
@@ -46,8 +48,11 @@ contains
       call refuse("BOUND", BOUND)
     end select
     if (DIM /= 2) call refuse("DIM", DIM)
-    if (MIRK /= 0) call refuse("MIRK", MIRK)
-    if (REC_PRIM /= 0) call refuse("REC_PRIM", REC_PRIM)
+    if (MIRK < 0 .or. MIRK > 2 .or. (order == 2 .and. MIRK == 1)) call refuse("MIRK", MIRK)
+    ! REC_PRIM = 1 with the second-order HLL/HLLE/HLLC reads an unset
+    ! z-velocity in the reference (16_hll_flow_mcl.f95:911-958)
+    if (REC_PRIM < 0 .or. REC_PRIM > 1 .or. (REC_PRIM == 1 .and. flux_solver >= 2 .and. flux_solver <= 4)) &
+      call refuse("REC_PRIM", REC_PRIM)
     if (REC_ELEC /= 0) call refuse("REC_ELEC", REC_ELEC)
     if (VDRIFT /= 0) call refuse("VDRIFT", VDRIFT)
     if (flux_rec_mp5 /= 0) call refuse("flux_rec_mp5", flux_rec_mp5)
@@ -56,7 +61,7 @@ contains
     if (low_vel /= 0) call refuse("low_vel", low_vel)
     if (SLC /= 0) call refuse("SLC", SLC)
     if (source_rec /= 0 .or. source_rec_mpx /= 0 .or. var_source_rec /= 0) call refuse("source_rec", 1)
-    if (order < 2 .or. order > 4) call refuse("order", order)
+    if (order < 1 .or. order > 4) call refuse("order", order)
   contains
     subroutine refuse(what, val)
       character(len=*), intent(in) :: what

@@ -9,6 +9,9 @@ module cueva_shims
   implicit none
   integer(c_int32_t) :: hij_dummy(3)
   real(c_double) :: vals_dummy(2)
+  ! fix-up records already written for step fix_h, per kind: with REC_PRIM = 1
+  ! a step has several recoveries, and each writes its own records
+  integer :: fix_h = -huge(1), fix_done(0:2) = 0
 contains
 
   subroutine check(rc, what)
@@ -28,11 +31,11 @@ contains
     end if
   end subroutine check
 
-  ! The fix-up records of this step's recovery in the reference's files
+  ! The fix-up records of the recovery just run in the reference's files
   ! (units 301-303, opened by the driver as 01_openfiles.f95:115-117 does),
-  ! with the headers it writes at h = 1 and the record of
-  ! 12_varprimitive.f95:544,601,663: h, h*Delt, running count, i, j, l, x(i),
-  ! y(j) and the two values.
+  ! with the headers every recovery writes at h = 1 and the record of
+  ! 12_varprimitive.f95:544,601,663: h, h*Delt, running count in the step, i,
+  ! j, l, x(i), y(j) and the two values.
   subroutine write_fixups
     integer :: kind, u, e
     integer(c_int64_t) :: n
@@ -46,17 +49,22 @@ contains
       write(303,*) "iteration    Delt     contador   i    j     RK-step     x(i)     y(j)   v*^2  v_drift^2 "
       write(303,*) "========================================================================================"
     end if
+    if (h /= fix_h) then
+      fix_h = h
+      fix_done = 0
+    end if
     do kind = 0, 2
       u = 301 + kind
       call check(cueva_fixup_records(ctx, int(kind, c_int), hij_dummy, vals_dummy, 0_c_int64_t, n), "fixup_records")
-      if (n == 0) cycle
+      if (n <= fix_done(kind)) cycle
       allocate(hij(3 * n), vals(2 * n))
       call check(cueva_fixup_records(ctx, int(kind, c_int), hij, vals, n, n), "fixup_records")
-      do e = 1, int(n)
+      do e = fix_done(kind) + 1, int(n)
         write(u,*) h, h * Delt, e, hij(3 * e - 1), hij(3 * e), l, 0.5d0 * prm%longx - hij(3 * e - 1) * Delx, &
                    hij(3 * e) * Dely, vals(2 * e - 1), vals(2 * e)
       end do
       deallocate(hij, vals)
+      fix_done(kind) = int(n)
     end do
   end subroutine write_fixups
 
@@ -153,14 +161,27 @@ contains
     call check(cueva_electricfield(ctx), "electricfield")
   end subroutine electricfield
 
-  subroutine varprimitivecardano      ! 12_varprimitive.f95:6 (counters of this call)
-    integer(c_int32_t) :: cnt(3), fb
+  ! 12_varprimitive.f95:6.  l_swap = 0: stage l's state (REC_PRIM = 1,
+  ! main_cueva_code.f95:220-224); 1 or 2 after varfin, whose counters are
+  ! those of the step (all its recoveries)
+  subroutine varprimitivecardano
+    integer(c_int32_t) :: cnt(3), fb, where(4)
+    if (l_swap == 0) then
+      call check(cueva_c2p_stage(ctx, int(l, c_int)), "varprimitivecardano")
+      call check(cueva_last_error(ctx, where), "varprimitivecardano")   ! waits: a stop here stops
+      call write_fixups
+      return
+    end if
     call check(cueva_c2p(ctx, int(l_swap, c_int), cnt, fb), "varprimitivecardano")
     call write_fixups
     contador_r = contador_r + cnt(1)
     contador_p = contador_p + cnt(2)
     contador_v = contador_v + cnt(3)
   end subroutine varprimitivecardano
+
+  subroutine interconvar              ! 13_intercon_var.f95:6
+    call check(cueva_interconvar(ctx, int(l, c_int)), "interconvar")
+  end subroutine interconvar
 
   ! the global block of lector (22_writer.f95:818-991): one line per call
   ! with h, t and the 37 diagnostics in the reference's unit order 80..217

@@ -3,7 +3,7 @@
 !
 !   main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED
 !                  V0 BX0 BZ0 SEED HGLB OUTPREFIX [HLOC [RESTART_IN [HINI
-!                  [PRIMITIVE_REC [SLOP [LIMIT]]]]]]
+!                  [PRIMITIVE_REC [SLOP [LIMIT [MIRK [REC_PRIM]]]]]]]]
 !
 ! IMAX x JMAX is the native KH grid (x periodic, y outflow); ROTATED = 1 runs
 ! it in the frame x' = y, y' = -x (the reference's BOUND 6).  Writes
@@ -25,8 +25,9 @@
 !   V0 BX0 BZ0 SEED are then unused.
 ! HINI: number of the first step, as the reference's hini (default 1): steps
 !   HINI .. HINI+NSTEPS-1 run, from t = (HINI-1) Delt.
-! PRIMITIVE_REC, SLOP, LIMIT: the reference's primitive_rec, slop and limit
-!   (parameters.f95:17-18,26; defaults 0, 2, 2).
+! PRIMITIVE_REC, SLOP, LIMIT, MIRK, REC_PRIM: the reference's primitive_rec,
+!   slop, limit, MIRK and REC_PRIM (parameters.f95:11,17-18,23,26; defaults
+!   0, 2, 2, 0, 0); MIRK 1 runs with ORDER 1, MIRK 2 with ORDER 2.
 program main_cueva_gpu
   use iso_c_binding
   use cueva_gpu
@@ -34,7 +35,7 @@ program main_cueva_gpu
   use cueva_shims
   use cueva_io
   implicit none
-  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini, primrec, slop, limit
+  integer :: ni, nj, nsteps, fs, rec, order, rotated, nplane, ios, nargs, hloc, hini, primrec, slop, limit, mirk, recprim
   integer(c_int64_t) :: seed
   real(c_double) :: sigma, v0, bx0, bz0, dv(5), g(37)
   character(len=512) :: arg, prefix, restart_in
@@ -43,9 +44,9 @@ program main_cueva_gpu
   real(c_double), allocatable :: prims(:), cons(:)
 
   nargs = command_argument_count()
-  if (nargs < 14 .or. nargs > 20) then
+  if (nargs < 14 .or. nargs > 22) then
     write(*,*) "usage: main_cueva_gpu IMAX JMAX NSTEPS FLUX_SOLVER REC_MP ORDER SIGMA ROTATED V0 BX0 BZ0 SEED HGLB ", &
-               "OUTPREFIX [HLOC [RESTART_IN [HINI [PRIMITIVE_REC [SLOP [LIMIT]]]]]]"
+               "OUTPREFIX [HLOC [RESTART_IN [HINI [PRIMITIVE_REC [SLOP [LIMIT [MIRK [REC_PRIM]]]]]]]]"
     stop 2
   end if
   call get_command_argument(1, arg); read(arg, *) ni
@@ -84,6 +85,14 @@ program main_cueva_gpu
   if (nargs >= 20) then
     call get_command_argument(20, arg); read(arg, *) limit
   end if
+  mirk = 0
+  if (nargs >= 21) then
+    call get_command_argument(21, arg); read(arg, *) mirk
+  end if
+  recprim = 0
+  if (nargs >= 22) then
+    call get_command_argument(22, arg); read(arg, *) recprim
+  end if
 
   call check(cueva_params_default(prm), "params_default")
   if (rotated == 1) then
@@ -96,9 +105,9 @@ program main_cueva_gpu
     prm%longx = 1.d0; prm%longy = 2.d0
   end if
   prm%flux_solver = fs; prm%rec_mp = rec; prm%order = order; prm%primitive_rec = primrec
-  prm%slop = slop; prm%limit = limit
+  prm%slop = slop; prm%limit = limit; prm%mirk = mirk; prm%rec_prim = recprim
   prm%sigma = sigma; prm%cfl = 0.4d0
-  lmax = order   ! SSP2(2,2,2), SSP2(3,3,2)-LUM, SSP3(4,3,3)
+  lmax = order   ! stages: first order, SSP2(2,2,2) or MIRK2, SSP2(3,3,2)-LUM, SSP3(4,3,3)
   hmax = nsteps
   call check(cueva_derived(prm, dv), "derived")
   Delx = dv(1); Dely = dv(2); Delt = dv(3)
@@ -147,6 +156,11 @@ program main_cueva_gpu
       end do
       call varint
       call electricvarint
+      if (prm%rec_prim == 1) then        ! main_cueva_code.f95:220-228
+        l_swap = 0
+        call varprimitivecardano
+        call interconvar
+      end if
       call boundary_conserved
       call boundary_electric
       call flowinit
@@ -172,7 +186,11 @@ program main_cueva_gpu
     l = lmax
     call varfin
     call electricfield
-    l_swap = 1
+    if (prm%rec_prim == 1) then          ! main_cueva_code.f95:301-309
+      l_swap = 2
+    else
+      l_swap = 1
+    end if
     call varprimitivecardano
     call boundary
     t_advance = t_advance + Delt

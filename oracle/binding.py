@@ -21,7 +21,7 @@ class Cfg(ctypes.Structure):
                 ("wb", ctypes.c_double * 8), ("wbt", ctypes.c_double * 8), ("Ab", ctypes.c_double * 64),
                 ("Abt", ctypes.c_double * 64), ("primitive_rec", ctypes.c_int32), ("hllc_seed", ctypes.c_int32),
                 ("libm_mode", ctypes.c_int32), ("slop", ctypes.c_int32), ("limit", ctypes.c_int32),
-                ("pad1", ctypes.c_int32)]
+                ("mirk", ctypes.c_int32), ("rec_prim", ctypes.c_int32), ("pad1", ctypes.c_int32)]
 
 
 _P = ctypes.POINTER(ctypes.c_double)
@@ -37,6 +37,7 @@ def load():
         lib.oracle_default_cfg.argtypes = [ctypes.POINTER(Cfg), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        lib.oracle_scheme_tables.argtypes = [ctypes.POINTER(Cfg)]
         lib.oracle_create.restype = ctypes.c_void_p
         lib.oracle_create.argtypes = [ctypes.POINTER(Cfg)]
         lib.oracle_destroy.argtypes = [ctypes.c_void_p]
@@ -86,6 +87,11 @@ class Oracle:
         self.c.primitive_rec = int(params.primitive_rec)
         self.c.slop = int(getattr(params, "slop", 2) or 2)
         self.c.limit = int(getattr(params, "limit", 2) or 2)
+        self.c.mirk = int(getattr(params, "mirk", 0))
+        self.c.rec_prim = int(getattr(params, "rec_prim", 0))
+        if lib.oracle_scheme_tables(ctypes.byref(self.c)):
+            raise ValueError("oracle: order %d with MIRK = %d is not a scheme of the reference"
+                             % (params.order, self.c.mirk))
         if libm is None:
             libm = "cr" if params.crmath else "c"
         self.c.libm_mode = 1 if libm == "cr" else 0

@@ -95,7 +95,26 @@ struct oracle_state {
   int cur_l;
   int32_t *fix_ij[3];
   double *fix_val[3];
+  /* B^n and E^n kept by varfin for the MIRK electric field (19_varfin.f95:24-30) */
+  double *bn[3], *en[3];
+  /* stage primitives Vxint..Vzint, Wint, rhoint, pint, enthpyint (fourvectors.f95:12-13):
+   * static arrays of the reference, zero until written, kept from step to step */
+  double *vint[MAXL][3], *wint[MAXL], *rhoint[MAXL], *pint[MAXL], *hint[MAXL];
+  /* the velocity, pressure and density the face solves reconstruct and the
+   * stage right-hand side reads (K_VX..K_RHO): the step's primitives, or
+   * with REC_PRIM = 1 the current stage's (set per stage by oracle_step) */
+  const double *kin[5];
 };
+enum { K_VX, K_VY, K_VZ, K_P, K_RHO };
+
+static void set_kin(oracle_state *s, int l) {
+  const int rp = s->c.rec_prim && l >= 0;
+  s->kin[K_VX] = rp ? s->vint[l][0] : s->prim[OR_VX];
+  s->kin[K_VY] = rp ? s->vint[l][1] : s->prim[OR_VY];
+  s->kin[K_VZ] = rp ? s->vint[l][2] : s->prim[OR_VZ];
+  s->kin[K_P] = rp ? s->pint[l] : s->prim[OR_P];
+  s->kin[K_RHO] = rp ? s->rhoint[l] : s->prim[OR_RHO];
+}
 
 #define IX(s, i, j) (((size_t)((j) + G)) * (size_t)(s)->nx + (size_t)((i) + G))
 #define AT(s, a, i, j) ((a)[IX(s, i, j)])
@@ -104,8 +123,12 @@ static inline double sq(double x) { return x * x; }
 static inline double pw3(double x) { return x * (x * x); }
 static inline double pw4(double x) { return x * (x * (x * x)); }
 static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
-static inline double fmin2(double a, double b) { return b < a ? b : a; }
-static inline double fmax2(double a, double b) { return b > a ? b : a; }
+/* Fortran min / max as flang evaluates them (probed with the flang of this
+ * image): the first argument when it compares strictly less (greater), else
+ * the second -- so min(NaN, x) = x, min(x, NaN) = NaN, min(-0, +0) = +0;
+ * min(a, b, c) = min(min(a, b), c) */
+static inline double fmin2(double a, double b) { return a < b ? a : b; }
+static inline double fmax2(double a, double b) { return a > b ? a : b; }
 
 /* ------------------------------------------------------------------ */
 /* configuration                                                       */
@@ -116,7 +139,7 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
                        double cfl) {
   memset(c, 0, sizeof(*c));
   if (imax < 8 || jmax < 8) return 1;
-  if (lmax < 2 || lmax > 4) return 2;
+  if (lmax < 1 || lmax > 4) return 2;
   if (flux_solver != OR_HLL && flux_solver != OR_HLLE && flux_solver != OR_HLLC && flux_solver != OR_HLL_MCL &&
       flux_solver != OR_LAX && flux_solver != OR_HLLC_MCL && flux_solver != OR_HLLE_MCL)
     return 3;
@@ -144,7 +167,26 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
   double cp = sqrt(fmin2(c->dx, c->dy) / 1.0);
   c->kappa_psi = 1.0 / (cp * cp);
   c->kappa_phi = 1.0 / (cp * cp);
-  if (lmax == 2) { /* imex_scheme.f95:40-49, const = 1 - 1/sqrt(2) (parameters.f95:240) */
+  c->mirk = 0;
+  c->rec_prim = 0;
+  return oracle_scheme_tables(c) ? 2 : 0;
+}
+
+int oracle_scheme_tables(oracle_cfg *c) {
+  const int lmax = c->lmax;
+  memset(c->wb, 0, sizeof(c->wb));
+  memset(c->wbt, 0, sizeof(c->wbt));
+  memset(c->Ab, 0, sizeof(c->Ab));
+  memset(c->Abt, 0, sizeof(c->Abt));
+  if (c->mirk < 0 || c->mirk > 2) return 1;
+  if (lmax == 1) { /* imex_scheme.f95:13-20: the first-order scheme of MIRK = 1 (and IMEX order 1) */
+    c->wb[0] = 1.0;
+    c->Ab[0][0] = 1.0;
+  } else if (lmax == 2 && c->mirk == 2) { /* MIRK2, imex_scheme.f95:24-36 */
+    c->wb[0] = 0.5; c->wb[1] = 0.5;
+    c->Ab[1][0] = 1.0;
+  } else if (lmax == 2) { /* imex_scheme.f95:40-49, const = 1 - 1/sqrt(2) (parameters.f95:240) */
+    if (c->mirk != 0) return 1; /* imex_scheme.f95:51-56: the reference stops */
     double g = 1.0 - 1.0 / sqrt(2.0);
     c->wb[0] = 0.5; c->wb[1] = 0.5;
     c->wbt[0] = 0.5; c->wbt[1] = 0.5;
@@ -159,7 +201,7 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
     c->Abt[0][0] = 1.0 / 5.0;
     c->Abt[1][0] = 1.0 / 10.0; c->Abt[1][1] = 1.0 / 5.0;
     c->Abt[2][0] = 1.0 / 3.0; c->Abt[2][1] = 1.0 / 3.0; c->Abt[2][2] = 1.0 / 3.0;
-  } else { /* imex_scheme.f95:280-291 with parameters.f95:242-244 */
+  } else if (lmax == 4) { /* imex_scheme.f95:280-291 with parameters.f95:242-244 */
     double a1 = 0.24169426078821, b1 = 0.06042356519705, e1 = 0.12915286960590;
     c->wb[1] = 1.0 / 6.0; c->wb[2] = 1.0 / 6.0; c->wb[3] = 2.0 / 3.0;
     c->wbt[1] = 1.0 / 6.0; c->wbt[2] = 1.0 / 6.0; c->wbt[3] = 2.0 / 3.0;
@@ -169,6 +211,8 @@ int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly, 
     c->Abt[1][0] = -1.0 * a1; c->Abt[1][1] = a1;
     c->Abt[2][1] = 1.0 - a1; c->Abt[2][2] = a1;
     c->Abt[3][0] = b1; c->Abt[3][1] = e1; c->Abt[3][2] = ((0.5 - b1) - e1) - a1; c->Abt[3][3] = a1;
+  } else {
+    return 1;
   }
   return 0;
 }
@@ -193,6 +237,14 @@ oracle_state *oracle_create(const oracle_cfg *c) {
   for (int k = 0; k < FY_N; k++) s->fy[k] = plane(s);
   for (int l = 0; l < c->lmax; l++)
     for (int k = 0; k < NRHS; k++) s->rhs[l][k] = plane(s);
+  set_kin(s, -1);
+  if (c->mirk)
+    for (int d = 0; d < 3; d++) { s->bn[d] = plane(s); s->en[d] = plane(s); }
+  if (c->mirk == 2 || c->rec_prim)
+    for (int l = 0; l < c->lmax; l++) {
+      for (int d = 0; d < 3; d++) s->vint[l][d] = plane(s);
+      s->wint[l] = plane(s); s->rhoint[l] = plane(s); s->pint[l] = plane(s); s->hint[l] = plane(s);
+    }
   return s;
 }
 
@@ -208,7 +260,11 @@ void oracle_destroy(oracle_state *s) {
   for (int k = 0; k < FY_N; k++) free(s->fy[k]);
   for (int l = 0; l < s->c.lmax; l++)
     for (int k = 0; k < NRHS; k++) free(s->rhs[l][k]);
-  for (int k = 0; k < 3; k++) { free(s->fix_ij[k]); free(s->fix_val[k]); }
+  for (int k = 0; k < 3; k++) { free(s->fix_ij[k]); free(s->fix_val[k]); free(s->bn[k]); free(s->en[k]); }
+  for (int l = 0; l < MAXL; l++) {
+    for (int d = 0; d < 3; d++) free(s->vint[l][d]);
+    free(s->wint[l]); free(s->rhoint[l]); free(s->pint[l]); free(s->hint[l]);
+  }
   free(s->fe_kind);
   free(s->fe_val);
   free(s);
@@ -375,14 +431,61 @@ static void varint(oracle_state *s) {
   for (int k = 0; k < 11; k++) fill_ghosts(s, s->st[cons[k]]);
 }
 
-/* electricvarint, IMEX branch: 11_elecvarint.f95:279-328 */
+/* MIRK coefficients, parameters.f95:233-235: cm_1 = -0.05, cm_2 = 1,
+ * cm_4 = 0.5 (1 - cm_1)**2 / cm_1 */
+static const double CM1 = -0.05, CM2 = 1.0;
+static double cm4(void) { return (0.5 * sq(1.0 - CM1)) / CM1; }
+
+/* electricvarint, MIRK branches (DIM = 2): 11_elecvarint.f95:210-277.
+ * MIRK = 1: the stage E is E^n.  MIRK = 2: stage 1 takes the explicit
+ * value E*; later stages the linearly implicit update from E^n, v^n, B^n,
+ * W^n and the stage sum of the E equation. */
+static void electricvarint_mirk(oracle_state *s, int l) {
+  const double sig = s->c.sigma, dt = s->c.dt;
+  double **P = s->prim, **S = s->st;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      if (s->c.mirk == 1) {
+        S[ST_EX][k] = P[OR_EX][k]; S[ST_EY][k] = P[OR_EY][k]; S[ST_EZ][k] = P[OR_EZ][k];
+        continue;
+      }
+      if (l == 0) {
+        S[ST_EX][k] = S[ST_EXA][k]; S[ST_EY][k] = S[ST_EYA][k]; S[ST_EZ][k] = S[ST_EZA][k];
+        continue;
+      }
+      double w = s->W[k], vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+      double ex = P[OR_EX][k], ey = P[OR_EY][k], ez = P[OR_EZ][k];
+      double bx = P[OR_BX][k], by = P[OR_BY][k], bz = P[OR_BZ][k];
+      double sb = (dt * sig) * w;
+      double edv = (ex * vx + ey * vy) + ez * vz;
+      double rx = bz * vy - by * vz, ry = bx * vz - bz * vx, rz = by * vx - bx * vy;
+      double inv = 1.0 / (1.0 + sb * (1.0 - CM1));
+      S[ST_EX][k] = inv * ((((ex * (1.0 - sb * CM1)) + (sb * vx) * edv) - sb * rx) + dt * s->intsum[AC_EX][k]);
+      S[ST_EY][k] = inv * ((((ey * (1.0 - sb * CM1)) + (sb * vy) * edv) - sb * ry) + dt * s->intsum[AC_EY][k]);
+      S[ST_EZ][k] = inv * ((((ez * (1.0 - sb * CM1)) + (sb * vz) * edv) - sb * rz) + dt * s->intsum[AC_EZ][k]);
+    }
+  fill_ghosts(s, S[ST_EX]);
+  fill_ghosts(s, S[ST_EY]);
+  fill_ghosts(s, S[ST_EZ]);
+}
+
+/* electricvarint, IMEX branch: 11_elecvarint.f95:279-328 (REC_PRIM = 1: :331-378) */
 static void electricvarint(oracle_state *s, int l) {
+  if (s->c.mirk) {
+    electricvarint_mirk(s, l);
+    return;
+  }
   const double abt = s->c.Abt[l][l], sig = s->c.sigma, dt = s->c.dt;
   double **P = s->prim, **S = s->st;
   for (int j = 0; j <= s->c.jmax; j++)
     for (int i = 0; i <= s->c.imax; i++) {
       size_t k = IX(s, i, j);
       double w = s->W[k], vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+      if (s->c.rec_prim) { /* :331-378: the stage's primitives before its recovery */
+        w = s->wint[l][k];
+        vx = s->vint[l][0][k]; vy = s->vint[l][1][k]; vz = s->vint[l][2][k];
+      }
       double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k];
       double eax = S[ST_EXA][k], eay = S[ST_EYA][k], eaz = S[ST_EZA][k];
       double a = (abt * sig) * dt;
@@ -408,14 +511,19 @@ static void electricvarint(oracle_state *s, int l) {
   fill_ghosts(s, S[ST_EZ]);
 }
 
-/* flowinit, current density only: 14_flowinit.f95:258-301 */
-static void flowinit(oracle_state *s) {
+/* flowinit, current density only: 14_flowinit.f95:258-301 (REC_PRIM = 1:
+ * :376-420, from the stage's primitives) */
+static void flowinit(oracle_state *s, int l) {
   const double sig = s->c.sigma;
   double **P = s->prim, **S = s->st;
   for (int j = 0; j <= s->c.jmax; j++)
     for (int i = 0; i <= s->c.imax; i++) {
       size_t k = IX(s, i, j);
       double w = s->W[k], vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+      if (s->c.rec_prim) {
+        w = s->wint[l][k];
+        vx = s->vint[l][0][k]; vy = s->vint[l][1][k]; vz = s->vint[l][2][k];
+      }
       double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
       double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k], q = S[ST_Q][k];
       double edv = (ex * vx + ey * vy) + ez * vz;
@@ -430,6 +538,11 @@ static void flowinit(oracle_state *s) {
     fill_ghosts(s, S[ST_JX]);
     fill_ghosts(s, S[ST_JY]);
     fill_ghosts(s, S[ST_JZ]);
+    /* and the stage velocity (24_boundary_flux_conserved.f95:1426-1560 and
+     * the like); the stage rho and p keep zero ghosts, which no routine of
+     * the reference fills */
+    if (s->c.rec_prim)
+      for (int d = 0; d < 3; d++) fill_ghosts(s, s->vint[l][d]);
   }
 }
 
@@ -538,7 +651,7 @@ void oracle_mp_line(int order, const double *u, int n, double *up, double *um, i
 /* ------------------------------------------------------------------ */
 
 static double slot_x(const oracle_state *s, int slot, size_t k) {
-  double *const *S = s->st, *const *P = s->prim;
+  double *const *S = s->st;
   switch (slot) {
     case XS_PSI: return S[ST_PSI][k];
     case XS_PHI: return S[ST_PHI][k];
@@ -560,11 +673,11 @@ static double slot_x(const oracle_state *s, int slot, size_t k) {
     case XS_SX: return S[ST_SX][k];
     case XS_SY: return S[ST_SY][k];
     case XS_SZ: return S[ST_SZ][k];
-    case XS_VX: return P[OR_VX][k];
-    case XS_VY: return P[OR_VY][k];
-    case XS_VZ: return P[OR_VZ][k];
-    case XS_P: return P[OR_P][k];
-    case XS_RHO: return P[OR_RHO][k];
+    case XS_VX: return s->kin[K_VX][k];
+    case XS_VY: return s->kin[K_VY][k];
+    case XS_VZ: return s->kin[K_VZ][k];
+    case XS_P: return s->kin[K_P][k];
+    case XS_RHO: return s->kin[K_RHO][k];
     case XS_JX: return S[ST_JX][k];
     default: return S[ST_JZ][k];
   }
@@ -733,6 +846,12 @@ static int g_mcl_seed_y = 0;
 
 static int g_hllc_maxiter = 100; /* 200 in 16_hlle_flow_mcl.f95:4385 */
 
+/* set when get_hllc_vel reaches its iteration limit: the reference prints
+ * "[GET_HLLC_VEL]: Non convergence" and stops (funciones.f95:238-244).  A
+ * NaN iterate ends the loop instead (err > minerr is false) and NaN is
+ * returned, which the callers' branches on v* then stop on. */
+static int g_hllc_fail = 0;
+
 double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
   double oldv = clampf(rt, -1.0, 1.0);
   double v = v0;
@@ -744,7 +863,10 @@ double oracle_hllc_vel(double a, double b, double c, double rt, double v0) {
     double err = fabs(oldv - v);
     oldv = v;
     iter = iter + 1;
-    if (iter >= g_hllc_maxiter) return NAN; /* the reference stops here */
+    if (iter >= g_hllc_maxiter) { /* the reference stops here */
+      g_hllc_fail = 1;
+      return NAN;
+    }
     if (!(err > 1.0e-12)) break;
   }
   g_hllc_last_iterate = v;
@@ -766,6 +888,7 @@ static double g_hllc_zero = 1.0e-9;
 /* contact speed of 16_hllc_flow_mpx.f95:3185-3334.  v0 is the point of the
  * first Newton evaluation (see oracle_hllc_vel). */
 static double hllc_contact(double a, double b, double c, double v0) {
+  g_hllc_fail = 0;
   double det = sq(b) - (4.0 * a) * c;
   double qr = -(0.5 * (b + copysign(1.0, b) * fabs(sqrt(det))));
   double rt2 = c / qr;
@@ -825,7 +948,7 @@ static int hllc_x(const side_t *L, const side_t *R, double f[FX_N]) {
   double b = ((e2p + b2p) - tau_h) - sx_f;
   double c = sx_h - erx;
   double v = hllc_contact(a, b, c, g_mcl_seed_x == g_mcl_seed_x ? g_mcl_seed_x : (g_hllc_seed_mode ? R->bz : NAN));
-  if (v != v) return 7;
+  if (g_hllc_fail) return 7;
 
   double pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
   double psiR = (S2 * psi_h - v * ex_f) / (S2 - v);
@@ -931,7 +1054,7 @@ static int hllc_y(const side_t *L, const side_t *R, double f[FY_N]) {
   double b = ((e2p + b2p) - tau_h) - sy_f;
   double c = sy_h - ery;
   double v = hllc_contact(a, b, c, g_mcl_seed_y ? g_hllc_last_iterate : (g_hllc_seed_mode ? -b : NAN));
-  if (v != v) return 7;
+  if (g_hllc_fail) return 7;
 
   double pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
   double psiR = (S2 * psi_h - v * ey_f) / (S2 - v);
@@ -1187,12 +1310,12 @@ static int hlle_xy(double gam, const side_t *xl, const side_t *xr, const side_t 
   double bx = ((((sq(xey_h) + sq(xez_h)) + (sq(xby_h) + sq(xbz_h))) - X.tau_h) - X.s_f[0]);
   double cx = X.s_h[0] - X.er[0];
   double vx = hllc_contact(ax, bx, cx, g_hllc_seed_mode ? Y.er[1] : NAN);
-  if (vx != vx) return 7;
+  if (g_hllc_fail) return 7;
   double ay = Y.tau_f - Y.er[1];
   double by = ((((sq(yex_h) + sq(yez_h)) + (sq(ybx_h) + sq(ybz_h))) - Y.tau_h) - Y.s_f[1]);
   double cy = Y.s_h[1] - Y.er[1];
   double vy = hllc_contact(ay, by, cy, g_hllc_seed_mode ? g_hllc_last_iterate : NAN);
-  if (vy != vy) return 7;
+  if (g_hllc_fail) return 7;
 
   double o[6];
   st = hlle_fluid(&X, 0, a, b, vx, o);
@@ -1431,7 +1554,7 @@ static inline double mcl_lf(double al, double ar, double ul, double ur) { return
 
 /* the face between cell (i,j) and its +x (dir 0) or +y (dir 1) neighbour */
 static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
-  double *const *S = s->st, *const *P = s->prim;
+  double *const *S = s->st;
   const int di = dir == 0, dj = dir == 1;
   size_t km = IX(s, i - di, j - dj), k0 = IX(s, i, j), k1 = IX(s, i + di, j + dj),
          k2 = IX(s, i + 2 * di, j + 2 * dj);
@@ -1450,11 +1573,11 @@ static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
   mcl_rec(S[ST_SX], km, k0, k1, k2, &L.sx, &R.sx, NULL, NULL);
   mcl_rec(S[ST_SY], km, k0, k1, k2, &L.sy, &R.sy, NULL, NULL);
   mcl_rec(S[ST_SZ], km, k0, k1, k2, &L.sz, &R.sz, NULL, NULL);
-  mcl_rec(P[OR_VX], km, k0, k1, k2, &L.vx, &R.vx, NULL, NULL);
-  mcl_rec(P[OR_VY], km, k0, k1, k2, &L.vy, &R.vy, NULL, NULL);
-  mcl_rec(P[OR_VZ], km, k0, k1, k2, &L.vz, &R.vz, NULL, NULL);
-  mcl_rec(P[OR_P], km, k0, k1, k2, &L.p, &R.p, NULL, NULL);
-  mcl_rec(P[OR_RHO], km, k0, k1, k2, &L.rho, &R.rho, NULL, NULL);
+  mcl_rec(s->kin[K_VX], km, k0, k1, k2, &L.vx, &R.vx, NULL, NULL);
+  mcl_rec(s->kin[K_VY], km, k0, k1, k2, &L.vy, &R.vy, NULL, NULL);
+  mcl_rec(s->kin[K_VZ], km, k0, k1, k2, &L.vz, &R.vz, NULL, NULL);
+  mcl_rec(s->kin[K_P], km, k0, k1, k2, &L.p, &R.p, NULL, NULL);
+  mcl_rec(s->kin[K_RHO], km, k0, k1, k2, &L.rho, &R.rho, NULL, NULL);
   double fl[6], fr[6];
   mcl_side_fluxes(s->c.gamma, s->c.sigma, dir, &L, fl);
   mcl_side_fluxes(s->c.gamma, s->c.sigma, dir, &R, fr);
@@ -1504,7 +1627,7 @@ static void mcl_face(const oracle_state *s, int i, int j, int dir, double *f) {
 /* binary built here; hllc_seed = 0 starts both from clamp(rt2) instead). */
 /* ------------------------------------------------------------------ */
 static void mcl_sides(const oracle_state *s, int i, int j, int dir, side_t *L, side_t *R) {
-  double *const *S = s->st, *const *P = s->prim;
+  double *const *S = s->st;
   const int di = dir == 0, dj = dir == 1;
   size_t km = IX(s, i - di, j - dj), k0 = IX(s, i, j), k1 = IX(s, i + di, j + dj),
          k2 = IX(s, i + 2 * di, j + 2 * dj);
@@ -1524,11 +1647,11 @@ static void mcl_sides(const oracle_state *s, int i, int j, int dir, side_t *L, s
   mcl_rec(S[ST_SX], km, k0, k1, k2, &L->sx, &R->sx, NULL, NULL);
   mcl_rec(S[ST_SY], km, k0, k1, k2, &L->sy, &R->sy, NULL, NULL);
   mcl_rec(S[ST_SZ], km, k0, k1, k2, &L->sz, &R->sz, NULL, NULL);
-  mcl_rec(P[OR_VX], km, k0, k1, k2, &L->vx, &R->vx, NULL, NULL);
-  mcl_rec(P[OR_VY], km, k0, k1, k2, &L->vy, &R->vy, NULL, NULL);
-  mcl_rec(P[OR_VZ], km, k0, k1, k2, &L->vz, &R->vz, NULL, NULL);
-  mcl_rec(P[OR_P], km, k0, k1, k2, &L->p, &R->p, NULL, NULL);
-  mcl_rec(P[OR_RHO], km, k0, k1, k2, &L->rho, &R->rho, NULL, NULL);
+  mcl_rec(s->kin[K_VX], km, k0, k1, k2, &L->vx, &R->vx, NULL, NULL);
+  mcl_rec(s->kin[K_VY], km, k0, k1, k2, &L->vy, &R->vy, NULL, NULL);
+  mcl_rec(s->kin[K_VZ], km, k0, k1, k2, &L->vz, &R->vz, NULL, NULL);
+  mcl_rec(s->kin[K_P], km, k0, k1, k2, &L->p, &R->p, NULL, NULL);
+  mcl_rec(s->kin[K_RHO], km, k0, k1, k2, &L->rho, &R->rho, NULL, NULL);
   side_t *sd[2] = {L, R};
   for (int t = 0; t < 2; t++) {
     mcl_side_t m;
@@ -1750,9 +1873,11 @@ static int hlle_mcl_faces(const oracle_state *s, int i, int j, double *fx, doubl
   if (st) return st;
   g_hllc_maxiter = 200;
   double vx = hlle_mcl_contact(0, &X, g_hllc_seed_mode ? Y.er[1] : NAN);
-  double vy = vx == vx ? hlle_mcl_contact(1, &Y, g_hllc_seed_mode ? g_hllc_last_iterate : NAN) : NAN;
+  int fail = g_hllc_fail;
+  double vy = fail ? NAN : hlle_mcl_contact(1, &Y, g_hllc_seed_mode ? g_hllc_last_iterate : NAN);
+  fail = fail || g_hllc_fail;
   g_hllc_maxiter = 100;
-  if (vx != vx || vy != vy) return 7;
+  if (fail) return 7;
   st = hlle_mcl_fluxes(0, &xl, &xr, &X, vx, fx);
   if (!st) st = hlle_mcl_fluxes(1, &yl, &yr, &Y, vy, fy);
   return st;
@@ -1775,12 +1900,15 @@ static void lax_cell_fluxes(const oracle_state *s, int i, int j, int dir, double
     for (int t = 0; t < 6; t++) out[t] = 0.0;
     return;
   }
-  double *const *S = s->st, *const *P = s->prim;
+  double *const *S = s->st;
   size_t k = IX(s, i, j);
-  double vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k], p = P[OR_P][k], rho = P[OR_RHO][k];
+  /* v^n, p^n, rho^n, or with REC_PRIM = 1 the stage's primitives
+   * (14_flowinit.f95:376-489: Wint and enthpyint as interconvar forms them) */
+  double vx = s->kin[K_VX][k], vy = s->kin[K_VY][k], vz = s->kin[K_VZ][k], p = s->kin[K_P][k],
+         rho = s->kin[K_RHO][k];
   double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
   double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k];
-  double w = s->W[k];                                           /* 06_conserved_var.f95:92 */
+  double w = s->c.rec_prim ? s->wint[s->cur_l - 1][k] : s->W[k]; /* 06_conserved_var.f95:92 */
   double eps = p / ((s->c.gamma - 1.0) * rho);                  /* :105 */
   double h = rho * (1.0 + eps) + p;                             /* :109 */
   double e2 = (sq(ex) + sq(ey)) + sq(ez), b2 = (sq(bx) + sq(by)) + sq(bz);
@@ -1863,6 +1991,10 @@ static void fluxandsource(oracle_state *s, int l) {
 #define DXF(a) ((X[a][k] - X[a][kw]) / dx)
 #define DYF(a) ((Y[a][k] - Y[a][ks]) / dy)
       double vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k], w = s->W[k];
+      if (s->c.rec_prim) { /* 17_fluxandsource.f95:460-474 */
+        vx = s->vint[l][0][k]; vy = s->vint[l][1][k]; vz = s->vint[l][2][k];
+        w = s->wint[l][k];
+      }
       double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
       double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k], q = S[ST_Q][k];
       double edv = (ex * vx + ey * vy) + ez * vz;
@@ -1906,16 +2038,68 @@ static void sumfin(oracle_state *s, int l) {
   }
 }
 
+/* varfin (19_varfin.f95:20-50), which also keeps B^n and E^n (Bxn..Ezn,
+ * :24-30) for the MIRK electric field, and electricfield with MIRK = 0
+ * (20_electricfield.f95:153-167): U^{n+1} = U^n + dt finsum.  The MIRK
+ * electric field follows in electricfield_mirk. */
 static void varfin(oracle_state *s) {
   const double dt = s->c.dt;
   for (int a = 0; a < AC_N; a++) {
     double *u = (double *)un_plane(s, a);
+    const int e = a >= AC_EX && a <= AC_EZ;
+    if (e && s->c.mirk) continue;
     for (int j = 0; j <= s->c.jmax; j++)
       for (int i = 0; i <= s->c.imax; i++) {
         size_t k = IX(s, i, j);
         u[k] = u[k] + dt * s->finsum[a][k];
       }
   }
+}
+
+/* electricfield, MIRK = 1 (20_electricfield.f95:13-48) and MIRK = 2
+ * (:56-146), over the interior after varfin: bn, en are B^n and E^n;
+ * vst, wst the last stage's velocity and its W, which the MIRK2 branch
+ * recomputes from the velocity and stores (:66-71). */
+static void electricfield_mirk(oracle_state *s, double *const bn[3], double *const en[3], double *const vst[3],
+                               double *wst) {
+  const double sig = s->c.sigma, dt = s->c.dt, c4 = cm4();
+  double **P = s->prim, **S = s->st, **F = s->finsum;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      if (s->c.mirk == 1) {
+        double w = s->W[k], vx = P[OR_VX][k], vy = P[OR_VY][k], vz = P[OR_VZ][k];
+        double ex = P[OR_EX][k], ey = P[OR_EY][k], ez = P[OR_EZ][k];
+        double sb = (dt * sig) * w;
+        double edv = (ex * vx + ey * vy) + ez * vz;
+        double nx = bn[2][k] * vy - bn[1][k] * vz, ny = bn[0][k] * vz - bn[2][k] * vx,
+               nz = bn[1][k] * vx - bn[0][k] * vy;
+        double bx = P[OR_BX][k], by = P[OR_BY][k], bz = P[OR_BZ][k];
+        double rx = bz * vy - by * vz, ry = bx * vz - bz * vx, rz = by * vx - bx * vy;
+        double inv = 1.0 / (1.0 + sb);
+        P[OR_EX][k] = inv * (((ex + (sb * vx) * edv) - sb * (CM2 * nx + (1.0 - CM2) * rx)) + dt * F[AC_EX][k]);
+        P[OR_EY][k] = inv * (((ey + (sb * vy) * edv) - sb * (CM2 * ny + (1.0 - CM2) * ry)) + dt * F[AC_EY][k]);
+        P[OR_EZ][k] = inv * (((ez + (sb * vz) * edv) - sb * (CM2 * nz + (1.0 - CM2) * rz)) + dt * F[AC_EZ][k]);
+        continue;
+      }
+      double vx = vst[0][k], vy = vst[1][k], vz = vst[2][k];
+      double v2 = (sq(vx) + sq(vy)) + sq(vz);
+      double w = 1.0 / sqrt(1.0 - v2);
+      wst[k] = w;
+      double sdb = (dt * sig) * w;
+      double eix = S[ST_EX][k], eiy = S[ST_EY][k], eiz = S[ST_EZ][k];
+      double edv = (eix * vx + eiy * vy) + eiz * vz;
+      double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k];
+      double rx = bz * vy - by * vz, ry = bx * vz - bz * vx, rz = by * vx - bx * vy;
+      double a = -1.0 + sdb * (1.0 - CM1);
+      double den = 1.0 + (0.5 * CM1 - c4) * sdb;
+      P[OR_EX][k] = (eix + ((0.5 * a) / den) * (eix - en[0][k])) +
+                    (0.5 / den) * ((((-(sdb * eix)) + (sdb * edv) * vx) + dt * F[AC_EX][k]) - sdb * rx);
+      P[OR_EY][k] = (eiy + ((0.5 * a) / den) * (eiy - en[1][k])) +
+                    (0.5 / den) * ((((-(sdb * eiy)) + (sdb * edv) * vy) + dt * F[AC_EY][k]) - sdb * ry);
+      P[OR_EZ][k] = (eiz + ((0.5 * a) / den) * (eiz - en[2][k])) +
+                    (0.5 / den) * ((((-(sdb * eiz)) + (sdb * edv) * vz) + dt * F[AC_EZ][k]) - sdb * rz);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1994,7 +2178,13 @@ int oracle_c2p_point(double gamma, double eps1, const double in[14], double out[
   if (y1 <= 1.0 && y2 <= 1.0) return 23;
   if (y1 >= 1.0 && y2 <= 1.0) Wn = y1;
   if (y1 <= 1.0 && y2 >= 1.0) Wn = y2;
-  if (Wn != Wn) return 24;
+  /* No root selected (NaN roots): the reference then keeps Wnro from the
+   * previous cell's Newton loop (:463-500 assign nothing).  With a NaN or
+   * infinite quartic coefficient every Newton update is NaN whatever the
+   * start, so the cell goes on to NaN results as in the reference; with
+   * finite coefficients the previous cell's W would matter, and the
+   * restatement stops instead (status 24). */
+  if (Wn != Wn && isfinite(b3) && isfinite(b2) && isfinite(b1) && isfinite(b0)) return 24;
 
   /* Newton on W, :510-526 */
   int nrw = 1;
@@ -2088,16 +2278,78 @@ int oracle_get_fixups(const oracle_state *s, int kind, int32_t *ij, double *vals
   return n;
 }
 
+/* varprimitivecardano with l_swap = 0 on stage l (12_varprimitive.f95:111-135,
+ * 792-806; REC_PRIM = 1, main_cueva_code.f95:220-224), started from the
+ * stage's primitives as they stand (the previous step's, zero at first),
+ * then interconvar (13_intercon_var.f95:13-72) and its ghost fills */
+static int c2p_stage(oracle_state *s, int l) {
+  double **S = s->st;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double in[14] = {S[ST_D][k], S[ST_TAU][k], S[ST_SX][k], S[ST_SY][k], S[ST_SZ][k],
+                       S[ST_EX][k], S[ST_EY][k], S[ST_EZ][k], S[ST_BX][k], S[ST_BY][k], S[ST_BZ][k],
+                       s->vint[l][0][k], s->vint[l][1][k], s->vint[l][2][k]};
+      double out[5] = {0, 0, 0, s->rhoint[l][k], s->pint[l][k]};
+      int fl[3];
+      double rec[6];
+      int st = oracle_c2p_point(s->c.gamma, s->eps1, in, out, fl, rec);
+      if (st) {
+        s->stop[2] = i;
+        s->stop[3] = j;
+        return st;
+      }
+      s->cnt[0] += fl[0]; s->cnt[1] += fl[1]; s->cnt[2] += fl[2];
+      for (int kind = 0; kind < 3; kind++)
+        if (fl[kind] && add_fixup(s, kind, i, j, rec[2 * kind], rec[2 * kind + 1])) return 90;
+      s->vint[l][0][k] = out[0]; s->vint[l][1][k] = out[1]; s->vint[l][2][k] = out[2];
+      s->rhoint[l][k] = out[3]; s->pint[l][k] = out[4];
+    }
+  const double gam = s->c.gamma;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double vx = s->vint[l][0][k], vy = s->vint[l][1][k], vz = s->vint[l][2][k];
+      double rho = s->rhoint[l][k], p = s->pint[l][k];
+      double ex = S[ST_EX][k], ey = S[ST_EY][k], ez = S[ST_EZ][k];
+      double bx = S[ST_BX][k], by = S[ST_BY][k], bz = S[ST_BZ][k];
+      double v2 = (sq(vx) + sq(vy)) + sq(vz);
+      double w = 1.0 / sqrt(1.0 - v2);
+      double e2 = (sq(ex) + sq(ey)) + sq(ez);
+      double b2 = (sq(bx) + sq(by)) + sq(bz);
+      double eps = p / ((gam - 1.0) * rho);
+      double h = rho * (1.0 + eps) + p;
+      s->wint[l][k] = w;
+      s->hint[l][k] = h;
+      S[ST_D][k] = rho * w;
+      S[ST_TAU][k] = (0.5 * (e2 + b2) + h * sq(w)) - p;
+      S[ST_SX][k] = (bz * ey - by * ez) + (h * sq(w)) * vx;
+      S[ST_SY][k] = (bx * ez - bz * ex) + (h * sq(w)) * vy;
+      S[ST_SZ][k] = (by * ex - bx * ey) + (h * sq(w)) * vz;
+    }
+  static const int cons[11] = {ST_PSI, ST_PHI, ST_BX, ST_BY, ST_BZ, ST_Q, ST_D, ST_TAU, ST_SX, ST_SY, ST_SZ};
+  for (int k = 0; k < 11; k++) fill_ghosts(s, S[cons[k]]);
+  fill_ghosts(s, S[ST_EX]);
+  fill_ghosts(s, S[ST_EY]);
+  fill_ghosts(s, S[ST_EZ]);
+  return 0;
+}
+
+/* varprimitivecardano after varfin: l_swap = 1 (the step's primitives as
+ * the start), or with REC_PRIM = 1 l_swap = 2 (the last stage's velocity,
+ * rho and p, 12_varprimitive.f95:59-83) */
 static int c2p(oracle_state *s) {
   double **P = s->prim;
-  s->nfix[0] = s->nfix[1] = s->nfix[2] = 0;
+  const int L = s->c.lmax - 1, rp = s->c.rec_prim;
+  if (!rp) s->nfix[0] = s->nfix[1] = s->nfix[2] = 0;
   for (int j = 0; j <= s->c.jmax; j++)
     for (int i = 0; i <= s->c.imax; i++) {
       size_t k = IX(s, i, j);
       double in[14] = {s->D[k], s->tau[k], s->Sx[k], s->Sy[k], s->Sz[k],
                        P[OR_EX][k], P[OR_EY][k], P[OR_EZ][k], P[OR_BX][k], P[OR_BY][k], P[OR_BZ][k],
-                       P[OR_VX][k], P[OR_VY][k], P[OR_VZ][k]};
-      double out[5] = {0, 0, 0, P[OR_RHO][k], P[OR_P][k]};
+                       rp ? s->vint[L][0][k] : P[OR_VX][k], rp ? s->vint[L][1][k] : P[OR_VY][k],
+                       rp ? s->vint[L][2][k] : P[OR_VZ][k]};
+      double out[5] = {0, 0, 0, rp ? s->rhoint[L][k] : P[OR_RHO][k], rp ? s->pint[L][k] : P[OR_P][k]};
       int fl[3];
       double rec[6];
       int st = oracle_c2p_point(s->c.gamma, s->eps1, in, out, fl, rec);
@@ -2124,6 +2376,7 @@ static void install_modes(const oracle_state *s);
 int oracle_recover(oracle_state *s) {
   install_modes(s);
   s->cnt[0] = s->cnt[1] = s->cnt[2] = 0;
+  s->nfix[0] = s->nfix[1] = s->nfix[2] = 0;
   s->stop[0] = 1;
   s->stop[1] = s->c.lmax;
   return c2p(s);
@@ -2145,6 +2398,34 @@ static void install_modes(const oracle_state *s) {
   g_limit = s->c.limit ? s->c.limit : 2;
 }
 
+/* swap1, 08_swap1.f95:13-33: the primitives of the step start into the
+ * stage-1 arrays (interior) */
+static void swap1(oracle_state *s) {
+  double **P = s->prim;
+  for (int j = 0; j <= s->c.jmax; j++)
+    for (int i = 0; i <= s->c.imax; i++) {
+      size_t k = IX(s, i, j);
+      double gam = s->c.gamma, rho = P[OR_RHO][k], p = P[OR_P][k];
+      double eps = p / ((gam - 1.0) * rho);
+      s->rhoint[0][k] = rho;
+      s->pint[0][k] = p;
+      s->vint[0][0][k] = P[OR_VX][k]; s->vint[0][1][k] = P[OR_VY][k]; s->vint[0][2][k] = P[OR_VZ][k];
+      s->wint[0][k] = s->W[k];
+      s->hint[0][k] = rho * (1.0 + eps) + p;
+    }
+}
+
+/* B^n and E^n of the interior, as varfin keeps them (19_varfin.f95:24-30) */
+static void keep_bn_en(oracle_state *s) {
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j <= s->c.jmax; j++)
+      for (int i = 0; i <= s->c.imax; i++) {
+        size_t k = IX(s, i, j);
+        s->bn[d][k] = s->prim[OR_BX + d][k];
+        s->en[d][k] = s->prim[OR_EX + d][k];
+      }
+}
+
 int oracle_step(oracle_state *s, int nsteps) {
   install_modes(s);
   for (int h = 0; h < nsteps; h++) {
@@ -2156,11 +2437,22 @@ int oracle_step(oracle_state *s, int nsteps) {
     startconvar(s);
     cleanvarint(s);
     s->cnt[0] = s->cnt[1] = s->cnt[2] = 0;
+    s->nfix[0] = s->nfix[1] = s->nfix[2] = 0;
+    if (s->vint[0][0]) swap1(s);
     for (int l = 0; l < s->c.lmax; l++) {
       for (int m = 0; m < l; m++) sumint(s, l, m);
       varint(s);
       electricvarint(s, l);
-      flowinit(s);
+      if (s->c.rec_prim) { /* main_cueva_code.f95:220-228 */
+        int st = c2p_stage(s, l);
+        if (st) {
+          s->stop[0] = h + 1;
+          s->stop[1] = l + 1;
+          return st;
+        }
+      }
+      set_kin(s, l);
+      flowinit(s, l);
       if (s->c.flux_solver == OR_HLL || s->c.flux_solver == OR_HLLE || s->c.flux_solver == OR_HLLC) reconstruct(s);
       s->cur_l = l + 1;
       int st = face_loop(s);
@@ -2172,7 +2464,12 @@ int oracle_step(oracle_state *s, int nsteps) {
       fluxandsource(s, l);
       sumfin(s, l);
     }
+    if (s->c.mirk) keep_bn_en(s);
     varfin(s);
+    if (s->c.mirk) {
+      const int L = s->c.lmax - 1;
+      electricfield_mirk(s, s->bn, s->en, s->vint[L], s->wint[L]);
+    }
     int st = c2p(s);
     if (st) {
       s->stop[0] = h + 1;

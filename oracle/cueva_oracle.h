@@ -34,7 +34,8 @@ enum { OR_LAX = 1, OR_HLL_MCL = 2, OR_HLLE_MCL = 3, OR_HLLC_MCL = 4, OR_HLL = 5,
 
 typedef struct {
   int32_t imax, jmax;
-  int32_t lmax;            /* IMEX stages: 2 = SSP2(2,2,2), 3 = SSP2(3,3,2)-LUM, 4 = SSP3(4,3,3) */
+  int32_t lmax;            /* stages = order: 1 (MIRK1 or first-order IMEX), 2 = SSP2(2,2,2) or MIRK2,
+                              3 = SSP2(3,3,2)-LUM, 4 = SSP3(4,3,3) */
   int32_t flux_solver;     /* 5 HLL, 6 HLLE, 7 HLLC */
   int32_t rec_order;       /* 5, 7 or 9 (MP5/MP7/MP9) */
   int32_t bc_x, bc_y;      /* OR_BC_COPY / OR_BC_PERIODIC */
@@ -48,16 +49,22 @@ typedef struct {
   int32_t libm_mode;       /* 0: acos/cos/pow of the C library (as the reference build); 1: correctly rounded */
   int32_t slop;            /* second-order slope limiter (parameters.f95:17): 1 MinMod, 2 MC, 3 Superbee */
   int32_t limit;           /* the same for recons_var_primitive (parameters.f95:18) */
+  int32_t mirk;            /* 0 IMEX, 1 MIRK first order, 2 MIRK second order (parameters.f95:11) */
+  int32_t rec_prim;        /* 1: primitive recovery of every stage state (parameters.f95:23) */
   int32_t pad1;
 } oracle_cfg;
 
 typedef struct oracle_state oracle_state;
 
 /* Fill cfg with the reference's constants for a grid and scheme
- * (parameters.f95:303-334, imex_scheme.f95:40-49,280-291). */
+ * (parameters.f95:303-334, imex_scheme.f95:13-49,77-86,280-291), MIRK = 0. */
 int oracle_default_cfg(oracle_cfg *c, int imax, int jmax, double lx, double ly,
                        int lmax, int flux_solver, int rec_order, int bc_x, int bc_y,
                        double sigma, double cfl);
+/* Refill the Butcher tables of cfg for its lmax (= order) and mirk
+ * (imex_scheme.f95:13-58); nonzero for the combination the reference stops
+ * on (order 2 with MIRK = 1). */
+int oracle_scheme_tables(oracle_cfg *c);
 
 oracle_state *oracle_create(const oracle_cfg *c);
 void oracle_destroy(oracle_state *s);

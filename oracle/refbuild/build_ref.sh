@@ -12,7 +12,9 @@
 #
 # usage: build_ref.sh NAME IMAX JMAX ORDER FLUX_SOLVER REC_MP BOUND SIGMA CFL LX LY
 #   FLUX_SOLVER 1=Lax-Friedrichs, 2=HLL+MCL, 4=HLLC+MCL, 5=HLL 6=HLLE 7=HLLC (+MPx);  REC_MP 0=MP5 1=MP7 2=MP9
-#   ORDER 2=SSP2(2,2,2) 4=SSP3(4,3,3);  BOUND 1 copy, 2 periodic, 6 periodic-y/copy-x
+# environment: SLOP LIMIT (default 2), PRIMITIVE_REC MIRK REC_PRIM (default 0) set the
+# parameters of the same names.
+#   ORDER 1 (MIRK=1 or first-order IMEX) 2=SSP2(2,2,2) or MIRK2 3=SSP2(3,3,2)-LUM 4=SSP3(4,3,3);  BOUND 1 copy, 2 periodic, 6 periodic-y/copy-x
 set -euo pipefail
 
 if [ $# -ne 11 ]; then
@@ -55,6 +57,8 @@ sed -E \
   -e "s/^( *integer,parameter +:: +primitive_rec +=) *[0-9]+/\1 ${PRIMITIVE_REC:-0}/" \
   -e "s/^( *integer,parameter +:: +slop +=) *[0-9]+/\1 ${SLOP:-2}/" \
   -e "s/^( *integer,parameter +:: +limit +=) *[0-9]+/\1 ${LIMIT:-2}/" \
+  -e "s/^( *integer,parameter +:: +MIRK +=) *[0-9]+/\1 ${MIRK:-0}/" \
+  -e "s/^( *integer,parameter +:: +REC_PRIM +=) *[0-9]+/\1 ${REC_PRIM:-0}/" \
   "$REF/parameters.f95" > "$WORK/parameters.f95"
 
 # Every substitution must have hit exactly its line.
@@ -72,6 +76,8 @@ check "CFL +=  *$CFL( |$)"
 check "primitive_rec +=  *${PRIMITIVE_REC:-0}( |$)"
 check "slop +=  *${SLOP:-2}( |$)"
 check "limit +=  *${LIMIT:-2}( |$)"
+check "MIRK +=  *${MIRK:-0}( |$)"
+check "REC_PRIM +=  *${REC_PRIM:-0}( |$)"
 
 cd "$WORK"
 UNITS="scalar threevectors fourvectors funciones imex_scheme mp5 mp7 mp9

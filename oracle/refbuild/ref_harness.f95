@@ -4,8 +4,9 @@
 ! oracle/refbuild/build_ref.sh and its output becomes golden fixtures.
 !
 ! It restates the per-step call sequence of the reference driver
-! (main_cueva_code.f95:47-356) with I/O and the commented-out adaptive dt
-! left out, and replaces the RESTART read (03_restart_primitive.f95:50-61)
+! (main_cueva_code.f95:47-356, with the REC_PRIM = 1 stage recovery of
+! :220-228 and the final l_swap of :301-309) with I/O and the commented-out
+! adaptive dt left out, and replaces the RESTART read (03_restart_primitive.f95:50-61)
 ! by a binary read of the same 14 fields, so that seeded initial data reach
 ! the reference bit for bit (with -restart it makes the reference's own
 ! text read instead).
@@ -20,7 +21,7 @@
 !   OUTPREFIX  each dump step h writes OUTPREFIX_hNNNNN.bin:
 !              int32 counters (contador_r, contador_p, contador_v), then 19 arrays:
 !              the 14 fields above followed by D tau Sx Sy Sz
-!   -debug     also write OUTPREFIX_stage.bin after step 1 with, for every stage l:
+!   -debug     also write OUTPREFIX_stageL.bin after step 1 (-debug-step H: step H) with, for every stage l:
 !              W, Exast..Ezast, the 14 stage fields, Jx Jy Jz, the 28 face fluxes,
 !              and the 17 stage right-hand sides (see write_stage below)
 !   -globals   after the last step, run the reference's own lector with h = 1,
@@ -51,7 +52,7 @@ program cueva_ref_harness
   implicit none
 
   character(len=1024) :: arg, infile, prefix, fname
-  integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out, hlector
+  integer             :: nsteps, ndump, iarg, nargs, kd, u_in, u_out, hlector, hdebug
   integer, allocatable :: dumps(:)
   logical             :: debug, globals, from_text, c2p_only, c2p_hll
 
@@ -72,11 +73,17 @@ program cueva_ref_harness
   c2p_only = .false.
   c2p_hll = .false.
   hlector = 1
+  hdebug = 1
   iarg = 4
   do while (iarg <= nargs)
      call get_command_argument(iarg, arg)
      if (trim(arg) == "-debug") then
         debug = .true.
+     else if (trim(arg) == "-debug-step") then
+        debug = .true.
+        iarg = iarg + 1
+        call get_command_argument(iarg, arg)
+        read(arg,*) hdebug
      else if (trim(arg) == "-globals") then
         globals = .true.
      else if (trim(arg) == "-restart") then
@@ -158,6 +165,11 @@ program cueva_ref_harness
         end do
         call varint
         call electricvarint
+        if (REC_PRIM == 1) then
+           l_swap = 0
+           call varprimitivecardano
+           call interconvar
+        end if
         call boundary_conserved
         call boundary_electric
         call flowinit
@@ -182,13 +194,17 @@ program cueva_ref_harness
         end if
         call fluxandsource
         call sumfin
-        if (debug .and. h == 1) call write_stage(l)
+        if (debug .and. h == hdebug) call write_stage(l)
      end do
 
      l = lmax
      call varfin
      call electricfield
-     l_swap = 1
+     if (REC_PRIM == 1) then
+        l_swap = 2
+     else
+        l_swap = 1
+     end if
      call varprimitivecardano
      call boundary
      t_init = 1

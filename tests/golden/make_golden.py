@@ -76,7 +76,30 @@ CASES = {
     # IMEX SSP2(3,3,2)-LUM (order = 3), the scheme of most of the reference's parameters_*.f95
     "kh_hllc_mp9_lum_32x64": (32, 64, 7, 2, 3, 1.0e6, 10, [1, 10], [1, 10], None, {"slop": 1, "limit": 1}),
     "kh_hll_mcl_lum_32x64": (32, 64, 2, 0, 3, 1.0e3, 10, [1, 10], [1, 10], None, {"slop": 1}),
+    # MIRK electric field integration (MIRK = 1 with order 1, MIRK = 2 with order 2;
+    # 11_elecvarint.f95:210-277, 20_electricfield.f95:13-146)
+    "kh_hll_mirk1_32x64": (32, 64, 5, 0, 1, 1.0e3, 10, [1, 10], [1, 10], None, {"mirk": 1}),
+    "kh_hllc_mirk2_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"mirk": 2}),
+    "kh_hll_mcl_mirk2_bz_32x64": (32, 64, 2, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None, {"mirk": 2, "bz0": 0.1}),
+    # REC_PRIM = 1: a primitive recovery of every stage (main_cueva_code.f95:220-228).  The
+    # reference never fills the ghosts of its stage rho and p, so the MPx face states at the
+    # boundaries see zeros: NaN enters there in the first step (HLL), or HLLC stops in it.
+    # Lax-Friedrichs reconstructs the step's primitives (15_lax_flow.f95 has no REC_PRIM
+    # branch) and stays finite.
+    "kh_hll_recprim_32x64": (32, 64, 5, 0, 2, 1.0e3, 1, [1], [1], None, {"rec_prim": 1}),
+    "kh_hll_recprim_mirk2_32x64": (32, 64, 5, 0, 2, 1.0e3, 1, [1], [1], None, {"rec_prim": 1, "mirk": 2}),
+    "kh_hllc_recprim_32x64": (32, 64, 7, 0, 2, 1.0e6, 1, [], [], 1, {"rec_prim": 1}),
+    "kh_lax_recprim_32x64": (32, 64, 1, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None, {"rec_prim": 1}),
+    # ... and with SSP2(3,3,2)-LUM and a guide field the reference stops in the
+    # recovery of the third stage of its first step (Cardano det < 0)
+    "kh_lax_recprim_lum_bz_32x64": (32, 64, 1, 0, 3, 1.0e6, 1, [], [], 1, {"rec_prim": 1, "bz0": 0.1}),
 }
+# parameters of the reference build set from a case's keywords (build_ref.sh environment)
+BUILD_KEYS = (("primitive_rec", 0), ("slop", 2), ("limit", 2), ("mirk", 0), ("rec_prim", 0))
+
+
+def build_env(init_kw):
+    return dict(os.environ, **{k.upper(): str(init_kw.get(k, d)) for k, d in BUILD_KEYS})
 SEED = 20261202
 
 
@@ -98,9 +121,7 @@ def main(only=None):
         if not os.path.exists(exe):
             subprocess.run([os.path.join(ROOT, "oracle", "refbuild", "build_ref.sh"), name, str(imax), str(jmax),
                             str(order), str(fs), str(rec), "6", "%.1fd0" % sigma if sigma < 1e4 else "1.d6",
-                            "0.4d0", "2.d0", "1.d0"], check=True,
-                           env=dict(os.environ, PRIMITIVE_REC=str(init_kw.get("primitive_rec", 0)),
-                                    SLOP=str(init_kw.get("slop", 2)), LIMIT=str(init_kw.get("limit", 2))))
+                            "0.4d0", "2.d0", "1.d0"], check=True, env=build_env(init_kw))
         p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order,
                                     rotated=True, seed=SEED, **init_kw)
         if mod:
@@ -162,8 +183,16 @@ def main(only=None):
                 arrays["face_event_val"] = np.array(ev_val)
             if stop is not None:
                 assert stopped, "%s: the reference was expected to stop" % name
-                assert not os.path.exists(os.path.join(tmp, "out_h%05d.bin" % dumps[-1]).replace(
-                    "h%05d" % dumps[-1], "h%05d" % stop))
+                assert not os.path.exists(os.path.join(tmp, "out_h%05d.bin" % stop))
+                # the recovery's stop message and its (h, i, j, l)
+                out = res.stdout + res.stderr
+                stops = C2P_STOPS + [("Revise Vxstar velocity", 8), ("GET_HLLC_VEL", 7)]
+                msg = [x.strip() for x in out.splitlines() if any(f in x for f, _ in stops)]
+                if msg:
+                    meta["stop"] = {"code": [c for f, c in stops if f in msg[0]][0], "message": msg[0]}
+                    if "--->" in msg[0]:
+                        h, i, j, l = msg[0].split("--->")[1].split()[:4]
+                        meta["stop"].update({"h": int(h), "i": int(i), "j": int(j), "l": int(l)})
         with open(os.path.join(HERE, name + ".json"), "w") as fh:
             json.dump(meta, fh, indent=1, sort_keys=True)
         if arrays:
@@ -385,10 +414,20 @@ def _read_fixups(path, with_h=False):
     lines = open(path).read().splitlines()
     if not lines:
         return []
-    # two header records, each wrapped onto two lines at 80 columns; the
-    # records after them are ten list-directed items, also wrapped
+    # two header records, each wrapped onto two lines at 80 columns, written
+    # by every recovery of the first step (once, or with REC_PRIM = 1 once per
+    # stage and once after varfin); the records are ten list-directed items,
+    # also wrapped
     assert lines[0].split()[0] == "iteration" and lines[2].strip().startswith("=")
-    tok = " ".join(lines[4:]).split()
+    body, skip = [], 0
+    for ln in lines:
+        if skip:
+            skip -= 1
+        elif ln.split()[:1] == ["iteration"] or ln.strip().startswith("="):
+            skip = 1
+        else:
+            body.append(ln)
+    tok = " ".join(body).split()
     assert len(tok) % 10 == 0
     recs = []
     for k in range(0, len(tok), 10):

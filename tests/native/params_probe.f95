@@ -19,6 +19,8 @@ program params_probe
   write(*,'(a,i0)') "primitive_rec ", prm%primitive_rec
   write(*,'(a,i0)') "slop ", prm%slop
   write(*,'(a,i0)') "limit ", prm%limit
+  write(*,'(a,i0)') "mirk ", prm%mirk
+  write(*,'(a,i0)') "rec_prim ", prm%rec_prim
   write(*,'(a,i0)') "bc_x ", prm%bc_x
   write(*,'(a,i0)') "bc_y ", prm%bc_y
   write(*,'(a,es25.17)') "longx ", prm%longx

@@ -30,9 +30,13 @@ def test_driver_built_and_checks_arguments():
                                              (cueva.LAX, 0, 2, {"bz0": 0.1}), (cueva.HLLC_MCL, 0, 2, {"bz0": 0.1}),
                                              (cueva.HLLE_MCL, 0, 2, {"bz0": 0.1}), (cueva.HLLC, 0, 2, {"primitive_rec": 1}),
                                              (cueva.HLLC_MCL, 0, 2, {"slop": 3}),
-                                             (cueva.HLL_MCL, 0, 2, {"primitive_rec": 1, "limit": 1, "slop": 1})],
+                                             (cueva.HLL_MCL, 0, 2, {"primitive_rec": 1, "limit": 1, "slop": 1}),
+                                             (cueva.HLL, 0, 1, {"mirk": 1}), (cueva.HLLC, 0, 2, {"mirk": 2}),
+                                             (cueva.LAX, 0, 2, {"rec_prim": 1, "bz0": 0.1}),
+                                             (cueva.LAX, 0, 2, {"rec_prim": 1, "mirk": 2})],
                          ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl", "hlle_mcl",
-                              "hllc_mp5_primrec", "hllc_mcl_superbee", "hll_mcl_minmod_primrec_minmod"])
+                              "hllc_mp5_primrec", "hllc_mcl_superbee", "hll_mcl_minmod_primrec_minmod", "hll_mirk1",
+                              "hllc_mirk2", "lax_recprim", "lax_recprim_mirk2"])
 def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built (needs flang)")
@@ -42,9 +46,10 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     print("driver", fs, rec, order, flush=True)
     args = [DRIVER, str(ni), str(nj), str(steps), str(fs), str(rec), str(order), repr(sigma), "1", "0.5", "0.01",
             repr(bz0), "20261202", "1", prefix]
-    if kw.get("primitive_rec") or kw.get("slop") or kw.get("limit"):
-        # HLOC RESTART_IN HINI PRIMITIVE_REC SLOP LIMIT
-        args += ["0", "-", "1", str(kw.get("primitive_rec", 0)), str(kw.get("slop", 2)), str(kw.get("limit", 2))]
+    if any(kw.get(k) for k in ("primitive_rec", "slop", "limit", "mirk", "rec_prim")):
+        # HLOC RESTART_IN HINI PRIMITIVE_REC SLOP LIMIT MIRK REC_PRIM
+        args += ["0", "-", "1"] + [str(kw.get(k, d)) for k, d in (("primitive_rec", 0), ("slop", 2), ("limit", 2),
+                                                                   ("mirk", 0), ("rec_prim", 0))]
     res = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     p, prims = cueva.kh_problem(ni, nj, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True, **kw)
@@ -69,11 +74,13 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     assert np.array_equal(glob[:, 0], np.arange(1, steps + 1))
 
 
-def _run_driver(cwd, ni, nj, fs, rec, order, sigma, bz0, steps, hloc, restart_in="-", hini=1, hglb=1, primrec=0):
+def _run_driver(cwd, ni, nj, fs, rec, order, sigma, bz0, steps, hloc, restart_in="-", hini=1, hglb=1, primrec=0,
+                slop=2, limit=2, mirk=0, rec_prim=0):
     for d in ("data_restart", "data_global"):
         os.makedirs(os.path.join(cwd, d), exist_ok=True)
     args = [DRIVER, str(ni), str(nj), str(steps), str(fs), str(rec), str(order), repr(sigma), "1", "0.5", "0.01",
-            repr(bz0), "20261202", str(hglb), "run", str(hloc), restart_in, str(hini), str(primrec)]
+            repr(bz0), "20261202", str(hglb), "run", str(hloc), restart_in, str(hini), str(primrec), str(slop),
+            str(limit), str(mirk), str(rec_prim)]
     res = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=cwd)
     assert res.returncode == 0, res.stdout + res.stderr
     return open(os.path.join(cwd, "run.bin"), "rb").read()
@@ -134,12 +141,15 @@ def write_restart_text(path, prims):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["kh_hll_lowp_32x64", "kh_hllc_lowp_32x64"])
-def test_driver_fixup_files_are_the_reference_s(tmp_path, name):
+@pytest.mark.parametrize("name,steps", [("kh_hll_lowp_32x64", None), ("kh_hllc_lowp_32x64", None),
+                                        ("kh_lax_recprim_32x64", 1)])
+def test_driver_fixup_files_are_the_reference_s(tmp_path, name, steps):
     """Crafted data whose recovery takes the pressure fix-up every step
     (tests/crafted.py): the driver's fix-up files are the reference's own
     byte for byte (12_varprimitive.f95:33-40,601) and its fields after the
-    fixture's steps are the reference's."""
+    fixture's steps are the reference's.  With REC_PRIM = 1 every recovery
+    of the first step writes the files' headers (one per stage, one after
+    varfin)."""
     import hashlib
     import json
     import crafted
@@ -152,18 +162,20 @@ def test_driver_fixup_files_are_the_reference_s(tmp_path, name):
     assert np.array_equal(cueva.read_restart(src, p.imax, p.jmax), prims)
     ni, nj = meta["native_grid"]
     print("driver fix-up files", name, flush=True)
+    init = meta.get("init", {})
+    steps = steps or meta["nsteps"]
     raw = _run_driver(str(tmp_path), ni, nj, meta["flux_solver"], meta["rec_mp"], meta["order"], meta["sigma"], 0.0,
-                      meta["nsteps"], 0, restart_in=src, hglb=0)
+                      steps, 0, restart_in=src, hglb=0, rec_prim=init.get("rec_prim", 0), mirk=init.get("mirk", 0))
     for kind, fn in (("rho", "fix-up_densidad.dat"), ("p", "fix-up_presion.dat"),
                      ("v", "fix-up_velocity_drift.dat")):
         got = hashlib.sha256(open(str(tmp_path / fn), "rb").read()).hexdigest()
         assert got == meta["fixups"][kind]["sha256"], (kind, open(str(tmp_path / fn)).read()[:800])
     cnt = np.frombuffer(raw[:12], dtype="<i4").tolist()
-    assert cnt == meta["dumps"][str(meta["nsteps"])]["counters"]
+    assert cnt == meta["dumps"][str(steps)]["counters"]
     arr = np.frombuffer(raw[12:], dtype="<f8").reshape(19, p.jmax + 13, p.imax + 13)
     inner = arr[:, 6:6 + p.jmax + 1, 6:6 + p.imax + 1]
     fields = list(cueva.PRIM_NAMES) + list(cueva.CONS_NAMES)
-    ref = meta["dumps"][str(meta["nsteps"])]["sha256"]
+    ref = meta["dumps"][str(steps)]["sha256"]
     bad = [f for k, f in enumerate(fields) if hashlib.sha256(np.ascontiguousarray(inner[k]).tobytes()).hexdigest()
            != ref[f]]
     assert not bad, bad

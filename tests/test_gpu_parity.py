@@ -48,6 +48,10 @@ CASES = {
     "hlle_mcl": (cueva.HLLE_MCL, 0, 2, 1.0e6, {}),
     "hlle_mcl_bz": (cueva.HLLE_MCL, 0, 2, 1.0e6, {"bz0": 0.1}),
     "hllc_mp5_primrec": (cueva.HLLC, 0, 2, 1.0e6, {"primitive_rec": 1}),
+    # MIRK electric field integration (first order with MIRK1, MIRK2 with order 2)
+    "hll_mirk1": (cueva.HLL, 0, 1, 1.0e3, {"mirk": 1}),
+    "hllc_mirk2": (cueva.HLLC, 0, 2, 1.0e6, {"mirk": 2}),
+    "hll_mcl_mirk2_bz": (cueva.HLL_MCL, 0, 2, 1.0e3, {"mirk": 2, "bz0": 0.1}),
 }
 
 
@@ -103,7 +107,8 @@ def test_gpu_equals_oracle_bitwise_100_steps(fs, sigma):
 
 
 @pytest.mark.parametrize("case", ["hll_mp5", "hllc_mp5", "hll_mp9_ssp3", "hlle_mp7_bz", "hll_mcl_bz", "lax_bz",
-                                  "hllc_mcl_bz", "hlle_mcl_bz", "hllc_mp5_primrec"])
+                                  "hllc_mcl_bz", "hlle_mcl_bz", "hllc_mp5_primrec", "hll_mirk1", "hllc_mirk2",
+                                  "hll_mcl_mirk2_bz"])
 def test_routine_entry_points_match_fused_step(case):
     """The per-subroutine entry points (the drop-in path) and cueva_step agree bit for bit."""
     fs, rec, order, sigma, kw = CASES[case]
@@ -163,6 +168,8 @@ BITWISE_DUMPS = {
     "kh_hllc_mp5_primrec_superbee_32x64": (1, 10),
     # IMEX SSP2(3,3,2)-LUM
     "kh_hllc_mp9_lum_32x64": (1, 10), "kh_hll_mcl_lum_32x64": (1, 10),
+    # MIRK1 (order 1) and MIRK2
+    "kh_hll_mirk1_32x64": (1, 10), "kh_hllc_mirk2_32x64": (1, 10), "kh_hll_mcl_mirk2_bz_32x64": (1, 10),
 }
 
 

@@ -1,0 +1,142 @@
+"""REC_PRIM = 1 on the GPU: the primitive recovery of every stage state
+(main_cueva_code.f95:220-228, l_swap = 0 in 12_varprimitive.f95, then
+13_intercon_var.f95), the stage's own velocity, pressure and density in
+flowinit, the face solves and fluxandsource, and the final recovery started
+from the last stage's primitives (l_swap = 2).
+
+What the reference fixtures pin (tests/golden/kh_*recprim*; the oracle is
+held to the same fixtures in tests/test_oracle_golden.py):
+
+* Lax-Friedrichs reconstructs no stage primitive at the faces (only
+  flowinit's cell fluxes read them), so its fields stay finite: bit for bit
+  with the reference at step 1, and at step 10 within 10x the reference's
+  own spread under another C library code path (its acos/cos/pow misround
+  where the GPU's are correctly rounded; the oracle with the C library is
+  bit for bit there).
+* HLL + MP5 reconstructs the stage rho and p, whose ghosts no routine of the
+  reference fills (they stay zero), so NaN enters at the boundary faces in
+  the first step: the same points are NaN, every other value bit for bit.
+  Later steps spread NaN with the sign bits the reference's compiled code
+  happens to give them (they steer its minmod), which nothing pins.
+* HLLC stops in the first step ("Revise Vxstar velocity", status 8), and
+  Lax-Friedrichs with SSP2(3,3,2)-LUM and a guide field stops in the third
+  stage's recovery (Cardano det < 0 at h, l, i, j = 1, 3, 32, 0).
+The fused cueva_step (which runs this mode as the per-subroutine sequence)
+and the per-subroutine entry points, called one by one, agree bit for bit.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import crafted
+import cueva
+from oracle.binding import Oracle
+from parity import FIELDS, interior, rel_l1
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def golden(name):
+    meta = json.load(open(os.path.join(HERE, "golden", name + ".json")))
+    npz = os.path.join(HERE, "golden", name + ".npz")
+    return meta, (dict(np.load(npz)) if os.path.exists(npz) else {})
+
+
+def same_or_both_nan(x, y):
+    x, y = np.asarray(x, dtype="<f8"), np.asarray(y, dtype="<f8")
+    return (x.view("<u8") == y.view("<u8")) | (np.isnan(x) & np.isnan(y))
+
+
+def run(s, n, by_routines):
+    for _ in range(n):
+        if by_routines:
+            s.step_by_routines()
+        else:
+            s.step(1)
+
+
+@pytest.mark.parametrize("by_routines", [False, True], ids=["cueva_step", "routines"])
+@pytest.mark.parametrize("name", ["kh_lax_recprim_32x64", "kh_hll_recprim_32x64", "kh_hll_recprim_mirk2_32x64"])
+def test_gpu_against_reference_fixture(name, by_routines):
+    meta, arrays = golden(name)
+    say("fixture", name, "by routines" if by_routines else "cueva_step")
+    p, prims = crafted.problem(meta)
+    assert p.rec_prim == 1
+    with cueva.Solver(p) as s:
+        s.upload(prims)
+        done = 0
+        for d in sorted(int(k) for k in meta["dumps"]):
+            run(s, d - done, by_routines)
+            done = d
+            cnt, _ = s.step_counters()
+            assert cnt == meta["dumps"][str(d)]["counters"]
+            g = interior(*s.download(), p)
+            ref = arrays["h%d" % d]
+            bad = {f: int((~same_or_both_nan(g[k], ref[k])).sum()) for k, f in enumerate(FIELDS)}
+            say(name, "step", d, "NaN", int(np.isnan(g).sum()), "differing values", sum(bad.values()))
+            if d == 1:
+                assert sum(bad.values()) == 0, bad
+            else:
+                # later the host C library's misrounded acos/cos/pow in the
+                # reference (the GPU rounds them correctly) have moved values;
+                # within 10x the reference's own spread under another glibc path
+                err = rel_l1(g, ref)
+                spread = meta["dumps"][str(d)]["libm_spread"]
+                assert all(err[f] <= max(1e-12, 10.0 * spread[f]) for f in FIELDS), (err, spread)
+
+
+@pytest.mark.parametrize("rotated", [True, False], ids=["rotated", "native"])
+@pytest.mark.parametrize("fs,steps", [(cueva.LAX, 5), (cueva.HLL, 1), (cueva.HLLE, 1)], ids=["lax", "hll", "hlle"])
+def test_gpu_equals_oracle(fs, steps, rotated):
+    """Same arithmetic as the oracle (correctly rounded Cardano on both sides)."""
+    say("REC_PRIM oracle pair", fs, steps, "rotated" if rotated else "native")
+    p, prims = cueva.kh_problem(32, 64, sigma=1.0e3, flux_solver=fs, rotated=rotated, rec_prim=1)
+    o = Oracle(p)
+    o.upload(prims)
+    with cueva.Solver(p) as s:
+        s.upload(prims)
+        for h in range(steps):
+            s.step(1)
+            cnt, _ = s.step_counters()
+            o.step(1)
+            assert cnt == o.counters(), h
+        g = interior(*s.download(), p)
+    r = interior(*o.download(), p)
+    assert same_or_both_nan(g, r).all(), int((~same_or_both_nan(g, r)).sum())
+
+
+@pytest.mark.parametrize("name", ["kh_hllc_recprim_32x64", "kh_lax_recprim_lum_bz_32x64"])
+def test_gpu_stops_where_the_reference_stops(name):
+    meta, _ = golden(name)
+    stop = meta["stop"]
+    say("REC_PRIM stop", name, stop["message"])
+    p, prims = crafted.problem(meta)
+    with cueva.Solver(p) as s:
+        s.upload(prims)
+        rc = s.step_status(meta["expected_stop_step"])
+        code, where = s.last_error()
+    assert rc == code == stop["code"]
+    if "i" in stop:
+        assert where == (stop["h"], stop["l"], stop["i"], stop["j"])
+    else:  # no cell in the reference's message: the oracle's
+        o = Oracle(p)
+        o.upload(prims)
+        assert o.step_status(meta["expected_stop_step"]) == code
+        assert where == tuple(o.last_stop())
+
+
+def test_second_order_solvers_refused():
+    """The reference's REC_PRIM branch of its second-order solvers reads an
+    unset z-velocity (16_hll_flow_mcl.f95:911-958): not a scheme to run."""
+    for fs in (cueva.HLL_MCL, cueva.HLLE_MCL, cueva.HLLC_MCL):
+        p = cueva.default_params(imax=64, jmax=32, flux_solver=fs, rec_prim=1)
+        with pytest.raises(cueva.CuevaError) as e:
+            cueva.Solver(p)
+        assert e.value.code == 1  # CUEVA_E_ARG

@@ -10,7 +10,7 @@ constants folded from the same file's text (with Fortran's literal kinds:
 parameters_TM.f95's Lx carries a single precision exponent).  A variant that
 selects a branch the library does not run is refused with every such
 constant named: parameters_CE.f95 (flux_rec_mp5 = 1), parameters_REC_old.f95
-(MIRK = 2, BOUND = 3).
+(BOUND = 3; its MIRK = 2 with order 2 is run).
 """
 import os
 import re
@@ -100,15 +100,19 @@ def constants(path):
 def expected_refusals(c):
     """The constants cueva_params_from_module must refuse: branches of the
     reference's step the library does not run."""
-    bad = [n for n in ("DIM", "MIRK", "REC_PRIM", "REC_ELEC", "VDRIFT", "flux_rec_mp5", "REC_4VECTOR", "low_vel",
+    bad = [n for n in ("DIM", "REC_ELEC", "VDRIFT", "flux_rec_mp5", "REC_4VECTOR", "low_vel",
                        "SLC") if c[n] != (2 if n == "DIM" else 0)]
+    if c["REC_PRIM"] not in (0, 1) or (c["REC_PRIM"] == 1 and 2 <= c["flux_solver"] <= 4):
+        bad.append("REC_PRIM")
+    if c["MIRK"] not in (0, 1, 2) or (c["order"] == 2 and c["MIRK"] == 1):  # imex_scheme.f95:51-56
+        bad.append("MIRK")
     if c["BOUND"] not in (1, 2, 6):
         bad.append("BOUND")
     if c["conserved_to_primitive"] != 1:
         bad.append("conserved_to_primitive")
     if c["source_rec"] or c["source_rec_mpx"] or c["var_source_rec"]:
         bad.append("source_rec")
-    if not 2 <= c["order"] <= 4:
+    if not 1 <= c["order"] <= 4:
         bad.append("order")
     return sorted(bad)
 
@@ -150,6 +154,7 @@ def test_params_from_reference_module(tmp_path, variant):
     assert got["status"] == (1 if refused else 0), res.stdout
     for n in ("imax", "jmax", "order", "flux_solver", "rec_mp", "primitive_rec", "slop", "limit"):
         assert got[n] == c[n], n
+    assert got["mirk"] == c["MIRK"] and got["rec_prim"] == c["REC_PRIM"]
     if "BOUND" not in refused:
         assert (got["bc_x"], got["bc_y"]) == {1: (0, 0), 2: (1, 1), 6: (0, 1)}[c["BOUND"]]
     assert got["longx"] == c["Lx"] and got["longy"] == c["Ly"]
