@@ -81,6 +81,10 @@ CASES = {
     "kh_hll_mirk1_32x64": (32, 64, 5, 0, 1, 1.0e3, 10, [1, 10], [1, 10], None, {"mirk": 1}),
     "kh_hllc_mirk2_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"mirk": 2}),
     "kh_hll_mcl_mirk2_bz_32x64": (32, 64, 2, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None, {"mirk": 2, "bz0": 0.1}),
+    # the highly relativistic shear (v0 = 0.9, W = 2.3; Bx = 0.001) with MP9 + HLLC and SSP3(4,3,3):
+    # the reference stops in step 7 (Cardano det < 0 in the recovery after stage 4; at 32 x 64 it
+    # stops in step 1, get_hllc_vel not converging)
+    "kh_cfg5_hllc_mp9_ssp3_128x256": (128, 256, 7, 2, 4, 1.0e6, 10, [1, 6], [6], 7, {"v0": 0.9, "bx0": 0.001}),
     # REC_PRIM = 1: a primitive recovery of every stage (main_cueva_code.f95:220-228).  The
     # reference never fills the ghosts of its stage rho and p, so the MPx face states at the
     # boundaries see zeros: NaN enters there in the first step (HLL), or HLLC stops in it.

@@ -32,11 +32,11 @@ def test_driver_built_and_checks_arguments():
                                              (cueva.HLLC_MCL, 0, 2, {"slop": 3}),
                                              (cueva.HLL_MCL, 0, 2, {"primitive_rec": 1, "limit": 1, "slop": 1}),
                                              (cueva.HLL, 0, 1, {"mirk": 1}), (cueva.HLLC, 0, 2, {"mirk": 2}),
-                                             (cueva.LAX, 0, 2, {"rec_prim": 1, "bz0": 0.1}),
+                                             (cueva.LAX, 0, 2, {"rec_prim": 1, "mirk": 2, "bz0": 0.1}),
                                              (cueva.LAX, 0, 2, {"rec_prim": 1, "mirk": 2})],
                          ids=["hllc_mp5", "hlle_mp7", "hll_mp9_ssp3", "hll_mcl", "lax", "hllc_mcl", "hlle_mcl",
                               "hllc_mp5_primrec", "hllc_mcl_superbee", "hll_mcl_minmod_primrec_minmod", "hll_mirk1",
-                              "hllc_mirk2", "lax_recprim", "lax_recprim_mirk2"])
+                              "hllc_mirk2", "lax_recprim_mirk2_bz", "lax_recprim_mirk2"])
 def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     if not os.path.exists(DRIVER):
         pytest.skip("Fortran driver not built (needs flang)")
@@ -72,6 +72,26 @@ def test_driver_matches_fused_step(tmp_path, fs, rec, order, kw):
     assert glob.shape == (steps, 39)
     assert np.array_equal(glob[:, 2:], np.array(want_glob))
     assert np.array_equal(glob[:, 0], np.arange(1, steps + 1))
+
+
+@pytest.mark.gpu
+def test_driver_stops_where_the_oracle_stops(tmp_path):
+    """REC_PRIM = 1 with the IMEX E solve at sigma = 1e6 and a guide field: the
+    recovery of the second stage of step 3 finds det < 0 (as the oracle does);
+    the driver prints the reference's status and (h, l, i, j) and stops."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("Fortran driver not built (needs flang)")
+    from oracle.binding import Oracle
+    p, prims = cueva.kh_problem(32, 64, sigma=1.0e6, flux_solver=cueva.LAX, rotated=True, rec_prim=1, bz0=0.1)
+    o = Oracle(p)
+    o.upload(prims)
+    assert o.step_status(3) == 22
+    args = [DRIVER, "32", "64", "3", str(cueva.LAX), "0", "2", repr(1.0e6), "1", "0.5", "0.01", "0.1", "20261202",
+            "1", str(tmp_path / "run"), "0", "-", "1", "0", "2", "2", "0", "1"]
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert res.returncode != 0
+    assert int(res.stdout.split("returned status")[1].split()[0]) == 22, res.stdout
+    assert tuple(int(x) for x in res.stdout.split("--->")[1].split()[:4]) == tuple(o.last_stop()), res.stdout
 
 
 def _run_driver(cwd, ni, nj, fs, rec, order, sigma, bz0, steps, hloc, restart_in="-", hini=1, hglb=1, primrec=0,

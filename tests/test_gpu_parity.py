@@ -143,6 +143,21 @@ def test_ssp3_mp9_stops_like_the_reference():
         assert code == 22 and where[:2] == (8, 4)  # step 8, in the recovery after stage 4
 
 
+def test_relativistic_shear_ssp3_mp9_hllc_stops_like_the_reference():
+    """v0 = 0.9 (W = 2.3), Bx = 0.001, MP9 + HLLC + SSP3(4,3,3) at 128 x 256: the
+    reference stops in step 7, Cardano det < 0 in the recovery after stage 4 at
+    (i, j) = (118, 0); the GPU path stops there too."""
+    meta = json.load(open(os.path.join(HERE, "golden", "kh_cfg5_hllc_mp9_ssp3_128x256.json")))
+    st = meta["stop"]
+    p, prims = crafted.problem(meta)
+    with cueva.Solver(p) as s:
+        s.upload(prims)
+        rc = s.step_status(meta["expected_stop_step"])
+        code, where = s.last_error()
+    assert rc == code == st["code"] == 22
+    assert where == (st["h"], st["l"], st["i"], st["j"])
+
+
 def golden(name):
     meta = json.load(open(os.path.join(HERE, "golden", name + ".json")))
     npz = os.path.join(HERE, "golden", name + ".npz")
@@ -170,6 +185,8 @@ BITWISE_DUMPS = {
     "kh_hllc_mp9_lum_32x64": (1, 10), "kh_hll_mcl_lum_32x64": (1, 10),
     # MIRK1 (order 1) and MIRK2
     "kh_hll_mirk1_32x64": (1, 10), "kh_hllc_mirk2_32x64": (1, 10), "kh_hll_mcl_mirk2_bz_32x64": (1, 10),
+    # v0 = 0.9, MP9 + HLLC + SSP3(4,3,3) (the reference's C library misrounds a pow in step 1)
+    "kh_cfg5_hllc_mp9_ssp3_128x256": (),
 }
 
 

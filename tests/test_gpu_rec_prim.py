@@ -95,7 +95,9 @@ def test_gpu_against_reference_fixture(name, by_routines):
 @pytest.mark.parametrize("rotated", [True, False], ids=["rotated", "native"])
 @pytest.mark.parametrize("fs,steps", [(cueva.LAX, 5), (cueva.HLL, 1), (cueva.HLLE, 1)], ids=["lax", "hll", "hlle"])
 def test_gpu_equals_oracle(fs, steps, rotated):
-    """Same arithmetic as the oracle (correctly rounded Cardano on both sides)."""
+    """Same arithmetic as the oracle (correctly rounded Cardano on both sides),
+    and the same stop where the oracle stops (HLLE: its NaN wave-speed check,
+    16_hlle_flow_mpx.f95:2795, on the zero stage rho and p ghosts)."""
     say("REC_PRIM oracle pair", fs, steps, "rotated" if rotated else "native")
     p, prims = cueva.kh_problem(32, 64, sigma=1.0e3, flux_solver=fs, rotated=rotated, rec_prim=1)
     o = Oracle(p)
@@ -103,9 +105,15 @@ def test_gpu_equals_oracle(fs, steps, rotated):
     with cueva.Solver(p) as s:
         s.upload(prims)
         for h in range(steps):
-            s.step(1)
+            rc = s.step_status(1)
+            ro = o.step_status(1)
+            assert rc == ro, (h, rc, ro)
+            if rc:
+                code, where = s.last_error()
+                stop = o.last_stop()
+                assert where == (h + 1,) + tuple(stop[1:]), (where, stop)
+                return
             cnt, _ = s.step_counters()
-            o.step(1)
             assert cnt == o.counters(), h
         g = interior(*s.download(), p)
     r = interior(*o.download(), p)
