@@ -73,6 +73,10 @@ CASES = {
                                         {"primitive_rec": 1, "limit": 1}),
     "kh_hllc_mp5_primrec_superbee_32x64": (32, 64, 7, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None,
                                            {"primitive_rec": 1, "limit": 3}),
+    # longer and higher-order runs of the primitive reconstruction (the identity up to rounding,
+    # so only bit-level drift would show a fault)
+    "kh_hllc_mcl_primrec_128x256": (128, 256, 4, 0, 2, 1.0e6, 50, [1, 10, 50], [50], None, {"primitive_rec": 1}),
+    "kh_hll_mcl_ssp3_primrec_32x64": (32, 64, 2, 0, 4, 1.0e3, 5, [1, 5], [1, 5], None, {"primitive_rec": 1}),
     # IMEX SSP2(3,3,2)-LUM (order = 3), the scheme of most of the reference's parameters_*.f95
     "kh_hllc_mp9_lum_32x64": (32, 64, 7, 2, 3, 1.0e6, 10, [1, 10], [1, 10], None, {"slop": 1, "limit": 1}),
     "kh_hll_mcl_lum_32x64": (32, 64, 2, 0, 3, 1.0e3, 10, [1, 10], [1, 10], None, {"slop": 1}),

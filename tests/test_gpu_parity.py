@@ -187,6 +187,9 @@ BITWISE_DUMPS = {
     "kh_hll_mirk1_32x64": (1, 10), "kh_hllc_mirk2_32x64": (1, 10), "kh_hll_mcl_mirk2_bz_32x64": (1, 10),
     # v0 = 0.9, MP9 + HLLC + SSP3(4,3,3) (the reference's C library misrounds a pow in step 1)
     "kh_cfg5_hllc_mp9_ssp3_128x256": (),
+    # primitive reconstruction: HLLC + MC limiter 50 steps at 128x256 (bitwise until the C
+    # library's first misrounding), HLL + MC limiter with SSP3(4,3,3)
+    "kh_hllc_mcl_primrec_128x256": (1,), "kh_hll_mcl_ssp3_primrec_32x64": (1, 5),
 }
 
 
