@@ -895,6 +895,9 @@ static int march_seg(cueva_ctx* c, bool fin, bool c2p) {
 // c2p (last stage only): with the primitive recovery (to P.pn) and, with
 // stage0, the next step's stage-0 state (to P.sn)
 static int launch_march(cueva_ctx* c, int l, bool fin, bool c2p = false, bool stage0 = false) {
+  // its face solves and right-hand side read the t^n primitives (no stage
+  // kinematics: REC_PRIM = 1 runs the per-subroutine sequence)
+  if (c->prm.rec_prim) return CUEVA_E_ARG;
   const void* fn = march_kernel_of(c, fin, c2p);
   int seg = march_seg(c, fin, c2p);
   if (!fn || seg <= 0) return CUEVA_E_HIP;

@@ -59,9 +59,12 @@ struct Planes {
 enum { K_VX, K_VY, K_VZ, K_P, K_RHO, K_N };
 
 // the plane a face solve or the stage right-hand side reads for kinematic
-// quantity q (K_*)
+// quantity q (K_*); STAGE false: the primitive plane, for kernels that never
+// run with REC_PRIM = 1 (the marching kernel), which then skip the test
+template <bool STAGE = true>
 __device__ inline const double* kin_plane(const Planes& P, int q) {
   const int pq = q == K_VX ? P_VX : q == K_VY ? P_VY : q == K_VZ ? P_VZ : q == K_P ? P_P : P_RHO;
+  if constexpr (!STAGE) return P.p[pq];
   return P.kin[q] ? P.kin[q] : P.p[pq];
 }
 

@@ -374,7 +374,7 @@ __device__ inline void lax_face(const Geom& g, const Planes& P, const Scheme& S,
 // array is dead after its put, which keeps the other solve's registers free
 // of it.  YFIRST solves the y face first (the two solves are independent
 // except under HLLE, whose y solve starts from the x solve's last iterate).
-template <int ORD, int SOLVER, typename T, bool YFIRST = false, typename PutX, typename PutY>
+template <int ORD, int SOLVER, typename T, bool YFIRST = false, bool STAGE_KIN = true, typename PutX, typename PutY>
 __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const Scheme& S, const StatusRef& st, int i,
                                           int j, PutX put_x, PutY put_y) {
   static_assert(!(YFIRST && SOLVER == 6), "the HLLE y solve needs the x solve first");
@@ -423,16 +423,16 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   }
   Side<T> xl, xr, yl, yr;
   // kinematics of both faces first: the fallbacks couple them
-  face_rec<ORD>(kin_plane(P, K_VX), c, sx, xl.vx, xr.vx);
-  face_rec<ORD>(kin_plane(P, K_VY), c, sx, xl.vy, xr.vy);
-  face_rec<ORD>(kin_plane(P, K_VZ), c, sx, xl.vz, xr.vz);
-  face_rec<ORD>(kin_plane(P, K_VX), c, sy, yl.vx, yr.vx);
-  face_rec<ORD>(kin_plane(P, K_VY), c, sy, yl.vy, yr.vy);
-  face_rec<ORD>(kin_plane(P, K_VZ), c, sy, yl.vz, yr.vz);
-  face_rec_pos<ORD>(kin_plane(P, K_P), c, sx, xl.p, xr.p);
-  face_rec_pos<ORD>(kin_plane(P, K_RHO), c, sx, xl.rho, xr.rho);
-  face_rec_pos<ORD>(kin_plane(P, K_P), c, sy, yl.p, yr.p);
-  face_rec_pos<ORD>(kin_plane(P, K_RHO), c, sy, yl.rho, yr.rho);
+  face_rec<ORD>(kin_plane<STAGE_KIN>(P, K_VX), c, sx, xl.vx, xr.vx);
+  face_rec<ORD>(kin_plane<STAGE_KIN>(P, K_VY), c, sx, xl.vy, xr.vy);
+  face_rec<ORD>(kin_plane<STAGE_KIN>(P, K_VZ), c, sx, xl.vz, xr.vz);
+  face_rec<ORD>(kin_plane<STAGE_KIN>(P, K_VX), c, sy, yl.vx, yr.vx);
+  face_rec<ORD>(kin_plane<STAGE_KIN>(P, K_VY), c, sy, yl.vy, yr.vy);
+  face_rec<ORD>(kin_plane<STAGE_KIN>(P, K_VZ), c, sy, yl.vz, yr.vz);
+  face_rec_pos<ORD>(kin_plane<STAGE_KIN>(P, K_P), c, sx, xl.p, xr.p);
+  face_rec_pos<ORD>(kin_plane<STAGE_KIN>(P, K_RHO), c, sx, xl.rho, xr.rho);
+  face_rec_pos<ORD>(kin_plane<STAGE_KIN>(P, K_P), c, sy, yl.p, yr.p);
+  face_rec_pos<ORD>(kin_plane<STAGE_KIN>(P, K_RHO), c, sy, yl.rho, yr.rho);
   // superluminal fallback, 16_hll_flow_mpx.f95:1392-1461 (same in hllc;
   // hlle only prints, 16_hlle_flow_mpx.f95:2270-2300)
   if (SOLVER != 6) {
@@ -651,7 +651,7 @@ k_faces_march(MarchArgs args) {
       }
     };
     if (face_ok) {
-      face_pair<ORD, SOLVER, T, !XLDS>(g, P, S, st, i, j, put_x, put_y);
+      face_pair<ORD, SOLVER, T, !XLDS, false>(g, P, S, st, i, j, put_x, put_y);
     } else {
       T z[FX_N > FY_N ? FX_N : FY_N];
 #pragma unroll
@@ -681,7 +681,7 @@ k_faces_march(MarchArgs args) {
         dfy[a] = ddy<T>(S, fyb[b][w][a][lane] - below);
       }
       const int64_t k = g.at(i, j);
-      const Rhs r = rhs_of<T>(P, S, k, dfx, dfy);
+      const Rhs r = rhs_of<T, false>(P, S, k, dfx, dfy);  // never with REC_PRIM = 1
       if constexpr (!FINAL) {
 #pragma unroll
         for (int t = 0; t < R_N; t++) P.rhs[l][t][k] = r.r[t];

@@ -21,14 +21,15 @@ template <typename FT> __device__ inline FT ddy(const Scheme& S, FT a) {
 // dfx[a] = (Fx(i+1/2) - Fx(i-1/2))/dx and dfy likewise, formed in the flux
 // arithmetic FT (held in FT until used: a float difference widens exactly),
 // combined with the FP64 sources of the stage state.
-template <typename FT>
+template <typename FT, bool STAGE_KIN = true>
 __device__ inline Rhs rhs_of(const Planes& P, const Scheme& S, int64_t k, const FT* dfx, const FT* dfy) {
 #pragma clang fp contract(off)  // FP64: as the reference build, also inside the FP32 face unit
 #define DX(a) ((double)dfx[a])
 #define DY(a) ((double)dfy[a])
   // v^n, or with REC_PRIM = 1 the stage velocity (17_fluxandsource.f95:460-474);
   // W as 06_conserved_var.f95 / 13_intercon_var.f95:27 form it from that velocity
-  double vx = kin_plane(P, K_VX)[k], vy = kin_plane(P, K_VY)[k], vz = kin_plane(P, K_VZ)[k];
+  double vx = kin_plane<STAGE_KIN>(P, K_VX)[k], vy = kin_plane<STAGE_KIN>(P, K_VY)[k],
+         vz = kin_plane<STAGE_KIN>(P, K_VZ)[k];
   double w = 1.0 / sqrt(1.0 - ((sq(vx) + sq(vy)) + sq(vz)));
   double q = P.s[S_Q][k];
   Rhs o;
