@@ -159,7 +159,7 @@ def load_packed():
     return None
 
 
-def time_form(cfg, form, steps, warmup, packed):
+def time_form(cfg, form, steps, warmup, packed, host_io=False):
     import cueva
     from cueva import problems
     p, prims = problems.config(cfg, **FORMS[form])
@@ -179,12 +179,33 @@ def time_form(cfg, form, steps, warmup, packed):
     kt = s.kernel_times()
     s.set_profiling(False)
     cnt, fallback = s.step_counters()
-    s.close()
     zones = p.imax * p.jmax
-    return {"ms_per_step": ms / steps, "value": zones * steps / (ms / 1e3),
-            "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()},
-            "kernel_launches": {k: v[1] for k, v in kt.items()},
-            "fixups": cnt, "c2p_fallbacks": fallback, "p": p}
+    out = {"ms_per_step": ms / steps, "value": zones * steps / (ms / 1e3),
+           "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()},
+           "kernel_launches": {k: v[1] for k, v in kt.items()},
+           "fixups": cnt, "c2p_fallbacks": fallback, "p": p}
+    if host_io:
+        # the host side of the boundary: the state down (14 primitive + 5
+        # conserved planes) and up again (14 primitive planes), synchronous
+        # calls on pageable host arrays, as a host that crossed PCIe every
+        # step would pay; the drop-in path keeps the state in HBM between
+        # the reference's output calls
+        t0 = time.perf_counter()
+        prims_h, _ = s.download(cons=True)
+        t1 = time.perf_counter()
+        s.upload(prims_h)
+        s.sync()
+        t2 = time.perf_counter()
+        down, up = (t1 - t0) * 1e3, (t2 - t1) * 1e3
+        out["host_boundary"] = {
+            "download_ms": down, "upload_ms": up,
+            "bytes_down": int(prims_h.nbytes * 19 // 14), "bytes_up": int(prims_h.nbytes),
+            "pcie_inclusive_value": zones / ((ms / steps + down + up) / 1e3),
+            "note": "zone-updates/s of a host that downloads and re-uploads the state every step; "
+                    "not the metric's value"}
+        del prims_h
+    s.close()
+    return out
 
 
 def accuracy(cfg, form, packed, steps=ACCURACY_STEPS):
@@ -283,7 +304,7 @@ def main():
         forms = ["plain_fp64", "mixed", "mixed_learned"] if packed is not None else ["plain_fp64", "mixed"]
     if "mixed_learned" in forms and packed is None:
         sys.exit("no trained parameters (cueva/c2p_mlp.json) for the learned form")
-    res = {f: time_form(cfg, f, args.steps, args.warmup, packed) for f in forms}
+    res = {f: time_form(cfg, f, args.steps, args.warmup, packed, host_io=(f == forms[0])) for f in forms}
     head = forms[0]  # plain FP64 unless one form was asked for
     r = res[head]
     p = r["p"]
@@ -337,10 +358,11 @@ def main():
                           "achieved_GBs": r["value"] * STEP_BYTES_PER_ZONE / 1e9,
                           "frac": r["value"] * STEP_BYTES_PER_ZONE / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_per_step": r["kernel_ms_per_step"], "fixups": r["fixups"], "c2p_fallbacks": r["c2p_fallbacks"],
+        "host_boundary": r.get("host_boundary"),
         "forms": {},
     }
     for f in forms:
-        e = {k: v for k, v in res[f].items() if k != "kernel_launches"}
+        e = {k: v for k, v in res[f].items() if k not in ("kernel_launches", "host_boundary")}
         e["form"] = form_name[f]
         if f != head:
             e["speedup_vs_plain_fp64"] = res[f]["value"] / r["value"]

@@ -47,7 +47,7 @@ CASES = {
     "kh_hllc_mp9_32x64": (32, 64, 7, 2, 2, 1.0e6, 10, [1, 10], [10], None),
     "kh_hll_mp9_ssp3_32x64": (32, 64, 5, 2, 4, 1.0e3, 10, [1, 7], [7], 8),
     "kh_cfg1_hll_mp5_128x256": (128, 256, 5, 0, 2, 1.0e3, 100, [1, 10, 100], [100], None),
-    "kh_hllc_mp5_128x256": (128, 256, 7, 0, 2, 1.0e6, 100, [1, 10, 100], [], None),
+    "kh_hllc_mp5_128x256": (128, 256, 7, 0, 2, 1.0e6, 100, [1, 10, 100], [100], None),
     "kh_hlle_mp7_bz_32x64": (32, 64, 6, 1, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
     "kh_hll_mcl_32x64": (32, 64, 2, 0, 2, 1.0e3, 10, [1, 10], [1, 10], None),
     "kh_hll_mcl_bz_32x64": (32, 64, 2, 0, 2, 1.0e6, 10, [1, 10], [1, 10], None, {"bz0": 0.1}),
