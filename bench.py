@@ -129,11 +129,14 @@ def cpu_baseline(workload_cfg, budget_s=12.0):
 
 
 # the forms the metric compares; crmath 1 = the reference's correctly rounded
-# acos/cos/cbrt (bitwise form), 0 = device math library
+# acos/cos/cbrt (bitwise form), 0 = device math library; hllc_seed 1 = the
+# reference build's first Newton point of the HLLC contact solve (bitwise
+# form), 0 = the clamped root of the quadratic it refines (the mixed forms,
+# which are not bitwise and converge from it in fewer iterations)
 FORMS = {
-    "plain_fp64": dict(precision=0, crmath=1, c2p_mode=0),
-    "mixed": dict(precision=1, crmath=0, c2p_mode=0),
-    "mixed_learned": dict(precision=1, crmath=0, c2p_mode=1),
+    "plain_fp64": dict(precision=0, crmath=1, c2p_mode=0, hllc_seed=1),
+    "mixed": dict(precision=1, crmath=0, c2p_mode=0, hllc_seed=0),
+    "mixed_learned": dict(precision=1, crmath=0, c2p_mode=1, hllc_seed=0),
 }
 # compulsory FP64 bytes per cell of the marching face kernel (every form: the
 # stage planes are FP64, the FP32 work stays on chip).  Every launch reads the

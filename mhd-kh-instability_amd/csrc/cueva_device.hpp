@@ -79,6 +79,14 @@ template <typename T> __device__ inline T dm4(T w, T x, T y, T z) {
   T m = mn(mn(mn(fab(w), fab(x)), fab(y)), fab(z));
   return ((T(0.125) * (sw + csign(x))) * fab((sw + csign(y)) * (sw + csign(z)))) * m;
 }
+// FP32 (mixed form): both minmods are medians with zero -- min(x, y) where
+// both are positive, max(x, y) where both are negative, 0 where the signs
+// differ, the sign factors being exactly 1 -- so one v_med3_f32 per pair
+// gives the same values (up to the sign of a zero result)
+template <> __device__ inline float dmm<float>(float x, float y) { return __builtin_amdgcn_fmed3f(x, y, 0.0f); }
+template <> __device__ inline float dm4<float>(float w, float x, float y, float z) {
+  return __builtin_amdgcn_fmed3f(__builtin_amdgcn_fmed3f(w, x, 0.0f), __builtin_amdgcn_fmed3f(y, z, 0.0f), 0.0f);
+}
 
 template <int ORD> struct MPc;
 template <> struct MPc<5> { static constexpr int K = 2; };
