@@ -66,6 +66,25 @@ __device__ inline float csign(float x) { return copysignf(1.0f, x); }
 __device__ inline double fab(double x) { return fabs(x); }
 __device__ inline float fab(float x) { return fabsf(x); }
 
+// Quotients of the face solves: the IEEE quotient in FP64 (the reference's
+// arithmetic, bit for bit); in FP32 (mixed form) the product with the
+// hardware reciprocal (v_rcp_f32, within 1 ulp), taken once for a
+// denominator several quotients share.  inv_sqrt: 1/sqrt(x) likewise
+// (v_rsq_f32 in FP32).
+template <typename T> struct Quot {
+  T d;
+  __device__ explicit Quot(T den) : d(den) {}
+  __device__ T operator()(T num) const { return num / d; }
+};
+template <> struct Quot<float> {
+  float r;
+  __device__ explicit Quot(float den) : r(__builtin_amdgcn_rcpf(den)) {}
+  __device__ float operator()(float num) const { return num * r; }
+};
+template <typename T> __device__ inline T quot(T num, T den) { return Quot<T>(den)(num); }
+__device__ inline double inv_sqrt(double x) { return 1.0 / sqrt(x); }
+__device__ inline float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // ---------------------------------------------------------------------------
 // MP5 / MP7 / MP9 interface values (mp5.f95:40-264, mp7.f95, mp9.f95)
 // ---------------------------------------------------------------------------
@@ -214,13 +233,13 @@ template <typename T> struct Side {
 // W as 16_hlle_flow_mpx.f95:2302-2305 forms it (squares subtracted one by one).
 template <int DIR, bool HLLE_W = false, typename T> __device__ inline void side_physics(T gam, Side<T>& o) {
   if constexpr (HLLE_W) {
-    o.w = T(1) / sqrt(((T(1) - sq(o.vx)) - sq(o.vy)) - sq(o.vz));
+    o.w = inv_sqrt(((T(1) - sq(o.vx)) - sq(o.vy)) - sq(o.vz));
   } else {
     T v2 = (sq(o.vx) + sq(o.vy)) + sq(o.vz);
-    o.w = T(1) / sqrt(T(1) - v2);
+    o.w = inv_sqrt(T(1) - v2);
   }
   o.e2b2 = ((((sq(o.ex) + sq(o.ey)) + sq(o.ez)) + sq(o.bx)) + sq(o.by)) + sq(o.bz);
-  T eps = o.p / ((gam - T(1)) * o.rho);
+  T eps = quot(o.p, (gam - T(1)) * o.rho);
   o.h = o.rho * (T(1) + eps) + o.p;
   T hw2 = o.h * sq(o.w);
   if constexpr (DIR == 0) {
@@ -312,7 +331,7 @@ __device__ inline T hllc_vel(T a, T b, T c, T rt, T v0, int& fail, T& vlast, int
   for (;;) {
     T fv = (a * v * v - b * v) + c;
     T dfdv = (T(2) * a) * v - b;
-    v = clampf(oldv - fv / dfdv, T(-1), T(1));
+    v = clampf(oldv - quot(fv, dfdv), T(-1), T(1));
     T err = fab(oldv - v);
     oldv = v;
     iter = iter + 1;
@@ -330,8 +349,8 @@ __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail
   T vlast;
   T det = sq(b) - (T(4) * a) * c;
   T qr = -(T(0.5) * (b + csign(b) * fab(sqrt(det))));
-  T rt2 = c / qr;
-  if (det < T(0)) rt2 = c / b;
+  T rt2 = quot(c, qr);
+  if (det < T(0)) rt2 = quot(c, b);
   T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
   T v = hllc_vel(a, -b, c, rt2, v0, fail, vlast);
   if (vlast_out) *vlast_out = vlast;
@@ -394,21 +413,22 @@ __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed
   int fail = 0;
   T v = MC ? hllc_contact(a, b, c, seed_ref, mcref, fail, T(1.0e-6), vlast) : hllc_contact(a, b, c, seed_ref, R.bz, fail);
   if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
+  const Quot<T> dR(S2 - v), dL(v - S1), dV(v);
   T pstr = (sx_f + (sq(ex_h) + sq(bx_h))) - (tau_f - erx) * v;
-  T psiR = (S2 * psi_h - v * ex_f) / (S2 - v);
-  T psiL = (-(S1 * psi_h) + v * ex_f) / (v - S1);
-  T phiR = (S2 * phi_h - v * bx_f) / (S2 - v);
-  T phiL = (-(S1 * phi_h) + v * bx_f) / (v - S1);
-  T sxR = (S2 * sx_h - tau_f * v) / (S2 - v);
-  T sxL = (-(S1 * sx_h) + tau_f * v) / (v - S1);
-  T syR = ((((S2 * sy_h - sy_f) - ery * v) - ex_h * ey_h) - bx_h * by_h) / (S2 - v);
-  T syL = ((((-(S1 * sy_h) + sy_f) + ery * v) + ex_h * ey_h) + bx_h * by_h) / (v - S1);
-  T szR = ((((S2 * sz_h - sz_f) - erz * v) - ex_h * ez_h) - bx_h * bz_h) / (S2 - v);
-  T szL = ((((-(S1 * sz_h) + sz_f) + erz * v) + ex_h * ez_h) + bx_h * bz_h) / (v - S1);
-  T tauR = ((sxR - erx) / v - pstr) + e2b2s;
-  T tauL = ((sxL - erx) / v - pstr) + e2b2s;
-  T ddR = (S2 * dd_h - dd_f) / (S2 - v);
-  T ddL = (-(S1 * dd_h) + dd_f) / (v - S1);
+  T psiR = dR(S2 * psi_h - v * ex_f);
+  T psiL = dL(-(S1 * psi_h) + v * ex_f);
+  T phiR = dR(S2 * phi_h - v * bx_f);
+  T phiL = dL(-(S1 * phi_h) + v * bx_f);
+  T sxR = dR(S2 * sx_h - tau_f * v);
+  T sxL = dL(-(S1 * sx_h) + tau_f * v);
+  T syR = dR((((S2 * sy_h - sy_f) - ery * v) - ex_h * ey_h) - bx_h * by_h);
+  T syL = dL((((-(S1 * sy_h) + sy_f) + ery * v) + ex_h * ey_h) + bx_h * by_h);
+  T szR = dR((((S2 * sz_h - sz_f) - erz * v) - ex_h * ez_h) - bx_h * bz_h);
+  T szL = dL((((-(S1 * sz_h) + sz_f) + erz * v) + ex_h * ez_h) + bx_h * bz_h);
+  T tauR = (dV(sxR - erx) - pstr) + e2b2s;
+  T tauL = (dV(sxL - erx) - pstr) + e2b2s;
+  T ddR = dR(S2 * dd_h - dd_f);
+  T ddL = dL(-(S1 * dd_h) + dd_f);
   T psifR, psifL, phifR, phifL, fdR, fdL, ftR, ftL, fsxR, fsxL, fsyR, fsyL, fszR, fszL;
   if (v == T(0)) {
     psifR = psifL = psi_f; phifR = phifL = phi_f;
@@ -497,21 +517,22 @@ __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed
   int fail = 0;
   T v = MC ? hllc_contact(a, b, c, seed_ref, mcref, fail, T(1.0e-6)) : hllc_contact(a, b, c, seed_ref, -b, fail);
   if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
+  const Quot<T> dR(S2 - v), dL(v - S1), dV(v);
   T pstr = (sy_f + (sq(ey_h) + sq(by_h))) - (tau_f - ery) * v;
-  T psiR = (S2 * psi_h - v * ey_f) / (S2 - v);
-  T psiL = (-(S1 * psi_h) + v * ey_f) / (v - S1);
-  T phiR = (S2 * phi_h - v * by_f) / (S2 - v);
-  T phiL = (-(S1 * phi_h) + v * by_f) / (v - S1);
-  T syR = (S2 * sy_h - tau_f * v) / (S2 - v);
-  T syL = (-(S1 * sy_h) + tau_f * v) / (v - S1);
-  T szR = ((((S2 * sz_h - sz_f) - erz * v) - ey_h * ez_h) - by_h * bz_h) / (S2 - v);
-  T szL = ((((-(S1 * sz_h) + sz_f) + erz * v) + ey_h * ez_h) + by_h * bz_h) / (v - S1);
-  T sxR = ((((S2 * sx_h - sx_f) - erx * v) - ey_h * ex_h) - by_h * bx_h) / (S2 - v);
-  T sxL = ((((-(S1 * sx_h) + sx_f) + erx * v) + ey_h * ex_h) + by_h * bx_h) / (v - S1);
-  T tauR = ((syR - ery) / v - pstr) + e2b2s;
-  T tauL = ((syL - ery) / v - pstr) + e2b2s;
-  T ddR = (S2 * dd_h - dd_f) / (S2 - v);
-  T ddL = (-(S1 * dd_h) + dd_f) / (v - S1);
+  T psiR = dR(S2 * psi_h - v * ey_f);
+  T psiL = dL(-(S1 * psi_h) + v * ey_f);
+  T phiR = dR(S2 * phi_h - v * by_f);
+  T phiL = dL(-(S1 * phi_h) + v * by_f);
+  T syR = dR(S2 * sy_h - tau_f * v);
+  T syL = dL(-(S1 * sy_h) + tau_f * v);
+  T szR = dR((((S2 * sz_h - sz_f) - erz * v) - ey_h * ez_h) - by_h * bz_h);
+  T szL = dL((((-(S1 * sz_h) + sz_f) + erz * v) + ey_h * ez_h) + by_h * bz_h);
+  T sxR = dR((((S2 * sx_h - sx_f) - erx * v) - ey_h * ex_h) - by_h * bx_h);
+  T sxL = dL((((-(S1 * sx_h) + sx_f) + erx * v) + ey_h * ex_h) + by_h * bx_h);
+  T tauR = (dV(syR - ery) - pstr) + e2b2s;
+  T tauL = (dV(syL - ery) - pstr) + e2b2s;
+  T ddR = dR(S2 * dd_h - dd_f);
+  T ddL = dL(-(S1 * dd_h) + dd_f);
   T psifR, psifL, phifR, phifL, fdR, fdL, ftR, ftL, fsxR, fsxL, fsyR, fsyL, fszR, fszL;
   if (v == T(0)) {
     psifR = psifL = psi_f; phifR = phifL = phi_f;
