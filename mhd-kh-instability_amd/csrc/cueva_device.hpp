@@ -84,6 +84,9 @@ template <> struct Quot<float> {
 template <typename T> __device__ inline T quot(T num, T den) { return Quot<T>(den)(num); }
 __device__ inline double inv_sqrt(double x) { return 1.0 / sqrt(x); }
 __device__ inline float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+// square root of the face solves: IEEE in FP64, v_sqrt_f32 (within 1 ulp) in FP32
+__device__ inline double face_sqrt(double x) { return sqrt(x); }
+__device__ inline float face_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // ---------------------------------------------------------------------------
 // MP5 / MP7 / MP9 interface values (mp5.f95:40-264, mp7.f95, mp9.f95)
@@ -348,7 +351,7 @@ __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail
                                  T* vlast_out = nullptr) {
   T vlast;
   T det = sq(b) - (T(4) * a) * c;
-  T qr = -(T(0.5) * (b + csign(b) * fab(sqrt(det))));
+  T qr = -(T(0.5) * (b + csign(b) * fab(face_sqrt(det))));
   T rt2 = quot(c, qr);
   if (det < T(0)) rt2 = quot(c, b);
   T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
@@ -583,13 +586,13 @@ __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed
 // wave estimates of one side, 16_hlle_flow_mpx.f95:2715-2740: the root
 // holds -vn*(1-cs^2), the first power of the normal speed
 template <typename T> __device__ inline void hlle_lambda(T gam, const Side<T>& o, T vn, T& lm, T& lp) {
-  T cs = sqrt((gam * o.p) / o.h);
+  T cs = face_sqrt(quot(gam * o.p, o.h));
   T cs2 = sq(cs);
   T v2 = (sq(o.vx) + sq(o.vy)) + sq(o.vz);
-  T root = sqrt((T(1) - v2) * ((T(1) - v2 * cs2) - vn * (T(1) - cs2)));
-  T den = T(1) - cs2 * v2;
-  lp = (vn * (T(1) - cs2) + cs * root) / den;
-  lm = (vn * (T(1) - cs2) - cs * root) / den;
+  T root = face_sqrt((T(1) - v2) * ((T(1) - v2 * cs2) - vn * (T(1) - cs2)));
+  const Quot<T> den(T(1) - cs2 * v2);
+  lp = den(vn * (T(1) - cs2) + cs * root);
+  lm = den(vn * (T(1) - cs2) - cs * root);
 }
 
 // min/max over the sides and the clamp to |s| < 1 (:2789-2818); 9 = NaN stop
@@ -610,10 +613,10 @@ __device__ inline int hlle_speeds(T gam, const Side<T>& L, const Side<T>& R, T& 
 }
 
 template <typename T> __device__ inline T hlle_state(T a, T b, T ul, T ur, T fl, T fr) {
-  return (((b * ur - a * ul) + fl) - fr) / (b - a);
+  return quot(((b * ur - a * ul) + fl) - fr, b - a);
 }
 template <typename T> __device__ inline T hlle_flux(T a, T b, T fl, T fr, T ul, T ur) {
-  return ((b * fl - a * fr) + (a * b) * (ur - ul)) / (b - a);
+  return quot((b * fl - a * fr) + (a * b) * (ur - ul), b - a);
 }
 
 // (E x B)_y of the y face's HLL states: the first Newton point of the x
@@ -696,9 +699,9 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
   T v;
   {
     T det = sq(cb) - (T(4) * ca) * cc;
-    T qr = -(T(0.5) * (cb + csign(cb) * fab(sqrt(det))));
-    T rt2 = cc / qr;
-    if (det < T(0)) rt2 = cc / cb;
+    T qr = -(T(0.5) * (cb + csign(cb) * fab(face_sqrt(det))));
+    T rt2 = quot(cc, qr);
+    if (det < T(0)) rt2 = quot(cc, cb);
     T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
     v = hllc_vel(ca, -cb, cc, rt2, v0, fail, vlast);
     if (fab(v) <= T(1.0e-9))
@@ -707,18 +710,19 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
       v = clampf(v, T(-1) + T(1.0e-6), T(1) - T(1.0e-6));
   }
   if (fail) return 7;  // a NaN v* goes on to the branches on v*, which stop (8)
+  const Quot<T> qR(sb - v), qL(v - sa), qV(v);
   T pstr = (s_f[n] + (sq(e[n]) + sq(b[n]))) - (tau_f - er[n]) * v;
   T sR[3], sL[3];
-  sR[n] = (sb * s_h[n] - tau_f * v) / (sb - v);
-  sL[n] = (-(sa * s_h[n]) + tau_f * v) / (v - sa);
-  sR[t1] = ((((sb * s_h[t1] - s_f[t1]) - er[t1] * v) - e[n] * e[t1]) - b[n] * b[t1]) / (sb - v);
-  sL[t1] = ((((-(sa * s_h[t1]) + s_f[t1]) + er[t1] * v) + e[n] * e[t1]) + b[n] * b[t1]) / (v - sa);
-  sR[t2] = ((((sb * s_h[t2] - s_f[t2]) - er[t2] * v) - e[n] * e[t2]) - b[n] * b[t2]) / (sb - v);
-  sL[t2] = ((((-(sa * s_h[t2]) + s_f[t2]) + er[t2] * v) + e[n] * e[t2]) + b[n] * b[t2]) / (v - sa);
-  T tauR = ((sR[n] - er[n]) / v - pstr) + e2b2s;
-  T tauL = ((sL[n] - er[n]) / v - pstr) + e2b2s;
-  T ddR = (sb * dd_h - dd_f) / (sb - v);
-  T ddL = (-(sa * dd_h) + dd_f) / (v - sa);
+  sR[n] = qR(sb * s_h[n] - tau_f * v);
+  sL[n] = qL(-(sa * s_h[n]) + tau_f * v);
+  sR[t1] = qR((((sb * s_h[t1] - s_f[t1]) - er[t1] * v) - e[n] * e[t1]) - b[n] * b[t1]);
+  sL[t1] = qL((((-(sa * s_h[t1]) + s_f[t1]) + er[t1] * v) + e[n] * e[t1]) + b[n] * b[t1]);
+  sR[t2] = qR((((sb * s_h[t2] - s_f[t2]) - er[t2] * v) - e[n] * e[t2]) - b[n] * b[t2]);
+  sL[t2] = qL((((-(sa * s_h[t2]) + s_f[t2]) + er[t2] * v) + e[n] * e[t2]) + b[n] * b[t2]);
+  T tauR = (qV(sR[n] - er[n]) - pstr) + e2b2s;
+  T tauL = (qV(sL[n] - er[n]) - pstr) + e2b2s;
+  T ddR = qR(sb * dd_h - dd_f);
+  T ddL = qL(-(sa * dd_h) + dd_f);
   // flux slots J, FD, Ftau, FSx, FSy, FSz are contiguous in both enums
   T* o = f + (DIR == 0 ? FX_J : FY_J);
   if (sa >= T(0)) {
@@ -726,21 +730,21 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
   } else if (v == T(0)) {  // L branch with the HLL fluxes
     o[0] = q_f; o[1] = dd_f; o[2] = tau_f; o[3] = s_f[0]; o[4] = s_f[1]; o[5] = s_f[2];
   } else {
-    T den = sb - sa;
+    const Quot<T> den(sb - sa);
     if (sa <= T(0) && T(0) <= v) {
       T cl = sa * (sb - v);
       o[0] = q_f;
-      o[1] = dd_f - (cl * (ddR - ddL)) / den;
-      o[2] = tau_f - (cl * (tauR - tauL)) / den;
+      o[1] = dd_f - den(cl * (ddR - ddL));
+      o[2] = tau_f - den(cl * (tauR - tauL));
 #pragma unroll
-      for (int k = 0; k < 3; k++) o[3 + k] = s_f[k] - (cl * (sR[k] - sL[k])) / den;
+      for (int k = 0; k < 3; k++) o[3 + k] = s_f[k] - den(cl * (sR[k] - sL[k]));
     } else if (v <= T(0) && T(0) <= sb) {
       T cr = sb * (v - sa);
       o[0] = q_f;
-      o[1] = dd_f + (cr * (ddR - ddL)) / den;
-      o[2] = tau_f + (cr * (tauR - tauL)) / den;
+      o[1] = dd_f + den(cr * (ddR - ddL));
+      o[2] = tau_f + den(cr * (tauR - tauL));
 #pragma unroll
-      for (int k = 0; k < 3; k++) o[3 + k] = s_f[k] + (cr * (sR[k] - sL[k])) / den;
+      for (int k = 0; k < 3; k++) o[3 + k] = s_f[k] + den(cr * (sR[k] - sL[k]));
     } else if (sb < T(0)) {
       o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = T(0);
     } else {
@@ -819,9 +823,9 @@ __device__ inline int hlle_mcl_face(T gam, Side<T>& L, Side<T>& R, T* f, bool se
   T v;
   {
     T det = sq(cb) - (T(4) * ca) * cc;
-    T qr = -(T(0.5) * (cb + csign(cb) * fab(sqrt(det))));
-    T rt2 = cc / qr;
-    if (det < T(0)) rt2 = cc / cb;
+    T qr = -(T(0.5) * (cb + csign(cb) * fab(face_sqrt(det))));
+    T rt2 = quot(cc, qr);
+    if (det < T(0)) rt2 = quot(cc, cb);
     T v0 = seed_ref ? vref : clampf(rt2, T(-1), T(1));
     v = hllc_vel(ca, -cb, cc, rt2, v0, fail, vlast, 200);
     if (fab(v) <= T(1.0e-9))
