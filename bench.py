@@ -373,6 +373,16 @@ def main():
             # stays on chip), so their compulsory bytes are the same 728 B
             e["roofline"], e["valu_roofline"] = rooflines(f)
         out["forms"][f] = e
+    if "mixed" in res and "mixed_learned" in res:
+        # what the learned start gains over the method without it (the
+        # Cardano recovery of the same mixed form): its recovery pass time
+        c_ms = res["mixed"]["kernel_ms_per_step"]["final"]
+        l_ms = res["mixed_learned"]["kernel_ms_per_step"]["final"]
+        out["forms"]["mixed_learned"]["vs_cardano_recovery"] = {
+            "recovery_pass_ms": l_ms, "cardano_recovery_pass_ms": c_ms, "time_ratio": l_ms / c_ms,
+            "fallbacks": res["mixed_learned"]["c2p_fallbacks"],
+            "note": "the recovery pass is bound by its HBM traffic, which both share: the learned start "
+                    "replaces the Cardano solve for the cells it is taken for and saves no time"}
     if not args.no_accuracy and len(forms) > 1:
         for f in forms[1:]:
             log("accuracy: %d steps of %s against plain FP64" % (ACCURACY_STEPS, f))
