@@ -24,6 +24,7 @@
 
 #include <math.h>
 #include <quadmath.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -218,6 +219,17 @@ int oracle_scheme_tables(oracle_cfg *c) {
 }
 
 static double *plane(const oracle_state *s) { return (double *)calloc(s->n, sizeof(double)); }
+
+/* scratch of the line sweeps: the oracle stops the process rather than go
+ * on without it */
+static double *scratch_doubles(size_t n) {
+  double *a = (double *)malloc(sizeof(double) * n);
+  if (!a) {
+    fprintf(stderr, "oracle: out of memory (%zu doubles)\n", n);
+    abort();
+  }
+  return a;
+}
 
 oracle_state *oracle_create(const oracle_cfg *c) {
   oracle_state *s = (oracle_state *)calloc(1, sizeof(*s));
@@ -589,8 +601,8 @@ static inline double vor_minus(int order, const double *u, int i) {
 void oracle_mp_line(int order, const double *u, int n, double *up, double *um, int positivity) {
   const double b2 = 4.0 / 3.0, epsm = 1.0e-10, alpha = 2.0;
   const int ext = order == 5 ? 3 : (order == 7 ? 4 : 5); /* du/d2 range mp5.f95:64, mp7.f95:69, mp9.f95:72 */
-  double *dub = (double *)malloc(sizeof(double) * (size_t)(n + 32));
-  double *d2b = (double *)malloc(sizeof(double) * (size_t)(n + 32));
+  double *dub = scratch_doubles((size_t)(n + 32));
+  double *d2b = scratch_doubles((size_t)(n + 32));
   double *du = dub + 8, *d2 = d2b + 8;
   for (int i = -ext; i <= n + ext; i++) {
     du[i] = u[i] - u[i - 1];
@@ -691,9 +703,9 @@ static double slot_y(const oracle_state *s, int slot, size_t k) {
 static void reconstruct(oracle_state *s) {
   const int im = s->c.imax, jm = s->c.jmax, ord = s->c.rec_order;
   const int nmax = (im > jm ? im : jm) + 2 * G + 1;
-  double *ub = (double *)malloc(sizeof(double) * nmax);
-  double *upb = (double *)malloc(sizeof(double) * nmax);
-  double *umb = (double *)malloc(sizeof(double) * nmax);
+  double *ub = scratch_doubles((size_t)nmax);
+  double *upb = scratch_doubles((size_t)nmax);
+  double *umb = scratch_doubles((size_t)nmax);
   double *u = ub + G, *up = upb + G, *um = umb + G;
   /* x sweep, rows -1..jmax (the row jmax+1 of the reference is never read) */
   for (int j = -1; j <= jm; j++)
