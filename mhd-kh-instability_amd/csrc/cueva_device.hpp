@@ -364,6 +364,20 @@ __device__ inline T hllc_contact(T a, T b, T c, bool seed_ref, T vref, int& fail
   return v;
 }
 
+// The star total energy's (S*_n - (E x B)*_n) / v* (16_hllc_flow_mpx.f95:
+// 3536-3574): FP64 divides by v* as the reference does.  FP32 (mixed form)
+// uses (S*_n - (E x B)*_n) / v* = (S2 c/v* - a)/(S2 - v*) (right state) or
+// (-S1 c/v* + a)/(v* - S1) (left), with c/v* = -(a v* + b) at the root of
+// the contact quadratic a v^2 + b v + c = 0 (c = S_n - (E x B)_n): where the
+// normal flow vanishes c and v* are both rounding noise in FP32 and their
+// quotient is not.  Only for |v*| < 1e-3: there the division amplifies
+// the FP32 rounding of c by 1/|v*| > 1000, while the root form carries the
+// FP32 convergence error of Newton's v* times a, which is the larger error
+// at moderate |v*|.
+template <typename T> __device__ inline bool star_tau_from_root(T v) {
+  return sizeof(T) == 4 && fab(v) < T(1.0e-3);
+}
+
 // HLLC x face, 16_hllc_flow_mpx.f95:2811-4295.  Returns 0 or a status.
 // seed_ref: first Newton point as the flang -O2 reference build (R.bz).
 // MC: the MC-limited solver (16_hllc_flow_mcl.f95:2480-4140): its own "_m"
@@ -430,6 +444,11 @@ __device__ inline int hllc_x(const Side<T>& L, const Side<T>& R, T* f, bool seed
   T szL = dL((((-(S1 * sz_h) + sz_f) + erz * v) + ex_h * ez_h) + bx_h * bz_h);
   T tauR = (dV(sxR - erx) - pstr) + e2b2s;
   T tauL = (dV(sxL - erx) - pstr) + e2b2s;
+  if (star_tau_from_root(v)) {
+    const T cv = -(a * v + b);
+    tauR = (dR(S2 * cv - a) - pstr) + e2b2s;
+    tauL = (dL(-(S1 * cv) + a) - pstr) + e2b2s;
+  }
   T ddR = dR(S2 * dd_h - dd_f);
   T ddL = dL(-(S1 * dd_h) + dd_f);
   T psifR, psifL, phifR, phifL, fdR, fdL, ftR, ftL, fsxR, fsxL, fsyR, fsyL, fszR, fszL;
@@ -534,6 +553,11 @@ __device__ inline int hllc_y(const Side<T>& L, const Side<T>& R, T* f, bool seed
   T sxL = dL((((-(S1 * sx_h) + sx_f) + erx * v) + ey_h * ex_h) + by_h * bx_h);
   T tauR = (dV(syR - ery) - pstr) + e2b2s;
   T tauL = (dV(syL - ery) - pstr) + e2b2s;
+  if (star_tau_from_root(v)) {
+    const T cv = -(a * v + b);
+    tauR = (dR(S2 * cv - a) - pstr) + e2b2s;
+    tauL = (dL(-(S1 * cv) + a) - pstr) + e2b2s;
+  }
   T ddR = dR(S2 * dd_h - dd_f);
   T ddL = dL(-(S1 * dd_h) + dd_f);
   T psifR, psifL, phifR, phifL, fdR, fdL, ftR, ftL, fsxR, fsxL, fsyR, fsyL, fszR, fszL;
@@ -721,6 +745,11 @@ __device__ inline int hlle_face(T gam, const Side<T>& L, const Side<T>& R, T* f,
   sL[t2] = qL((((-(sa * s_h[t2]) + s_f[t2]) + er[t2] * v) + e[n] * e[t2]) + b[n] * b[t2]);
   T tauR = (qV(sR[n] - er[n]) - pstr) + e2b2s;
   T tauL = (qV(sL[n] - er[n]) - pstr) + e2b2s;
+  if (star_tau_from_root(v)) {  // the same with the wave speeds sa, sb
+    const T cv = -(ca * v + cb);
+    tauR = (qR(sb * cv - ca) - pstr) + e2b2s;
+    tauL = (qL(-(sa * cv) + ca) - pstr) + e2b2s;
+  }
   T ddR = qR(sb * dd_h - dd_f);
   T ddL = qL(-(sa * dd_h) + dd_f);
   // flux slots J, FD, Ftau, FSx, FSy, FSz are contiguous in both enums

@@ -34,9 +34,10 @@ SCHEMES = {
                                  Ey=7e-2, Ez=2.4e-3, q=6e-2, rho=3e-6, p=3.3e-6, D=3.5e-6, tau=5.2e-6, Sx=1.6e-3,
                                  Sy=9e-6, Sz=3e-5),
                         abs=dict(phi=1e-9)),
-    # measured at step 10: Bx 2.8e-3, By 3.9e-7, vx 4.6e-4, vy 9.1e-7, Ez 7.3e-4, rho 1.3e-6,
-    # p 1.2e-6, D 1.3e-6, tau 1.1e-6, Sx 4.8e-4, Sy 1.5e-6; psi Bz vz Ex Ey q Sz exactly 0;
-    # mean |phi diff| 1.9e-10
+    # measured at step 10: Bx 2.8e-3, By 4.1e-7, vx 4.6e-4, vy 8.2e-7, Ez 7.4e-4, rho 1.0e-6,
+    # p 1.2e-6, D 9.7e-7, tau 1.0e-6, Sx 4.7e-4, Sy 1.4e-6; psi Bz vz Ex Ey q Sz exactly 0;
+    # mean |phi diff| 1.9e-10 (the bounds below are those of an earlier FP32 build, 4x its
+    # measurements: Bx 2.8e-3 ... rho 1.3e-6, Sy 1.5e-6)
     "hllc_mp5": dict(fs=cueva.HLLC, rec=0, bz0=0.0,
                      rel=dict(psi=0.0, Bx=1.2e-2, By=1.6e-6, Bz=0.0, Vx=1.9e-3, Vy=3.7e-6, Vz=0.0, Ex=0.0, Ey=0.0,
                               Ez=3e-3, q=0.0, rho=5.2e-6, p=4.8e-6, D=5.2e-6, tau=4.4e-6, Sx=2e-3, Sy=6e-6, Sz=0.0),
