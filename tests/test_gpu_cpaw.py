@@ -15,9 +15,12 @@ size.  With HLLC it gives FP64 HLLC's errors to four digits at 16^2 and
 32^2: the wave's normal velocity is zero, so the contact speed v* is FP32
 rounding noise, and the FP32 star states take (S* - (E x B)*)_n / v* through
 the contact quadratic's root instead of dividing noise by noise (without
-that, v_z and E_y were wrong by O(1) at 32^2).  At 64^2 the FP32 rounding
-still takes a cell across the recovery's det < 0 boundary, the one the
-reference's own FP64 recovery meets at 128^2; FP64 HLLC converges like HLL.
+that, v_z and E_y were wrong by O(1) at 32^2).  At 64^2 the normal
+velocity, zero in the exact wave, grows in the mixed form from its FP32
+rounding with an e-folding time of about 8 steps from step 100 on (v_x 2e-3
+at step 100, 0.37 at step 140 against FP64), until the recovery stops at
+step 153 (det < 0): a growing mode seeded by rounding about 1e9 times larger
+than FP64's, not checked here.  FP64 HLLC converges like HLL.
 """
 import sys
 
