@@ -276,8 +276,12 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
   {
     const char* e = getenv("CUEVA_MARCH");
     c->march = e ? atoi(e) != 0 : 1;
+    // resident-block rounds a marching launch is cut into: measured best at
+    // 4 for the FP64 kernels (2 blocks per CU; 1.5-3.6 % faster than 2 at
+    // 2048x4096 and 4096x8192) and 2 for the FP32 ones (4 blocks per CU):
+    // about 2000 blocks of ~67-row segments at 2048x4096 either way
     e = getenv("CUEVA_MARCH_ROUNDS");
-    c->march_rounds = e && atoi(e) > 0 ? atoi(e) : 2;
+    c->march_rounds = e && atoi(e) > 0 ? atoi(e) : (p->precision == CUEVA_MIXED ? 2 : 4);
     e = getenv("CUEVA_MARCH_SEG");
     c->march_rows = e && atoi(e) > 0 ? atoi(e) : 0;
     // default off: inside the last marching launch the recovery's registers
