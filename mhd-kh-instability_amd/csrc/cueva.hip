@@ -56,6 +56,7 @@ struct cueva_ctx {
   int rec_order;
   int march;         // fused step uses the marching face kernel (default; CUEVA_MARCH=0: face planes)
   int march_rounds;  // resident-block rounds a marching launch is cut into (CUEVA_MARCH_ROUNDS)
+  int march_xcd;     // marching blocks in XCD-aware tile order (CUEVA_MARCH_XCD, default on)
   int march_rows;    // rows per segment when set (CUEVA_MARCH_SEG; rounded up to 4m - 1)
   int c2p_in_march;  // Cardano recovery inside the last marching launch (CUEVA_C2P_IN_MARCH)
   int march_seg[3];  // rows per segment of the stage / last-stage / last-stage + recovery kernel (0: not sized yet)
@@ -282,6 +283,11 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     // about 2000 blocks of ~67-row segments at 2048x4096 either way
     e = getenv("CUEVA_MARCH_ROUNDS");
     c->march_rounds = e && atoi(e) > 0 ? atoi(e) : (p->precision == CUEVA_MIXED ? 2 : 4);
+    // the hardware deals workgroups to the 8 XCDs round robin; in the XCD-aware
+    // order each XCD marches a contiguous band of tiles, whose neighbours
+    // share halo columns and rows in its L2 (0.2 % FP64, 0.6 % FP32 per step)
+    e = getenv("CUEVA_MARCH_XCD");
+    c->march_xcd = e ? atoi(e) != 0 : 1;
     e = getenv("CUEVA_MARCH_SEG");
     c->march_rows = e && atoi(e) > 0 ? atoi(e) : 0;
     // default off: inside the last marching launch the recovery's registers
@@ -905,8 +911,13 @@ static int launch_march(cueva_ctx* c, int l, bool fin, bool c2p = false, bool st
   const void* fn = march_kernel_of(c, fin, c2p);
   int seg = march_seg(c, fin, c2p);
   if (!fn || seg <= 0) return CUEVA_E_HIP;
-  const dim3 grid((unsigned)march_strips(c->g), (unsigned)((c->g.jmax + 1 + seg - 1) / seg));
+  const int strips = march_strips(c->g), segs = (c->g.jmax + 1 + seg - 1) / seg;
+  const int xcd = c->march_xcd;
+  const dim3 grid = xcd ? dim3((unsigned)(8 * ((strips * segs + 7) / 8))) : dim3((unsigned)strips, (unsigned)segs);
   MarchArgs ma;
+  ma.strips = strips;
+  ma.segs = segs;
+  ma.xcd = xcd;
   ma.P = c->P;
   ma.g = c->g;
   ma.S = c->S;

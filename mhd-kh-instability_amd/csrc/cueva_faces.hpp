@@ -579,6 +579,8 @@ struct MarchArgs {
   int l, seg;
   StatusRef st;
   int stage0;  // C2P: also the next step's stage-0 state
+  int strips, segs;  // the tiles of the launch
+  int xcd;           // 1D grid in XCD-aware tile order (xcd_tile), else x = strip, y = segment
 };
 static_assert(offsetof(MarchArgs, P) == 0, "argument block layout");
 
@@ -614,9 +616,11 @@ k_faces_march(MarchArgs args) {
   __shared__ T fyp[FY_N][64];                   // wavefront 0's row below: previous step's top row
   __shared__ T fxb[XLDS ? MW : 1][FX_N][64];    // x-face rows (HLLE)
   if (halted(args.st)) return;  // uniform over the launch: before any barrier
+  int strip = (int)blockIdx.x, segment = (int)blockIdx.y;
+  if (args.xcd && !xcd_tile(args.strips, args.segs, strip, segment)) return;  // uniform over the block
   const int lane = threadIdx.x, w = threadIdx.y;
-  const int i = (int)blockIdx.x * MARCH_COLS - 1 + lane;
-  const int jc0 = (int)blockIdx.y * args.seg;
+  const int i = strip * MARCH_COLS - 1 + lane;
+  const int jc0 = segment * args.seg;
   const int jend = min(jc0 + args.seg - 1, args.g.jmax);
   int b = 0;
   for (int jt = jc0 - 1; jt <= jend; jt += MW, b ^= 1) {  // uniform over the block: the barrier below
