@@ -557,7 +557,8 @@ __global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Sch
 // One barrier per tile step: the y rows alternate between two LDS buffers,
 // and wavefront 0 copies the top row it needs next into its own row after
 // the barrier; a buffer is rewritten two tile steps later, after every
-// reader has passed the barrier in between.
+// reader has passed the barrier in between.  HLLE keeps one buffer and
+// takes a second barrier at the end of each tile step instead.
 //   FINAL false: the stage right-hand side -> rhs[l] (17_fluxandsource.f95),
 //                and the next stage's state (sumint, varint, electricvarint,
 //                flowinit) -> sn, interior points;
@@ -584,13 +585,29 @@ struct MarchArgs {
 };
 static_assert(offsetof(MarchArgs, P) == 0, "argument block layout");
 
-// Waves per SIMD the marching kernel is built for: in FP32 4 for HLLC (3-5
-// VGPRs spill at that bound), 2 otherwise (HLL would spill heavily, HLLE's
-// LDS allows 3 at most); in FP64 no bound (2 where the registers allow).
+// Waves per SIMD the marching kernel is built for, a register bound (the
+// compiler spills past it).  Measured per step at 2048x4096:
+//   FP64: 2 for every solver.  HLLC fits 237 VGPRs; the others need a few
+//     more than 256 and ran 1 wave per SIMD unbounded: bounded, HLL spills
+//     2-12 VGPRs and runs 36 % faster, HLLE (one y-row buffer, below) 36 %,
+//     Lax-Friedrichs 32 %, the MCL solvers (63-114 spilled) 17-20 %;
+//   FP32: 4 for HLLC (2 VGPRs spill) and HLLE (8-22 spill; 3 %), 3 for the
+//     others: the MCL solvers' 178-192 VGPRs ran 2 waves per SIMD unbounded
+//     and 9-19 % slower; HLL and Lax-Friedrichs fit 3 unbounded, and HLL at
+//     4 spills 25-56 VGPRs and ran 19 % slower.
 template <typename T, int SOLVER>
 constexpr int march_min_waves() {
-  return sizeof(T) == 8 ? (CUEVA_MARCH_F64_WAVES > 0 ? CUEVA_MARCH_F64_WAVES : 1)
-                        : (CUEVA_MARCH_F32_WAVES > 0 ? CUEVA_MARCH_F32_WAVES : (SOLVER == 7 ? 4 : 2));
+  if constexpr (sizeof(T) == 8) return CUEVA_MARCH_F64_WAVES > 0 ? CUEVA_MARCH_F64_WAVES : 2;
+  else return CUEVA_MARCH_F32_WAVES > 0 ? CUEVA_MARCH_F32_WAVES : (SOLVER == 7 || SOLVER == 6 ? 4 : 3);
+}
+
+// y-face row buffers: 2 (one barrier per tile step), or 1 for HLLE (a second
+// barrier at the end of the tile step, before the rows are rewritten), whose
+// x rows also pass through LDS: 63 KB per FP64 block instead of 91 KB, so
+// that two blocks fit a CU's 160 KB
+template <int SOLVER>
+constexpr int march_y_buffers() {
+  return SOLVER == 6 ? 1 : 2;
 }
 
 // Wavefronts per block = face rows per tile step.  4 everywhere by default;
@@ -612,7 +629,8 @@ __global__ void __launch_bounds__(64 * march_block_waves(sizeof(T) == 4, SOLVER)
 k_faces_march(MarchArgs args) {
   constexpr int MW = march_block_waves(sizeof(T) == 4, SOLVER);
   constexpr bool XLDS = SOLVER == 6;            // x solved first, its fluxes through LDS
-  __shared__ T fyb[2][MW][FY_N][64];            // y-face rows of this and the previous tile step
+  constexpr int YB = march_y_buffers<SOLVER>();
+  __shared__ T fyb[YB][MW][FY_N][64];           // y-face rows of this (and the previous) tile step
   __shared__ T fyp[FY_N][64];                   // wavefront 0's row below: previous step's top row
   __shared__ T fxb[XLDS ? MW : 1][FX_N][64];    // x-face rows (HLLE)
   if (halted(args.st)) return;  // uniform over the launch: before any barrier
@@ -623,13 +641,13 @@ k_faces_march(MarchArgs args) {
   const int jc0 = segment * args.seg;
   const int jend = min(jc0 + args.seg - 1, args.g.jmax);
   int b = 0;
-  for (int jt = jc0 - 1; jt <= jend; jt += MW, b ^= 1) {  // uniform over the block: the barrier below
+  for (int jt = jc0 - 1; jt <= jend; jt += MW, b ^= YB - 1) {  // uniform over the block: the barriers below
     // The arguments (plane pointers, geometry, scheme constants) are read
     // again from the kernel's argument block every tile step rather than
     // held in scalar registers across the loop, where they do not fit and
     // spill into vector-register lanes (FP64 HLLC: 156 SGPRs spilled, none
     // this way; FP32 HLLC: 147 VGPRs against 139).  FP64 HLL keeps them:
-    // re-reading costs it a wave per SIMD.
+    // re-reading measured 24 % slower there.
     constexpr bool REREAD = sizeof(T) == 4 || SOLVER != 5;
     const __attribute__((address_space(4))) MarchArgs* ap =
         (const __attribute__((address_space(4))) MarchArgs*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -708,7 +726,8 @@ k_faces_march(MarchArgs args) {
 #pragma unroll
       for (int a = 0; a < FY_N; a++) fyp[a][lane] = fyb[b][MW - 1][a][lane];
     }
-    if constexpr (XLDS) __builtin_amdgcn_wave_barrier();  // this step's fxb reads before the next step's writes
+    if constexpr (YB == 1) __syncthreads();  // every read of this step's rows before the next step's writes
+    else if constexpr (XLDS) __builtin_amdgcn_wave_barrier();  // this step's fxb reads before the next step's writes
   }
 }
 

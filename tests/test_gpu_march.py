@@ -53,13 +53,15 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run(p, prims, steps, march, seg=None):
-    old = {k: os.environ.get(k) for k in ("CUEVA_MARCH", "CUEVA_MARCH_SEG")}
+def run(p, prims, steps, march, seg=None, xcd=None, rounds=None):
+    knobs = {"CUEVA_MARCH_SEG": seg, "CUEVA_MARCH_XCD": xcd, "CUEVA_MARCH_ROUNDS": rounds}
+    old = {k: os.environ.get(k) for k in ("CUEVA_MARCH", *knobs)}
     os.environ["CUEVA_MARCH"] = "1" if march else "0"
-    if seg is not None:
-        os.environ["CUEVA_MARCH_SEG"] = str(seg)
-    else:
-        os.environ.pop("CUEVA_MARCH_SEG", None)
+    for k, v in knobs.items():  # read when the context is created
+        if v is not None:
+            os.environ[k] = str(v)
+        else:
+            os.environ.pop(k, None)
     try:
         with cueva.Solver(p) as s:
             s.upload(prims)
@@ -89,6 +91,26 @@ def test_march_equals_face_planes_fp64(name, shape):
         got, cgot = run(p, prims, steps, march=True, seg=seg)
         nd = ndiff(got, ref)
         say("  segment rows", seg, "differing points", sum(nd.values()))
+        assert cgot == cref
+        assert np.array_equal(got, ref), nd
+
+
+@pytest.mark.parametrize("name", ["hllc_mp5", "hlle_mp7_bz", "hll_mp9_ssp3", "lax_bz"])
+@pytest.mark.parametrize("precision", [cueva.FP64, cueva.MIXED])
+def test_block_order_and_rounds_do_not_change_bits(name, precision):
+    """The XCD-aware block order (the default), the plain 2D order, and the
+    launch cut into 1, 2, 3 or 4 resident-block rounds march the same tiles
+    with the same arithmetic: bit-identical fields."""
+    fs, rec, order, sigma, kw = CASES[name]
+    p, prims = cueva.kh_problem(197, 150, sigma=sigma, flux_solver=fs, rec_mp=rec, order=order, rotated=True,
+                                precision=precision, crmath=0, **kw)
+    steps = 3 if order == 4 else 4
+    say("case", name, "FP64" if precision == cueva.FP64 else "mixed", "grid %dx%d" % (p.imax, p.jmax))
+    ref, cref = run(p, prims, steps, march=True)
+    for xcd, rounds, seg in ((0, None, None), (0, 1, None), (1, 3, None), (1, 1, 23), (0, 2, 23)):
+        got, cgot = run(p, prims, steps, march=True, seg=seg, xcd=xcd, rounds=rounds)
+        nd = ndiff(got, ref)
+        say("  xcd", xcd, "rounds", rounds, "segment rows", seg, "differing points", sum(nd.values()))
         assert cgot == cref
         assert np.array_equal(got, ref), nd
 
