@@ -508,8 +508,21 @@ __device__ __forceinline__ void face_pair(const Geom& g, const Planes& P, const 
   }
 }
 
+// Waves per SIMD k_faces is built for (a register bound).  Unbounded, the
+// MCL solvers' face pairs need 177-181 VGPRs in FP32 (2 waves per SIMD) and
+// past 256 in FP64 (1 wave): bounded to 3 (FP32, 14-16 VGPRs spilled) the
+// face pass measured 18-19 % faster at 2048x4096, and to 2 (FP64) HLL-MCL
+// 8 % and HLLC-MCL 5 % faster, HLLE-MCL (176 spilled) 3 % slower, so it
+// stays unbounded.  The other solvers fit their waves unbounded.
+template <typename T, int SOLVER>
+constexpr int faces_min_waves() {
+  if constexpr (sizeof(T) == 8) return SOLVER == 2 || SOLVER == 4 ? 2 : 1;
+  else return SOLVER >= 2 && SOLVER <= 4 ? 3 : 1;
+}
+
 template <int ORD, int SOLVER, typename T = double>
-__global__ void __launch_bounds__(256) FACES_WAVES k_faces(Geom g, Planes P, Scheme S, StatusRef st) {
+__global__ void __launch_bounds__(256, (faces_min_waves<T, SOLVER>())) FACES_WAVES
+k_faces(Geom g, Planes P, Scheme S, StatusRef st) {
   if (halted(st)) return;
 #if CUEVA_XCD_FACES
   int bx, by;
