@@ -368,8 +368,11 @@ __device__ inline void final_update(const Geom& g, const Planes& P, const Scheme
 
 // Last stage + sumfin + varfin + electricfield + varprimitivecardano
 // (C2P false: without the primitive recovery, which then runs on its own).
+// Bounded to its 256-thread blocks: under the default 1024-thread bound the
+// FP64 form was held to 128 VGPRs and spilled 112 B per lane (1.70 ms per
+// step at 2048x4096 against 1.66 ms at 150 VGPRs and 3 waves per SIMD).
 template <typename FT, bool C2P = true>
-__global__ void k_final_fused(Geom g, Planes P, Scheme S, StatusRef sr) {
+__global__ void __launch_bounds__(256) k_final_fused(Geom g, Planes P, Scheme S, StatusRef sr) {
   if (halted(sr)) return;
   INTERIOR_IJ
   double nu[A_N];
