@@ -35,10 +35,17 @@ VALU_PEAK_INSTR = 1024 * 2.4e9 / 4
 EQUAL_L1_BAR = 1.0e-4      # relative L1 on rho after 100 steps, the mixed forms' accuracy bar
 METRIC = "zone-updates/s, 2048x4096 KH: plain FP64 vs mixed+learned-c2p at equal L1"
 
-# compulsory bytes of one SSP2(2,2,2) step per zone-update (stage k reads U^n,
-# P^n and earlier stage right-hand sides, writes its own; the last stage
-# writes U^{n+1}, P^{n+1}): 91 doubles
-STEP_BYTES_PER_ZONE = 728
+# compulsory bytes of one step per zone-update: stage k of lmax reads U^n
+# and P^n (19 planes) and the k-1 earlier stage right-hand sides (17 planes
+# each: 14 flux differences, 3 stiff E sources) and writes its own; the last
+# stage writes U^{n+1} and P^{n+1} (19) instead.  SSP2(2,2,2): 91 doubles =
+# 728 B; SSP2(3,3,2)-LUM: 161 = 1288 B; SSP3(4,3,3): 248 = 1984 B.
+
+
+def step_bytes_per_zone(lmax):
+    reads = 19 * lmax + 17 * lmax * (lmax - 1) // 2
+    writes = 17 * (lmax - 1) + 19
+    return 8 * (reads + writes)
 
 
 def log(*a):
@@ -315,16 +322,18 @@ def main():
         res[f].pop("p")
     zones = p.imax * p.jmax
     lmax = order  # stages of the IMEX scheme
+    step_bytes = step_bytes_per_zone(lmax)
 
     def rooflines(form):
         """HBM and VALU rooflines of the form's face launches (its dominant
-        kernel class): compulsory bytes per launch = the step's 728 B per
-        zone-update spread over the lmax face launches of a step; VALU
+        kernel class): compulsory bytes per launch = the step's compulsory
+        bytes per zone-update (728 B for SSP2(2,2,2)) spread over the lmax
+        face launches of a step; VALU
         instructions per launch from the committed rocprofv3 counter summary."""
         fr = res[form]
         kt = fr["kernel_ms_per_step"]
         avg_ms = kt["faces"] / (fr["kernel_launches"]["faces"] / args.steps)
-        alg = STEP_BYTES_PER_ZONE * zones / lmax
+        alg = step_bytes * zones / lmax
         achieved = alg / (avg_ms / 1e3) / 1e9
         prof = os.path.join(ROOT, "profiles", "faces_%s_%s.json" % (cfg, form))
         prof = json.load(open(prof)) if os.path.exists(prof) else {}
@@ -357,9 +366,9 @@ def main():
         "config": {"workload": wl, "form": form_name[head]},
         "roofline": hbm,
         "valu_roofline": valu,
-        "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_ZONE,
-                          "achieved_GBs": r["value"] * STEP_BYTES_PER_ZONE / 1e9,
-                          "frac": r["value"] * STEP_BYTES_PER_ZONE / 1e9 / HBM_PEAK_GBS},
+        "step_roofline": {"bytes_per_zone_update": step_bytes,
+                          "achieved_GBs": r["value"] * step_bytes / 1e9,
+                          "frac": r["value"] * step_bytes / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_per_step": r["kernel_ms_per_step"], "fixups": r["fixups"], "c2p_fallbacks": r["c2p_fallbacks"],
         "host_boundary": r.get("host_boundary"),
         "forms": {},
@@ -370,7 +379,7 @@ def main():
         if f != head:
             e["speedup_vs_plain_fp64"] = res[f]["value"] / r["value"]
             # all forms keep every plane in HBM in FP64 (the FP32 arithmetic
-            # stays on chip), so their compulsory bytes are the same 728 B
+            # stays on chip), so their compulsory bytes are the same
             e["roofline"], e["valu_roofline"] = rooflines(f)
         out["forms"][f] = e
     if "mixed" in res and "mixed_learned" in res:
