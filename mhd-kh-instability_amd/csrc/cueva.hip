@@ -293,8 +293,10 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     e = getenv("CUEVA_MARCH_SEG");
     c->march_rows = e && atoi(e) > 0 ? atoi(e) : 0;
     // default off: inside the last marching launch the recovery's registers
-    // cost the whole launch occupancy, measured 1.3 ms (FP32 faces) and 3.3
-    // ms (FP64 faces) per step at 2048x4096 against 0.58 ms for its own pass
+    // cost the whole launch occupancy or spill (FP32 faces at 2 waves per SIMD
+    // instead of 4; FP64 faces 256 B of scratch per lane): steps measured
+    // 0.83 ms (FP32 faces) and 0.12 ms (FP64 faces) slower at 2048x4096 than
+    // with the recovery in its own 0.55 ms pass
     e = getenv("CUEVA_C2P_IN_MARCH");
     c->c2p_in_march = e ? atoi(e) != 0 : 0;
   }
