@@ -606,8 +606,8 @@ static_assert(offsetof(MarchArgs, P) == 0, "argument block layout");
 //     Lax-Friedrichs 32 %, the MCL solvers (63-114 spilled) 17-20 %;
 //   FP32: 4 for HLLC (2 VGPRs spill) and HLLE (8-22 spill; 3 %), 3 for the
 //     others: the MCL solvers' 178-192 VGPRs ran 2 waves per SIMD unbounded
-//     and 9-19 % slower; HLL and Lax-Friedrichs fit 3 unbounded, and HLL at
-//     4 spills 25-56 VGPRs and ran 19 % slower.
+//     and 9-19 % slower; HLL and Lax-Friedrichs fit 3 unbounded, and at 4
+//     HLL spills 25-56 VGPRs and ran 19 % slower, Lax-Friedrichs 29 and 6 %.
 template <typename T, int SOLVER>
 constexpr int march_min_waves() {
   if constexpr (sizeof(T) == 8) return CUEVA_MARCH_F64_WAVES > 0 ? CUEVA_MARCH_F64_WAVES : 2;
