@@ -286,8 +286,8 @@ extern "C" int cueva_create(const cueva_params* p, cueva_ctx** out) {
     // the hardware deals workgroups to the 8 XCDs round robin; in the XCD-aware
     // order each XCD marches a contiguous band of tiles, whose neighbours
     // share halo columns and rows in its L2: 19-25 % fewer bytes fetched and
-    // FP32 steps 0.4 % faster, FP64 HLL/HLLC steps as fast as in the plain
-    // order, but FP64 HLLE (MP) steps 5 % slower, which keep the plain order
+    // FP32 steps 0.4 % faster, the other FP64 steps within 1 % of the plain
+    // order's, but FP64 HLLE (MP) steps 5 % slower, which keep the plain order
     e = getenv("CUEVA_MARCH_XCD");
     c->march_xcd = e ? atoi(e) != 0 : !(p->precision == CUEVA_FP64 && p->flux_solver == CUEVA_HLLE);
     e = getenv("CUEVA_MARCH_SEG");
