@@ -7,6 +7,9 @@ x 4096 up to 8192 x 16384), where the oracle would take hours.
   of the KH layer of every configuration (MP5 + HLLC at 2048 x 4096 and
   4096 x 8192, MP7 + HLLE with a guide field, MP9 + HLLC with SSP3(4,3,3)
   and v0 = 0.9 at 8192 x 16384), in FP64 and in the mixed form.
+* The same for the other flux solvers at 2048 x 4096 (HLL and HLLE with MP5,
+  Lax-Friedrichs and HLL/HLLE/HLLC with the MC limiter), whose marching and
+  face-plane kernels are each built for their own waves per SIMD.
 * In a doubly periodic box the flux differences telescope: over the cells
   that are not duplicates of the periodic edge, the step's update changes
   the sums of D, tau, S (from the state startconvar forms out of the
@@ -63,6 +66,28 @@ def test_march_equals_face_planes_at_full_size(cfg, precision):
     say("%s (%dx%d), %s, 2 steps: marching path" % (cfg, p.imax, p.jmax, "FP64" if precision == cueva.FP64 else "mixed"))
     a, ca = digests(p, prims, 2, march=True)
     say("%s: face-plane path" % cfg)
+    b, cb = digests(p, prims, 2, march=False)
+    del prims
+    assert ca == cb
+    bad = sorted(f for f in FIELDS if a[f] != b[f])
+    assert not bad, bad
+
+
+# the other flux solvers at 2048 x 4096 (each kernel built for its own register bound)
+SOLVERS = {"hll_mp5": (cueva.HLL, 0.0), "lax_mc": (cueva.LAX, 0.0), "hll_mc": (cueva.HLL_MCL, 0.0),
+           "hlle_mc": (cueva.HLLE_MCL, 0.1), "hllc_mc": (cueva.HLLC_MCL, 0.0), "hlle_mp5": (cueva.HLLE, 0.1)}
+
+
+@pytest.mark.parametrize("name,precision", [("hll_mp5", cueva.FP64), ("lax_mc", cueva.FP64), ("hlle_mc", cueva.FP64),
+                                            ("hllc_mc", cueva.FP64), ("hll_mc", cueva.MIXED), ("lax_mc", cueva.MIXED),
+                                            ("hlle_mp5", cueva.MIXED)])
+def test_march_equals_face_planes_other_solvers(name, precision):
+    fs, bz = SOLVERS[name]
+    p, prims = cueva.kh_problem(NI, NJ, sigma=1.0e6, flux_solver=fs, bz0=bz, precision=precision,
+                                **({} if precision == cueva.FP64 else {"crmath": 0}))
+    say("%s (%dx%d), %s, 2 steps: marching path" % (name, p.imax, p.jmax, "FP64" if precision == cueva.FP64 else "mixed"))
+    a, ca = digests(p, prims, 2, march=True)
+    say("%s: face-plane path" % name)
     b, cb = digests(p, prims, 2, march=False)
     del prims
     assert ca == cb
