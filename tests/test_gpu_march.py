@@ -16,6 +16,9 @@ per-subroutine entry points also use.
 * Mixed form: the FP32 faces are the same code in the same unit, the FP32
   flux differences the same operations, and the FP64 sources and update are
   evaluated without contraction in both units: bit-identical as well.
+* Launch shape: the XCD-aware and the plain 2D block order, and launches
+  cut into 1-4 resident-block rounds, march the same tiles with the same
+  arithmetic: bit-identical fields and counters.
 """
 import os
 import sys
